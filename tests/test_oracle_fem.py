@@ -1,0 +1,316 @@
+"""Pins the numpy oracle (oracle/fem.py) without the reference.
+
+Every check is chosen so that a plausible slip (a dropped term, a wrong sign
+or index, a transposed operand) fails: closed forms, FP64 central finite
+differences of energies and gradients, the reference's volume invariant
+(benches/test_aggregation.py:82,129) and SVD-convention invariance.
+"""
+
+import numpy as np
+import pytest
+
+import oracle.fem as of
+from apple_amd.mesh import cube
+
+RNG = np.random.default_rng(1234)
+
+
+def _random_F(T, scale=0.3, rng=RNG):
+    return np.eye(3) + scale * rng.standard_normal((T, 3, 3))
+
+
+def _rot(rng, T):
+    q = rng.standard_normal((T, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    return np.stack(
+        [
+            np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)], 1),
+            np.stack([2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)], 1),
+            np.stack([2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], 1),
+        ],
+        1,
+    )
+
+
+def _fem(material, mesh, rng=RNG):
+    dhdX, dV = of.region(mesh.points, mesh.cells)
+    T = mesh.n_cells
+    mu = 1e4 * np.exp(0.3 * rng.standard_normal(T))
+    lam = 10.0 * mu if material == of.SNH else None
+    return of.FemPotential(material, mesh.cells, dhdX, dV, mu, lam)
+
+
+# ----------------------------------------------------------------- setup
+
+
+@pytest.mark.parametrize("order", ["morton", "lex"])
+def test_affine_displacement_gives_constant_F(order):
+    m = cube(3, order=order)
+    A = RNG.standard_normal((3, 3)) * 0.2
+    u = m.points @ A.T
+    dhdX, _ = of.region(m.points, m.cells)
+    F = of.deformation_gradient(of.cell_values(u, m.cells), dhdX)
+    np.testing.assert_allclose(F, np.broadcast_to(np.eye(3) + A, F.shape), atol=1e-12)
+
+
+def test_volume_sums_match_mesh_volume():
+    m = cube(4, lo=-0.5, hi=1.5)
+    _, dV = of.region(m.points, m.cells)
+    assert dV.min() > 0
+    np.testing.assert_allclose(dV.sum(), 8.0, rtol=1e-13)
+    # the reference's aggregation invariant: sum of lumped quarter volumes = volume
+    np.testing.assert_allclose(of.lumped_vertex_volume(m.points, m.cells).sum(), 8.0, rtol=1e-13)
+
+
+def test_lame_converter_closed_form():
+    lam, mu = of.lame_converter(1.0, 0.25)
+    assert lam == pytest.approx(0.4) and mu == pytest.approx(0.4)
+    lam, mu = of.lame_converter(3.0, 0.0)
+    assert lam == 0.0 and mu == pytest.approx(1.5)
+
+
+# ----------------------------------------------------------------- SNH
+
+
+def test_snh_rest_state_is_stress_free():
+    F = np.broadcast_to(np.eye(3), (4, 3, 3)).copy()
+    mu = np.full(4, 3.0)
+    lam = np.full(4, 7.0)
+    np.testing.assert_allclose(of.snh_energy_density(F, mu, lam), 0.0, atol=1e-14)
+    np.testing.assert_allclose(of.snh_pk1(F, mu, lam), 0.0, atol=1e-14)
+
+
+def test_snh_uniform_scale_closed_form():
+    s = np.array([0.7, 1.0, 1.3, 2.0])
+    F = s[:, None, None] * np.eye(3)
+    mu, lam = np.full(4, 2.0), np.full(4, 5.0)
+    want = 1.5 * mu * (s**2 - 1) - mu * (s**3 - 1) + 0.5 * lam * (s**3 - 1) ** 2
+    np.testing.assert_allclose(of.snh_energy_density(F, mu, lam), want, rtol=1e-13)
+
+
+def _fd_pk1(energy, F, h=1e-6):
+    P = np.zeros_like(F)
+    for i in range(3):
+        for j in range(3):
+            E = np.zeros_like(F)
+            E[:, i, j] = h
+            P[:, i, j] = (energy(F + E) - energy(F - E)) / (2 * h)
+    return P
+
+
+def test_snh_pk1_is_energy_derivative():
+    F = _random_F(32)
+    mu = np.full(32, 1.3)
+    lam = np.full(32, 4.1)
+    P = of.snh_pk1(F, mu, lam)
+    Pfd = _fd_pk1(lambda G: of.snh_energy_density(G, mu, lam), F)
+    np.testing.assert_allclose(P, Pfd, rtol=1e-7, atol=1e-7)
+
+
+# ----------------------------------------------------------------- ARAP
+
+
+def test_svd_rv_convention():
+    F = _random_F(64, 0.8)
+    F[:8] *= -1.0  # inverted
+    U, s, V = of.svd_rv(F)
+    np.testing.assert_allclose(np.linalg.det(U), 1.0, atol=1e-12)
+    np.testing.assert_allclose(np.linalg.det(V), 1.0, atol=1e-12)
+    np.testing.assert_allclose(U @ (s[:, :, None] * np.swapaxes(V, 1, 2)), F, atol=1e-12)
+    assert np.all(np.abs(s[:, 0]) >= np.abs(s[:, 1])) and np.all(np.abs(s[:, 1]) >= np.abs(s[:, 2]))
+    assert np.all(s[:, :2] >= 0)
+    np.testing.assert_array_equal(np.sign(s[:, 2]), np.sign(np.linalg.det(F)))
+
+
+def test_arap_closed_forms():
+    R = _rot(RNG, 16)
+    mu = np.full(16, 2.5)
+    np.testing.assert_allclose(of.arap_energy_density(R, mu), 0.0, atol=1e-13)
+    s = np.linspace(0.5, 2.0, 16)
+    np.testing.assert_allclose(
+        of.arap_energy_density(s[:, None, None] * R, mu), 1.5 * mu * (s - 1) ** 2, rtol=1e-11, atol=1e-14
+    )
+    Rp, _ = of.polar_rv(s[:, None, None] * R)
+    np.testing.assert_allclose(Rp, R, atol=1e-12)
+
+
+def test_arap_pk1_is_energy_derivative():
+    F = _random_F(32, 0.4)
+    mu = np.full(32, 1.7)
+    P = of.arap_pk1(F, mu)
+    Pfd = _fd_pk1(lambda G: of.arap_energy_density(G, mu), F)
+    np.testing.assert_allclose(P, Pfd, rtol=1e-6, atol=1e-7)
+
+
+def _fd_element_hessian(pot, u, t, h=1e-5):
+    """Exact 12x12 Hessian of tet t by central differences of its gradient."""
+    cells = pot.cells[t]
+    H = np.zeros((12, 12))
+    for col in range(12):
+        a, i = divmod(col, 3)
+        up, um = u.copy(), u.copy()
+        up[cells[a], i] += h
+        um[cells[a], i] -= h
+        gp = pot.grad_cells(up)[t].reshape(12)
+        gm = pot.grad_cells(um)[t].reshape(12)
+        H[:, col] = (gp - gm) / (2 * h)
+    return H
+
+
+def _stretch_mesh_state(rng, m, A_diag=(1.3, 1.2, 1.25)):
+    """Stretched + rotated affine state, small noise: all sigma_i + sigma_j > 2."""
+    R = _rot(rng, 1)[0]
+    A = R @ np.diag(A_diag)
+    return m.points @ (A - np.eye(3)).T + 0.003 * rng.standard_normal(m.points.shape)
+
+
+@pytest.mark.parametrize("material", [of.SNH, of.ARAP])
+def test_element_hessian_matches_fd(material):
+    rng = np.random.default_rng(7)
+    m = cube(2)
+    pot = _fem(material, m, rng)
+    pot.mu[:] = 1.0
+    if pot.lam is not None:
+        pot.lam[:] = 3.0
+    if material == of.ARAP:
+        u = _stretch_mesh_state(rng, m)  # no lambda clamp fires: H is the exact Hessian
+    else:
+        u = 0.15 * rng.standard_normal(m.points.shape)
+    He = pot.element_hessians(u)
+    for t in range(0, pot.n_cells, 5):
+        np.testing.assert_allclose(He[t], _fd_element_hessian(pot, u, t), rtol=1e-5, atol=1e-7)
+
+
+@pytest.mark.parametrize("material", [of.SNH, of.ARAP])
+def test_global_grad_and_hess_prod_fd(material):
+    rng = np.random.default_rng(11)
+    m = cube(2)
+    pot = _fem(material, m, rng)
+    u = _stretch_mesh_state(rng, m) if material == of.ARAP else 0.1 * rng.standard_normal(m.points.shape)
+    v = rng.standard_normal(u.shape)
+    h = 1e-6
+    # directional derivative of the energy
+    d_fd = (pot.fun(u + h * v) - pot.fun(u - h * v)) / (2 * h)
+    assert np.dot(pot.grad(u).ravel(), v.ravel()) == pytest.approx(d_fd, rel=1e-7)
+    # hess_prod = derivative of grad along v
+    hv_fd = (pot.grad(u + h * v) - pot.grad(u - h * v)) / (2 * h)
+    np.testing.assert_allclose(pot.hess_prod(u, v), hv_fd, rtol=1e-5, atol=1e-6 * np.abs(hv_fd).max())
+
+
+def test_snh_hess_diag_is_exact_diagonal():
+    """SNH: every diag term is >= 0 and h6_diag == 0 is exact, so the clamped
+    reference diagonal equals the diagonal of the exact assembled Hessian."""
+    rng = np.random.default_rng(5)
+    m = cube(2)
+    pot = _fem(of.SNH, m, rng)
+    u = 0.1 * rng.standard_normal(m.points.shape)
+    hd = pot.hess_diag(u)
+    He = pot.element_hessians(u)
+    diag = np.zeros_like(hd)
+    np.add.at(diag, pot.cells.ravel(), np.einsum("tii->ti", He).reshape(-1, 3))
+    np.testing.assert_allclose(hd, diag, rtol=1e-12)
+    # and the element Hessian diagonal itself is the FD diagonal
+    np.testing.assert_allclose(
+        np.diag(He[3]), np.diag(_fd_element_hessian(pot, u, 3)), rtol=1e-6
+    )
+
+
+def test_arap_hess_diag_matches_fd_when_unclamped():
+    rng = np.random.default_rng(9)
+    m = cube(2)
+    pot = _fem(of.ARAP, m, rng)
+    u = _stretch_mesh_state(rng, m)
+    hd_cells = pot.hess_diag_cells(u)
+    for t in range(0, pot.n_cells, 7):
+        fd = np.diag(_fd_element_hessian(pot, u, t)).reshape(4, 3)
+        np.testing.assert_allclose(hd_cells[t], np.maximum(fd, 0.0), rtol=1e-5, atol=1e-8)
+
+
+@pytest.mark.parametrize("material", [of.SNH, of.ARAP])
+def test_hess_quad_clamp_and_consistency(material):
+    rng = np.random.default_rng(3)
+    m = cube(3)
+    pot = _fem(material, m, rng)
+    u = 0.08 * rng.standard_normal(m.points.shape)
+    v = rng.standard_normal(u.shape)
+    q_cells = pot.hess_quad_cells(u, v)
+    hp_cells = pot.hess_prod_cells(u, v)
+    vc = of.cell_values(v, pot.cells)
+    raw = np.einsum("tai,tai->t", hp_cells, vc)
+    # per-tet clamp max(0, .) (_base.py:379-380); hess_prod is unclamped
+    np.testing.assert_allclose(q_cells, np.maximum(raw, 0.0), rtol=1e-10, atol=1e-12 * np.abs(raw).max())
+    if np.all(raw >= 0):
+        assert pot.hess_quad(u, v) == pytest.approx(np.dot(pot.hess_prod(u, v).ravel(), v.ravel()), rel=1e-10)
+
+
+def test_arap_hessian_invariant_to_svd_convention():
+    rng = np.random.default_rng(21)
+    F = _random_F(16, 0.5, rng)
+    dhdX = rng.standard_normal((16, 4, 3))
+    p = rng.standard_normal((16, 4, 3))
+    U, s, V = of.svd_rv(F)
+    # flip the sign of one (U_i, V_i) pair: still a rotation-variant SVD of F
+    U2, V2 = U.copy(), V.copy()
+    U2[:, :, 0] *= -1
+    U2[:, :, 1] *= -1
+    V2[:, :, 0] *= -1
+    V2[:, :, 1] *= -1
+    np.testing.assert_allclose(U2 @ (s[:, :, None] * np.swapaxes(V2, 1, 2)), F, atol=1e-12)
+    lam = of.arap_lambdas(s)
+    dF = of.jvp(dhdX, p)
+    q1 = sum(lam[:, k] * of.ddot(Q, dF) ** 2 for k, Q in enumerate(of.arap_Qs(U, V)))
+    q2 = sum(lam[:, k] * of.ddot(Q, dF) ** 2 for k, Q in enumerate(of.arap_Qs(U2, V2)))
+    np.testing.assert_allclose(q1, q2, rtol=1e-12)
+
+
+def test_reference_arap_hess_prod_as_written_is_not_hv():
+    """Documents the argument swap of _arap.py:55-56 (the product implements H v)."""
+    rng = np.random.default_rng(2)
+    m = cube(2)
+    pot = _fem(of.ARAP, m, rng)
+    u = _stretch_mesh_state(rng, m)
+    v = rng.standard_normal(u.shape)
+    F = pot.F(u)
+    vc = of.cell_values(v, pot.cells)
+    good = of.arap_hess_prod(F, vc, pot.dhdX, pot.mu)
+    bad = of.arap_hess_prod_as_written(F, vc, pot.dhdX, pot.mu)
+    assert np.abs(good - bad).max() > 1e-3 * np.abs(good).max()
+
+
+@pytest.mark.parametrize("material", [of.SNH, of.ARAP])
+def test_mixed_derivative_fd(material):
+    """p^T dg/dmu (and dg/dlam) against FD of the gradient in the parameter."""
+    rng = np.random.default_rng(17)
+    m = cube(2)
+    pot = _fem(material, m, rng)
+    u = 0.05 * rng.standard_normal(m.points.shape)
+    p = rng.standard_normal(u.shape)
+    dmu, dlam = pot.mixed_cells(u, p)
+    h = 1e-3
+    for t in (0, 7, 19):
+        mu0 = pot.mu[t]
+        pot.mu[t] = mu0 + h
+        gp = pot.grad(u)
+        pot.mu[t] = mu0 - h
+        gm = pot.grad(u)
+        pot.mu[t] = mu0
+        assert dmu[t] == pytest.approx(np.sum((gp - gm) / (2 * h) * p), rel=1e-6)
+        if material == of.SNH:
+            l0 = pot.lam[t]
+            pot.lam[t] = l0 + h
+            gp = pot.grad(u)
+            pot.lam[t] = l0 - h
+            gm = pot.grad(u)
+            pot.lam[t] = l0
+            assert dlam[t] == pytest.approx(np.sum((gp - gm) / (2 * h) * p), rel=1e-6, abs=1e-12)
+
+
+def test_external_force_sign_convention():
+    u = RNG.standard_normal((5, 3))
+    f = RNG.standard_normal((2, 3))
+    ext = of.ExternalForce(np.array([1, 3]), f)
+    assert ext.fun(u) == pytest.approx(-(f * u[[1, 3]]).sum())
+    g = ext.grad(u)
+    np.testing.assert_allclose(g[[1, 3]], -f)
+    np.testing.assert_allclose(g[[0, 2, 4]], 0.0)
