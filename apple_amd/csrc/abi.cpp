@@ -1,0 +1,271 @@
+// C ABI: handles, setup upload, element evaluation, error reporting.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/apple_amd.h"
+#include "internal.h"
+#include "tetmath.h"
+
+namespace apl {
+
+static thread_local std::string g_err;
+void set_error(const std::string& s) { g_err = s; }
+
+template <class T>
+static int upload(apl_mesh* m, T** dst, const T* src, size_t n) {
+  *dst = nullptr;
+  if (n == 0) return 0;
+  APL_CHECK_HIP(hipMalloc((void**)dst, n * sizeof(T)));
+  m->device_bytes += n * sizeof(T);
+  APL_CHECK_HIP(hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+
+static int alloc_bytes(apl_mesh* m, void** dst, size_t n) {
+  *dst = nullptr;
+  if (n == 0) return 0;
+  APL_CHECK_HIP(hipMalloc(dst, n));
+  m->device_bytes += n;
+  APL_CHECK_HIP(hipMemset(*dst, 0, n));
+  return 0;
+}
+
+static void free_mesh(apl_mesh* m) {
+  DevLayout& d = m->d;
+  void* ptrs[] = {d.dminv,       d.dV,          d.slot,        d.blk_tet_off, d.blk_loc_off,   d.blk_nown,
+                  d.blk_sh_base, d.blk_inst,    d.inst_blk_off, d.loc_gid,    d.loc_inc_off,   d.loc_inc,
+                  d.sh_vtx,      d.sh_off,      d.sh_src,       d.vtx_sh_off, d.vblk_off,      d.vblk_inst,
+                  d.inst_vblk_off, d.inst_vtx_off, d.partial,   d.blk_red,    d.vblk_dots};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  delete m;
+}
+
+static unsigned to_internal(unsigned what) {
+  unsigned w = 0;
+  if (what & APL_ENERGY) w |= W_E;
+  if (what & APL_GRAD) w |= W_G;
+  if (what & APL_HESS_DIAG) w |= W_H;
+  if (what & APL_HESS_PROD) w |= W_HP;
+  if (what & APL_HESS_QUAD) w |= W_HQ;
+  if (what & APL_MIXED) w |= W_MIX;
+  return w;
+}
+
+}  // namespace apl
+
+using namespace apl;
+
+extern "C" {
+
+const char* apl_last_error(void) { return g_err.c_str(); }
+
+int apl_version(void) { return 10000; }
+
+int apl_mesh_create(int dtype, int64_t n_tets, int64_t n_verts, const int32_t* cells, const double* dminv,
+                    const double* dv, int64_t n_inst, const int64_t* inst_tet_off, const int64_t* inst_vtx_off,
+                    int chunks, int device, apl_mesh** out) {
+  if (out == nullptr) {
+    set_error("out is NULL");
+    return -1;
+  }
+  *out = nullptr;
+  if (dtype != APL_F32 && dtype != APL_F64) {
+    set_error("dtype must be APL_F32 or APL_F64");
+    return -1;
+  }
+  if (n_tets > 0 && (cells == nullptr || dminv == nullptr || dv == nullptr)) {
+    set_error("cells, dminv and dv must be non-NULL");
+    return -1;
+  }
+  for (int64_t i = 0; i < 4 * n_tets; ++i)
+    if (cells[i] < 0 || cells[i] >= n_verts) {
+      set_error("cell index out of range [0, n_verts)");
+      return -1;
+    }
+  int64_t one_t[2] = {0, n_tets}, one_v[2] = {0, n_verts};
+  if (n_inst <= 1 || inst_tet_off == nullptr || inst_vtx_off == nullptr) {
+    if (n_inst > 1) {
+      set_error("instance offsets are required for n_inst > 1");
+      return -1;
+    }
+    n_inst = 1;
+    inst_tet_off = one_t;
+    inst_vtx_off = one_v;
+  }
+  APL_CHECK_HIP(hipSetDevice(device));
+  apl_mesh* m = new apl_mesh();
+  m->dtype = dtype;
+  m->device = device;
+  int ncu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
+    m->n_cu = ncu;
+  const int nloc_cap = dtype == APL_F64 ? 512 : 1024;
+  std::string err = build_layout(cells, n_tets, n_verts, n_inst, inst_tet_off, inst_vtx_off, NTH,
+                                 chunks < 1 ? 1 : chunks, nloc_cap, m->L);
+  if (!err.empty()) {
+    set_error(err);
+    delete m;
+    return -1;
+  }
+  Layout& L = m->L;
+  DevLayout& d = m->d;
+  const size_t rs = dtype == APL_F64 ? 8 : 4;
+  int rc = 0;
+  // element data, SoA, in the mesh precision
+  {
+    std::vector<char> buf(9 * (size_t)n_tets * rs);
+    for (int64_t t = 0; t < n_tets; ++t)
+      for (int k = 0; k < 9; ++k) {
+        double v = dminv[t * 9 + k];
+        if (rs == 8)
+          reinterpret_cast<double*>(buf.data())[k * n_tets + t] = v;
+        else
+          reinterpret_cast<float*>(buf.data())[k * n_tets + t] = (float)v;
+      }
+    rc |= alloc_bytes(m, &d.dminv, buf.size());
+    if (!rc && !buf.empty() && hipMemcpy(d.dminv, buf.data(), buf.size(), hipMemcpyHostToDevice) != hipSuccess) rc = -1;
+    std::vector<char> bv((size_t)n_tets * rs);
+    for (int64_t t = 0; t < n_tets; ++t) {
+      if (rs == 8)
+        reinterpret_cast<double*>(bv.data())[t] = dv[t];
+      else
+        reinterpret_cast<float*>(bv.data())[t] = (float)dv[t];
+    }
+    rc |= alloc_bytes(m, &d.dV, bv.size());
+    if (!rc && !bv.empty() && hipMemcpy(d.dV, bv.data(), bv.size(), hipMemcpyHostToDevice) != hipSuccess) rc = -1;
+  }
+  rc |= upload(m, &d.slot, L.slot.data(), L.slot.size());
+  rc |= upload(m, &d.blk_tet_off, L.blk_tet_off.data(), L.blk_tet_off.size());
+  rc |= upload(m, &d.blk_loc_off, L.blk_loc_off.data(), L.blk_loc_off.size());
+  rc |= upload(m, &d.blk_nown, L.blk_nown.data(), L.blk_nown.size());
+  rc |= upload(m, &d.blk_sh_base, L.blk_sh_base.data(), L.blk_sh_base.size());
+  rc |= upload(m, &d.blk_inst, L.blk_inst.data(), L.blk_inst.size());
+  rc |= upload(m, &d.inst_blk_off, L.inst_blk_off.data(), L.inst_blk_off.size());
+  rc |= upload(m, &d.loc_gid, L.loc_gid.data(), L.loc_gid.size());
+  rc |= upload(m, &d.loc_inc_off, L.loc_inc_off.data(), L.loc_inc_off.size());
+  rc |= upload(m, &d.loc_inc, L.loc_inc.data(), L.loc_inc.size());
+  rc |= upload(m, &d.sh_vtx, L.sh_vtx.data(), L.sh_vtx.size());
+  rc |= upload(m, &d.sh_off, L.sh_off.data(), L.sh_off.size());
+  rc |= upload(m, &d.sh_src, L.sh_src.data(), L.sh_src.size());
+  {
+    std::vector<int32_t> vso((size_t)n_verts + 1, 0);
+    for (int64_t i = 0; i < L.n_shared_vtx; ++i) vso[L.sh_vtx[i] + 1] = L.sh_off[i + 1] - L.sh_off[i];
+    for (int64_t v = 0; v < n_verts; ++v) vso[v + 1] += vso[v];
+    rc |= upload(m, &d.vtx_sh_off, vso.data(), vso.size());
+  }
+  rc |= upload(m, &d.vblk_off, L.vblk_off.data(), L.vblk_off.size());
+  rc |= upload(m, &d.vblk_inst, L.vblk_inst.data(), L.vblk_inst.size());
+  rc |= upload(m, &d.inst_vblk_off, L.inst_vblk_off.data(), L.inst_vblk_off.size());
+  rc |= upload(m, &d.inst_vtx_off, inst_vtx_off, (size_t)n_inst + 1);
+  rc |= alloc_bytes(m, &d.partial, (size_t)MAX_CH * std::max<int64_t>(L.n_partials, 1) * rs);
+  rc |= alloc_bytes(m, (void**)&d.blk_red, 2 * (size_t)std::max<int64_t>(L.nblocks, 1) * sizeof(double));
+  rc |= alloc_bytes(m, (void**)&d.vblk_dots,
+                    (size_t)N_DOTS * std::max<size_t>(L.vblk_inst.size(), 1) * sizeof(double));
+  if (rc) {
+    free_mesh(m);
+    return -1;
+  }
+  // free the host-side tables the device now holds; keep the sizes
+  L.slot = std::vector<uint16_t>();
+  L.loc_inc = std::vector<uint16_t>();
+  L.loc_inc_off = std::vector<uint16_t>();
+  L.loc_gid = std::vector<int32_t>();
+  *out = m;
+  return 0;
+}
+
+int apl_mesh_destroy(apl_mesh* mesh) {
+  if (mesh == nullptr) return 0;
+  (void)hipSetDevice(mesh->device);
+  free_mesh(mesh);
+  return 0;
+}
+
+int apl_mesh_info(const apl_mesh* mesh, apl_mesh_info_t* info) {
+  if (mesh == nullptr || info == nullptr) {
+    set_error("NULL argument");
+    return -1;
+  }
+  apl_mesh* m = const_cast<apl_mesh*>(mesh);
+  (void)hipSetDevice(m->device);
+  const Layout& L = m->L;
+  info->n_tets = L.T;
+  info->n_verts = L.V;
+  info->n_inst = L.n_inst;
+  info->n_blocks = L.nblocks;
+  info->n_partials = L.n_partials;
+  info->n_shared_verts = L.n_shared_vtx;
+  info->n_vertex_blocks = (int64_t)L.vblk_inst.size();
+  info->nloc_max = L.nloc_max;
+  info->chunks = L.chunks;
+  info->dtype = m->dtype;
+  info->elem_grid_snh = elem_grid(m, MAT_SNH, W_E | W_G | W_H);
+  info->elem_grid_arap = elem_grid(m, MAT_ARAP, W_E | W_G | W_H);
+  info->device_bytes = (int64_t)m->device_bytes;
+  return 0;
+}
+
+int apl_elem_eval(apl_mesh* m, int material, const void* mu, const void* lam, const void* u, const void* p,
+                  unsigned what, double* energy, void* grad, void* hess_diag, void* hess_prod, double* hess_quad,
+                  void* dmu, void* dlam, void* stream) {
+  if (m == nullptr) {
+    set_error("mesh is NULL");
+    return -1;
+  }
+  if (material != APL_SNH && material != APL_ARAP) {
+    set_error("material must be APL_SNH or APL_ARAP");
+    return -1;
+  }
+  const unsigned w = to_internal(what);
+  const int acc = (what & APL_ACCUMULATE) ? 1 : 0;
+  if (w == 0) return 0;
+  if (mu == nullptr || u == nullptr || (material == APL_SNH && lam == nullptr)) {
+    set_error("mu, lam (SNH) and u must be non-NULL");
+    return -1;
+  }
+  if ((w & (W_HP | W_HQ | W_MIX)) && p == nullptr) {
+    set_error("p must be non-NULL for hess_prod, hess_quad and mixed derivatives");
+    return -1;
+  }
+  if (((w & W_E) && !energy) || ((w & W_G) && !grad) || ((w & W_H) && !hess_diag) || ((w & W_HP) && !hess_prod) ||
+      ((w & W_HQ) && !hess_quad) || ((w & W_MIX) && (!dmu || (material == APL_SNH && !dlam)))) {
+    set_error("a requested output pointer is NULL");
+    return -1;
+  }
+  APL_CHECK_HIP(hipSetDevice(m->device));
+  hipStream_t s = (hipStream_t)stream;
+  const unsigned egh = w & (W_E | W_G | W_H);
+  if (egh) {
+    // instantiated pass variants: E, G, H, E|G|H; other subsets run as separate passes
+    std::vector<unsigned> passes;
+    if (egh == (W_E | W_G | W_H) || egh == W_E || egh == W_G || egh == W_H)
+      passes.push_back(egh);
+    else
+      for (unsigned b : {W_E, W_G, W_H})
+        if (egh & b) passes.push_back(b);
+    for (unsigned pw : passes) {
+      if (launch_elem(m, material, pw, mu, lam, u, nullptr, nullptr, nullptr, grad, hess_diag, acc, 0, s)) return -1;
+      if ((pw & (W_G | W_H)) && launch_shared_reduce(m, pw, grad, hess_diag, acc, nullptr, s)) return -1;
+      if ((pw & W_E) && launch_inst_reduce(m, energy, nullptr, acc, nullptr, s)) return -1;
+    }
+  }
+  if (w & W_HP) {
+    if (launch_elem(m, material, W_HP, mu, lam, u, p, nullptr, nullptr, hess_prod, nullptr, acc, 0, s)) return -1;
+    if (launch_shared_reduce(m, W_HP, hess_prod, nullptr, acc, nullptr, s)) return -1;
+  }
+  if (w & W_HQ) {
+    if (launch_elem(m, material, W_HQ, mu, lam, u, p, nullptr, nullptr, nullptr, nullptr, acc, 0, s)) return -1;
+    if (launch_inst_reduce(m, nullptr, hess_quad, acc, nullptr, s)) return -1;
+  }
+  if (w & W_MIX) {
+    if (launch_elem(m, material, W_MIX, mu, lam, u, p, nullptr, nullptr, dmu, dlam, acc, 0, s)) return -1;
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,87 @@
+// Library-internal types shared by the element kernels, the vector kernels
+// and the C-ABI layer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "layout.h"
+
+namespace apl {
+
+constexpr int NTH = 256;       // threads per element / vector block
+constexpr int SPT = 4;         // local slots reduced per thread (nloc_cap / NTH)
+constexpr int MAX_CH = 6;      // scatter channels per incidence (grad + hdiag)
+constexpr int N_DOTS = 16;     // per vertex-block partial dot products
+
+enum Dtype : int { DT_F32 = 0, DT_F64 = 1 };
+
+void set_error(const std::string& s);
+
+#define APL_CHECK_HIP(expr)                                                      \
+  do {                                                                         \
+    hipError_t e_ = (expr);                                                    \
+    if (e_ != hipSuccess) {                                                    \
+      ::apl::set_error(std::string(#expr) + ": " + hipGetErrorString(e_));      \
+      return -1;                                                               \
+    }                                                                          \
+  } while (0)
+
+struct DevLayout {
+  void* dminv = nullptr;  // 9*T, SoA (k*T + t)
+  void* dV = nullptr;     // T
+  uint16_t* slot = nullptr;  // 4*T
+  int32_t* blk_tet_off = nullptr;
+  int32_t* blk_loc_off = nullptr;
+  int32_t* blk_nown = nullptr;
+  int32_t* blk_sh_base = nullptr;
+  int32_t* blk_inst = nullptr;
+  int64_t* inst_blk_off = nullptr;
+  int32_t* loc_gid = nullptr;
+  uint16_t* loc_inc_off = nullptr;
+  uint16_t* loc_inc = nullptr;
+  int32_t* sh_vtx = nullptr;
+  int32_t* sh_off = nullptr;
+  int32_t* sh_src = nullptr;
+  int32_t* vtx_sh_off = nullptr;  // V+1: partial range of every vertex (empty if owned)
+  int32_t* vblk_off = nullptr;
+  int32_t* vblk_inst = nullptr;
+  int64_t* inst_vblk_off = nullptr;
+  int64_t* inst_vtx_off = nullptr;
+  // workspace
+  void* partial = nullptr;   // MAX_CH * n_partials (SoA by channel)
+  double* blk_red = nullptr; // 2 * nblocks
+  double* vblk_dots = nullptr;  // N_DOTS * nvblk
+};
+
+}  // namespace apl
+
+struct apl_mesh {
+  int dtype = 0;
+  int device = 0;
+  int n_cu = 256;
+  apl::Layout L;
+  apl::DevLayout d;
+  size_t device_bytes = 0;
+};
+
+namespace apl {
+
+inline size_t real_size(const apl_mesh* m) { return m->dtype == DT_F64 ? 8 : 4; }
+
+// element pass: per-block partials (vertex sums of owned slots are written
+// directly into out_a/out_b; shared slots into the partial workspace)
+int launch_elem(apl_mesh* m, int material, unsigned what, const void* mu, const void* lam,
+                const void* x, const void* p, const double* alpha, const int32_t* active,
+                void* out_a, void* out_b, int accumulate, int grid_override, hipStream_t s);
+// vertex sums of shared vertices from the partials
+int launch_shared_reduce(apl_mesh* m, unsigned what, void* out_a, void* out_b, int accumulate,
+                         const int32_t* active, hipStream_t s);
+// per-instance sums of the block energies / quadratic forms
+int launch_inst_reduce(apl_mesh* m, double* out_e, double* out_q, int accumulate,
+                       const int32_t* active, hipStream_t s);
+int elem_grid(apl_mesh* m, int material, unsigned what);
+
+}  // namespace apl

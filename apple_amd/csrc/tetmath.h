@@ -1,0 +1,538 @@
+// Per-tet elasticity algebra for the element kernels (host + device).
+//
+// One tet = 4 vertices, 1 quadrature point.  Inputs are the 3x3 inverse rest
+// shape matrix B = inv(dX/dr) (rows 1..3 of the reference's dhdX; row 0 is
+// minus their sum, jax/fem/region/_region.py:84-108), the quadrature volume
+// dV and the 4 vertex displacements.  Everything is expressed through the
+// displacement gradient G = F - I, which keeps FP32 accurate near the rest
+// state: the energies and stresses below contain no "1 - 1" cancellation.
+//
+//   stable Neo-Hookean (warp/fem/_stable_neo_hookean.py:17-103)
+//     Psi = mu/2 (|F|^2 - 3) - mu (J - 1) + lam/2 (J - 1)^2
+//         = mu/2 |G|^2 - mu (c2(G) + det G) + lam/2 (J - 1)^2,
+//     J - 1 = tr G + c2(G) + det G, c2 = sum of principal 2x2 minors,
+//     cof F = I + tr(G) I - G^T + cof G,
+//     P = mu (F - cof F) + lam (J-1) cof F,  F - cof F = G + G^T - tr(G) I - cof G.
+//   ARAP (warp/fem/_arap.py:17-76, func/_misc.py:31-70)
+//     F^T F = I + E with E = G + G^T + G^T G; E's eigenpairs give V and
+//     sigma_i^2 = 1 + e_i, so sigma_i - 1 = e_i / (1 + sqrt(1 + e_i)) without
+//     cancellation.  U follows by Gram-Schmidt on F V, the sign of det F
+//     goes on the smallest singular value (rotation-variant SVD, U, V in
+//     SO(3)), R = U V^T and F - R = U diag(sigma - 1) V^T.
+//     The eigen-solver is cyclic Jacobi with exact 2x2 rotations
+//     (Golub & Van Loan, Matrix Computations, 4th ed., Alg. 8.5.2-8.5.3).
+//   Hessian pieces follow func/_hess_diag.py, _hess_prod.py, _hess_quad.py:
+//     SNH  H = lam g3 g3^T + mu I + (-mu + lam (J-1)) d2(det),
+//     ARAP H = mu (I - sum_k lambda_k q_k q_k^T), lambda_k = 2/max(s_i+s_j, 2).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+namespace apl {
+
+enum Material : int { MAT_SNH = 0, MAT_ARAP = 1 };
+
+template <class R>
+struct V3 {
+  R x, y, z;
+};
+
+template <class R>
+__host__ __device__ inline V3<R> v3(R x, R y, R z) {
+  return V3<R>{x, y, z};
+}
+template <class R>
+__host__ __device__ inline V3<R> operator+(V3<R> a, V3<R> b) {
+  return v3(a.x + b.x, a.y + b.y, a.z + b.z);
+}
+template <class R>
+__host__ __device__ inline V3<R> operator-(V3<R> a, V3<R> b) {
+  return v3(a.x - b.x, a.y - b.y, a.z - b.z);
+}
+template <class R>
+__host__ __device__ inline V3<R> operator*(R s, V3<R> a) {
+  return v3(s * a.x, s * a.y, s * a.z);
+}
+template <class R>
+__host__ __device__ inline R dot(V3<R> a, V3<R> b) {
+  return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+template <class R>
+__host__ __device__ inline V3<R> cross(V3<R> a, V3<R> b) {
+  return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+
+// 3x3 matrix, row-major m[i][j]
+template <class R>
+struct M3 {
+  R m[3][3];
+};
+
+template <class R>
+__host__ __device__ inline V3<R> col(const M3<R>& A, int j) {
+  return v3(A.m[0][j], A.m[1][j], A.m[2][j]);
+}
+template <class R>
+__host__ __device__ inline void set_col(M3<R>& A, int j, V3<R> c) {
+  A.m[0][j] = c.x;
+  A.m[1][j] = c.y;
+  A.m[2][j] = c.z;
+}
+template <class R>
+__host__ __device__ inline V3<R> mul(const M3<R>& A, V3<R> d) {
+  return v3(A.m[0][0] * d.x + A.m[0][1] * d.y + A.m[0][2] * d.z,
+            A.m[1][0] * d.x + A.m[1][1] * d.y + A.m[1][2] * d.z,
+            A.m[2][0] * d.x + A.m[2][1] * d.y + A.m[2][2] * d.z);
+}
+template <class R>
+__host__ __device__ inline R ddot(const M3<R>& A, const M3<R>& B) {
+  R s = 0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) s += A.m[i][j] * B.m[i][j];
+  return s;
+}
+template <class R>
+__host__ __device__ inline R det3(const M3<R>& A) {
+  return A.m[0][0] * (A.m[1][1] * A.m[2][2] - A.m[1][2] * A.m[2][1]) -
+         A.m[0][1] * (A.m[1][0] * A.m[2][2] - A.m[1][2] * A.m[2][0]) +
+         A.m[0][2] * (A.m[1][0] * A.m[2][1] - A.m[1][1] * A.m[2][0]);
+}
+// cofactor matrix: columns [c1 x c2, c2 x c0, c0 x c1] (func/_gradient.py:30-40)
+template <class R>
+__host__ __device__ inline M3<R> cof3(const M3<R>& A) {
+  M3<R> C;
+  V3<R> c0 = col(A, 0), c1 = col(A, 1), c2 = col(A, 2);
+  set_col(C, 0, cross(c1, c2));
+  set_col(C, 1, cross(c2, c0));
+  set_col(C, 2, cross(c0, c1));
+  return C;
+}
+
+// Per-tet geometry: rows d[0..3] of dhdX (d0 = -(d1+d2+d3)) and dV.
+template <class R>
+struct TetGeom {
+  V3<R> d[4];
+  R dV;
+};
+
+template <class R>
+__host__ __device__ inline TetGeom<R> make_geom(const R B[9], R dV) {
+  TetGeom<R> g;
+  g.d[1] = v3(B[0], B[1], B[2]);
+  g.d[2] = v3(B[3], B[4], B[5]);
+  g.d[3] = v3(B[6], B[7], B[8]);
+  g.d[0] = v3(-(B[0] + B[3] + B[6]), -(B[1] + B[4] + B[7]), -(B[2] + B[5] + B[8]));
+  g.dV = dV;
+  return g;
+}
+
+// G = sum_a u_a (x) d_a  (func/_deformation.py:9-25: u^T dhdX, also the jvp)
+template <class R>
+__host__ __device__ inline M3<R> outer_sum(const TetGeom<R>& g, const V3<R> u[4]) {
+  M3<R> G;
+  const R* uc[4] = {&u[0].x, &u[1].x, &u[2].x, &u[3].x};
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    R ui0 = uc[0][i], ui1 = uc[1][i], ui2 = uc[2][i], ui3 = uc[3][i];
+    G.m[i][0] = ui0 * g.d[0].x + ui1 * g.d[1].x + ui2 * g.d[2].x + ui3 * g.d[3].x;
+    G.m[i][1] = ui0 * g.d[0].y + ui1 * g.d[1].y + ui2 * g.d[2].y + ui3 * g.d[3].y;
+    G.m[i][2] = ui0 * g.d[0].z + ui1 * g.d[1].z + ui2 * g.d[2].z + ui3 * g.d[3].z;
+  }
+  return G;
+}
+
+// Invariants of F = I + G without cancellation.
+template <class R>
+struct Inv {
+  R trG, c2, detG, Jm1, G2;
+};
+
+template <class R>
+__host__ __device__ inline Inv<R> invariants(const M3<R>& G) {
+  Inv<R> v;
+  v.trG = G.m[0][0] + G.m[1][1] + G.m[2][2];
+  v.c2 = (G.m[0][0] * G.m[1][1] - G.m[0][1] * G.m[1][0]) +
+         (G.m[1][1] * G.m[2][2] - G.m[1][2] * G.m[2][1]) +
+         (G.m[0][0] * G.m[2][2] - G.m[0][2] * G.m[2][0]);
+  v.detG = det3(G);
+  v.Jm1 = v.trG + (v.c2 + v.detG);
+  v.G2 = ddot(G, G);
+  return v;
+}
+
+// cof F with F = I + G: I + tr(G) I - G^T + cof G
+template <class R>
+__host__ __device__ inline M3<R> cofF_from_G(const M3<R>& G, R trG) {
+  M3<R> C = cof3(G);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) C.m[i][j] -= G.m[j][i];
+  C.m[0][0] += R(1) + trG;
+  C.m[1][1] += R(1) + trG;
+  C.m[2][2] += R(1) + trG;
+  return C;
+}
+
+// F - cof F = G + G^T - tr(G) I - cof G
+template <class R>
+__host__ __device__ inline M3<R> F_minus_cofF(const M3<R>& G, R trG) {
+  M3<R> C = cof3(G);
+  M3<R> D;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) D.m[i][j] = G.m[i][j] + G.m[j][i] - C.m[i][j];
+  D.m[0][0] -= trG;
+  D.m[1][1] -= trG;
+  D.m[2][2] -= trG;
+  return D;
+}
+
+// ---------------------------------------------------------------- SVD (ARAP)
+
+template <class R>
+__host__ __device__ inline void jacobi_rotate(R (&a)[3][3], M3<R>& V, int p, int q) {
+  R apq = a[p][q];
+  R scale = fabs(a[p][p]) + fabs(a[q][q]);
+  if (!(fabs(apq) > R(1e-30) + scale * (sizeof(R) == 4 ? R(1e-9) : R(1e-18)))) {
+    a[p][q] = a[q][p] = R(0);
+    return;
+  }
+  R tau = (a[q][q] - a[p][p]) / (R(2) * apq);
+  R t = (tau >= R(0) ? R(1) : R(-1)) / (fabs(tau) + sqrt(R(1) + tau * tau));
+  R c = R(1) / sqrt(R(1) + t * t);
+  R s = t * c;
+  a[p][p] -= t * apq;
+  a[q][q] += t * apq;
+  a[p][q] = a[q][p] = R(0);
+  int r = 3 - p - q;
+  R arp = a[r][p], arq = a[r][q];
+  a[r][p] = a[p][r] = c * arp - s * arq;
+  a[r][q] = a[q][r] = s * arp + c * arq;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    R vp = V.m[i][p], vq = V.m[i][q];
+    V.m[i][p] = c * vp - s * vq;
+    V.m[i][q] = s * vp + c * vq;
+  }
+}
+
+template <class R>
+__host__ __device__ inline void swap_cols(M3<R>& V, R (&e)[3], int i, int j) {
+  R t = e[i];
+  e[i] = e[j];
+  e[j] = t;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    R a = V.m[r][i];
+    V.m[r][i] = V.m[r][j];
+    V.m[r][j] = -a;  // keep det V = +1
+  }
+}
+
+template <class R>
+struct Svd {
+  M3<R> U, V;
+  R s[3];    // signed singular values, |s0| >= |s1| >= |s2|, sign(s2) = sign(det F)
+  R sm1[3];  // s_i - 1 computed without cancellation
+};
+
+// Rotation-variant SVD of F = I + G (see file header).
+template <class R>
+__host__ __device__ inline Svd<R> svd_rv(const M3<R>& G) {
+  R a[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      R gtg = G.m[0][i] * G.m[0][j] + G.m[1][i] * G.m[1][j] + G.m[2][i] * G.m[2][j];
+      a[i][j] = G.m[i][j] + G.m[j][i] + gtg;
+    }
+  Svd<R> r;
+  M3<R>& V = r.V;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) V.m[i][j] = (i == j) ? R(1) : R(0);
+  const int sweeps = sizeof(R) == 4 ? 4 : 6;
+  for (int sw = 0; sw < sweeps; ++sw) {
+    jacobi_rotate(a, V, 0, 1);
+    jacobi_rotate(a, V, 0, 2);
+    jacobi_rotate(a, V, 1, 2);
+  }
+  R e[3] = {a[0][0], a[1][1], a[2][2]};
+  if (e[0] < e[1]) swap_cols(V, e, 0, 1);
+  if (e[0] < e[2]) swap_cols(V, e, 0, 2);
+  if (e[1] < e[2]) swap_cols(V, e, 1, 2);
+  // U and the singular values by Gram-Schmidt QR of F V, F = I + G
+  // (the QR step of McAdams et al., "Computing the SVD of 3x3 matrices with
+  // minimal branching", 2011): sigma = diag(R) keeps absolute accuracy
+  // eps*|F| even for near-singular F, unlike sqrt(1 + e_i).
+  V3<R> v0 = col(V, 0), v1 = col(V, 1), v2 = col(V, 2);
+  V3<R> f0 = v0 + mul(G, v0);
+  V3<R> f1 = v1 + mul(G, v1);
+  V3<R> f2 = v2 + mul(G, v2);
+  R n0 = sqrt(dot(f0, f0));
+  V3<R> u0;
+  if (n0 > R(1e-30)) {
+    u0 = (R(1) / n0) * f0;
+  } else {
+    u0 = v0;
+  }
+  V3<R> w = f1 - dot(u0, f1) * u0;
+  R n1 = sqrt(dot(w, w));
+  V3<R> u1;
+  if (n1 > R(1e-30) * (R(1) + n0)) {
+    u1 = (R(1) / n1) * w;
+  } else {
+    // any unit vector orthogonal to u0
+    V3<R> t = fabs(u0.x) < R(0.6) ? v3(R(1), R(0), R(0)) : v3(R(0), R(1), R(0));
+    w = t - dot(u0, t) * u0;
+    u1 = (R(1) / sqrt(dot(w, w))) * w;
+  }
+  V3<R> u2 = cross(u0, u1);
+  set_col(r.U, 0, u0);
+  set_col(r.U, 1, u1);
+  set_col(r.U, 2, u2);
+  const R sq[3] = {n0, n1, dot(u2, f2)};
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    // near sigma = 1 take sigma - 1 = e / (1 + sqrt(1 + e)) (no cancellation)
+    if (fabs(e[i]) < R(0.25) && sq[i] > R(0)) {
+      R sm1 = e[i] / (R(1) + sqrt(R(1) + e[i]));
+      r.sm1[i] = sm1;
+      r.s[i] = R(1) + sm1;
+    } else {
+      r.s[i] = sq[i];
+      r.sm1[i] = sq[i] - R(1);
+    }
+  }
+  return r;
+}
+
+// U diag(d) V^T
+template <class R>
+__host__ __device__ inline M3<R> udv(const Svd<R>& S, const R d[3]) {
+  M3<R> M;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      M.m[i][j] = S.U.m[i][0] * d[0] * S.V.m[j][0] + S.U.m[i][1] * d[1] * S.V.m[j][1] +
+                  S.U.m[i][2] * d[2] * S.V.m[j][2];
+  return M;
+}
+
+// lambda_k = 2 / max(s_i + s_j, 2) over pairs (0,1), (1,2), (2,0)  (func/_misc.py:31-43)
+template <class R>
+__host__ __device__ inline void arap_lambdas(const R s[3], R lam[3]) {
+  R p0 = s[0] + s[1], p1 = s[1] + s[2], p2 = s[2] + s[0];
+  lam[0] = R(2) / (p0 > R(2) ? p0 : R(2));
+  lam[1] = R(2) / (p1 > R(2) ? p1 : R(2));
+  lam[2] = R(2) / (p2 > R(2) ? p2 : R(2));
+}
+
+// Twist-mode coordinates of a vector d: (Q0 d, Q1 d, Q2 d) (func/_misc.py:56-70)
+//   Q0 = (U1 V0^T - U0 V1^T)/sqrt2, Q1 = (U1 V2^T - U2 V1^T)/sqrt2, Q2 = (U0 V2^T - U2 V0^T)/sqrt2
+template <class R>
+__host__ __device__ inline void twist_apply(const Svd<R>& S, V3<R> d, V3<R> out[3]) {
+  const R r2 = R(0.70710678118654752440);
+  R w0 = S.V.m[0][0] * d.x + S.V.m[1][0] * d.y + S.V.m[2][0] * d.z;
+  R w1 = S.V.m[0][1] * d.x + S.V.m[1][1] * d.y + S.V.m[2][1] * d.z;
+  R w2 = S.V.m[0][2] * d.x + S.V.m[1][2] * d.y + S.V.m[2][2] * d.z;
+  V3<R> U0 = col(S.U, 0), U1 = col(S.U, 1), U2 = col(S.U, 2);
+  out[0] = r2 * (w0 * U1 - w1 * U0);
+  out[1] = r2 * (w2 * U1 - w1 * U2);
+  out[2] = r2 * (w2 * U0 - w0 * U2);
+}
+
+// Q_k : dF for k = 0,1,2 via M = U^T dF V
+template <class R>
+__host__ __device__ inline void twist_coords(const Svd<R>& S, const M3<R>& dF, R q[3], R diagM[3]) {
+  R M[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    V3<R> Ua = col(S.U, a);
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      V3<R> Vb = col(S.V, b);
+      M[a][b] = dot(Ua, mul(dF, Vb));
+    }
+  }
+  const R r2 = R(0.70710678118654752440);
+  q[0] = r2 * (M[1][0] - M[0][1]);
+  q[1] = r2 * (M[1][2] - M[2][1]);
+  q[2] = r2 * (M[0][2] - M[2][0]);
+  diagM[0] = M[0][0];
+  diagM[1] = M[1][1];
+  diagM[2] = M[2][2];
+}
+
+// sum_k c_k Q_k as a matrix
+template <class R>
+__host__ __device__ inline M3<R> twist_combine(const Svd<R>& S, const R c[3]) {
+  const R r2 = R(0.70710678118654752440);
+  M3<R> M;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      R q0 = S.U.m[i][1] * S.V.m[j][0] - S.U.m[i][0] * S.V.m[j][1];
+      R q1 = S.U.m[i][1] * S.V.m[j][2] - S.U.m[i][2] * S.V.m[j][1];
+      R q2 = S.U.m[i][0] * S.V.m[j][2] - S.U.m[i][2] * S.V.m[j][0];
+      M.m[i][j] = r2 * (c[0] * q0 + c[1] * q1 + c[2] * q2);
+    }
+  return M;
+}
+
+// ---------------------------------------------------------------- per-tet evaluation
+
+enum : unsigned {
+  W_E = 1u,      // energy  sum Psi dV
+  W_G = 2u,      // gradient (4 x vec3)
+  W_H = 4u,      // clamped Hessian diagonal (4 x vec3)
+  W_HP = 8u,     // Hessian-vector product (4 x vec3)
+  W_HQ = 16u,    // clamped quadratic form v^T H v
+  W_MIX = 32u,   // mixed derivatives p^T dg/dmu, p^T dg/dlam (per tet)
+};
+
+template <class R>
+struct TetOut {
+  R e;          // Psi dV
+  V3<R> g[4];   // dV P d_a
+  V3<R> h[4];   // max(dV diag H, 0)
+  V3<R> hp[4];  // dV H v
+  R hq;         // max(dV v^T H v, 0)
+  R dmu, dlam;  // dV (dP/dmu):dF(v), dV (dP/dlam):dF(v)
+};
+
+// Evaluate the requested quantities for one tet.  u: displacements, v: the
+// direction for W_HP / W_HQ / W_MIX.
+template <int MAT, unsigned WHAT, class R>
+__host__ __device__ inline void tet_eval(const TetGeom<R>& g, R mu, R lam, const V3<R> u[4],
+                                         const V3<R> v[4], TetOut<R>& o) {
+  const M3<R> G = outer_sum(g, u);
+  const Inv<R> I = invariants(G);
+  constexpr bool needV = (WHAT & (W_HP | W_HQ | W_MIX)) != 0;
+  M3<R> dF;
+  if constexpr (needV) dF = outer_sum(g, v);
+  if constexpr (MAT == MAT_SNH) {
+    if constexpr ((WHAT & W_E) != 0)
+      o.e = g.dV * (R(0.5) * mu * I.G2 - mu * (I.c2 + I.detG) + R(0.5) * lam * I.Jm1 * I.Jm1);
+    constexpr bool needC = (WHAT & (W_G | W_H | W_HP | W_HQ | W_MIX)) != 0;
+    if constexpr (needC) {
+      const M3<R> C = cofF_from_G(G, I.trG);  // g3 = cof F
+      if constexpr ((WHAT & W_G) != 0) {
+        const M3<R> D = F_minus_cofF(G, I.trG);
+        M3<R> P;
+        const R lj = lam * I.Jm1;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) P.m[i][j] = mu * D.m[i][j] + lj * C.m[i][j];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) o.g[a] = g.dV * mul(P, g.d[a]);
+      }
+      if constexpr ((WHAT & W_H) != 0) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          V3<R> w = mul(C, g.d[a]);
+          R t = mu * dot(g.d[a], g.d[a]);
+          R hx = g.dV * (lam * w.x * w.x + t);
+          R hy = g.dV * (lam * w.y * w.y + t);
+          R hz = g.dV * (lam * w.z * w.z + t);
+          o.h[a] = v3(hx > R(0) ? hx : R(0), hy > R(0) ? hy : R(0), hz > R(0) ? hz : R(0));
+        }
+      }
+      if constexpr ((WHAT & (W_HP | W_HQ | W_MIX)) != 0) {
+        const R s = -mu + lam * I.Jm1;
+        const R cd = ddot(C, dF);
+        if constexpr ((WHAT & W_HQ) != 0) {
+          // 2 F : cof(dF) = d^2/dt^2 det(F + t dF)  (h6_quad, func/_hess_quad.py:59-74)
+          M3<R> cdF = cof3(dF);
+          R h6 = R(2) * (cdF.m[0][0] + cdF.m[1][1] + cdF.m[2][2] + ddot(G, cdF));
+          R q = g.dV * (lam * cd * cd + mu * ddot(dF, dF) + s * h6);
+          o.hq = q > R(0) ? q : R(0);
+        }
+        if constexpr ((WHAT & W_HP) != 0) {
+          // Hp = d/dt cof(F + t dF) (h6_prod, func/_hess_prod.py:61-77)
+          V3<R> f0 = v3(R(1) + G.m[0][0], G.m[1][0], G.m[2][0]);
+          V3<R> f1 = v3(G.m[0][1], R(1) + G.m[1][1], G.m[2][1]);
+          V3<R> f2 = v3(G.m[0][2], G.m[1][2], R(1) + G.m[2][2]);
+          V3<R> p0 = col(dF, 0), p1 = col(dF, 1), p2 = col(dF, 2);
+          M3<R> Hs;
+          set_col(Hs, 0, cross(f1, p2) - cross(f2, p1));
+          set_col(Hs, 1, cross(f2, p0) - cross(f0, p2));
+          set_col(Hs, 2, cross(f0, p1) - cross(f1, p0));
+          M3<R> T;
+          const R lc = lam * cd;
+#pragma unroll
+          for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) T.m[i][j] = lc * C.m[i][j] + mu * dF.m[i][j] + s * Hs.m[i][j];
+#pragma unroll
+          for (int a = 0; a < 4; ++a) o.hp[a] = g.dV * mul(T, g.d[a]);
+        }
+        if constexpr ((WHAT & W_MIX) != 0) {
+          const M3<R> D = F_minus_cofF(G, I.trG);
+          o.dmu = g.dV * ddot(D, dF);
+          o.dlam = g.dV * I.Jm1 * cd;
+        }
+      }
+    }
+  } else {  // ARAP
+    const Svd<R> S = svd_rv(G);
+    if constexpr ((WHAT & W_E) != 0)
+      o.e = g.dV * R(0.5) * mu * (S.sm1[0] * S.sm1[0] + S.sm1[1] * S.sm1[1] + S.sm1[2] * S.sm1[2]);
+    if constexpr ((WHAT & W_G) != 0) {
+      R d[3] = {mu * S.sm1[0], mu * S.sm1[1], mu * S.sm1[2]};
+      const M3<R> P = udv(S, d);
+#pragma unroll
+      for (int a = 0; a < 4; ++a) o.g[a] = g.dV * mul(P, g.d[a]);
+    }
+    R lk[3];
+    if constexpr ((WHAT & (W_H | W_HP | W_HQ)) != 0) arap_lambdas(S.s, lk);
+    if constexpr ((WHAT & W_H) != 0) {
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        V3<R> qd[3];
+        twist_apply(S, g.d[a], qd);
+        R t = dot(g.d[a], g.d[a]);
+        R hx = g.dV * mu * (t - (lk[0] * qd[0].x * qd[0].x + lk[1] * qd[1].x * qd[1].x + lk[2] * qd[2].x * qd[2].x));
+        R hy = g.dV * mu * (t - (lk[0] * qd[0].y * qd[0].y + lk[1] * qd[1].y * qd[1].y + lk[2] * qd[2].y * qd[2].y));
+        R hz = g.dV * mu * (t - (lk[0] * qd[0].z * qd[0].z + lk[1] * qd[1].z * qd[1].z + lk[2] * qd[2].z * qd[2].z));
+        o.h[a] = v3(hx > R(0) ? hx : R(0), hy > R(0) ? hy : R(0), hz > R(0) ? hz : R(0));
+      }
+    }
+    if constexpr ((WHAT & (W_HP | W_HQ | W_MIX)) != 0) {
+      R q[3], dM[3];
+      twist_coords(S, dF, q, dM);
+      if constexpr ((WHAT & W_HQ) != 0) {
+        R val = g.dV * mu * (ddot(dF, dF) - (lk[0] * q[0] * q[0] + lk[1] * q[1] * q[1] + lk[2] * q[2] * q[2]));
+        o.hq = val > R(0) ? val : R(0);
+      }
+      if constexpr ((WHAT & W_HP) != 0) {
+        R c[3] = {lk[0] * q[0], lk[1] * q[1], lk[2] * q[2]};
+        M3<R> Qc = twist_combine(S, c);
+        M3<R> T;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) T.m[i][j] = mu * (dF.m[i][j] - Qc.m[i][j]);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) o.hp[a] = g.dV * mul(T, g.d[a]);
+      }
+      if constexpr ((WHAT & W_MIX) != 0) {
+        o.dmu = g.dV * (S.sm1[0] * dM[0] + S.sm1[1] * dM[1] + S.sm1[2] * dM[2]);
+        o.dlam = R(0);
+      }
+    }
+  }
+}
+
+}  // namespace apl

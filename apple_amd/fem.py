@@ -1,0 +1,222 @@
+"""FEM energy potentials with liblaf/apple's interface, on the HIP kernels.
+
+``StableNeoHookean`` and ``Arap`` keep the methods of
+``WarpPotentialFem`` (warp/fem/_base.py:39-194) and the protocol of
+``WarpPotential`` (warp/model/_potential.py:9-32):
+
+    fun(u, output)            output (1,)  += sum_t Psi_t dV_t
+    grad(u, output)           output (V,3) += dE/du
+    hess_diag(u, output)      output (V,3) += max(dV diag H_e, 0) scattered
+    hess_prod(u, p, output)   output (V,3) += H p            (unclamped)
+    hess_quad(u, p, output)   output (1,)  += sum_t max(dV p_e^T H_e p_e, 0)
+
+with the reference's accumulate (``+=``) semantics: the caller zeroes the
+output first (``Model`` does).  ``u``/``p`` are (V,3) device tensors in the
+potential's dtype; scalar outputs may be float64 or the potential dtype.
+Additionally ``fused`` evaluates energy, gradient and Hessian diagonal in one
+pass, and ``mixed_derivative_prod`` returns p^T d(grad)/d(material) per tet
+(the adjoint's mixed term).
+
+Differences from the reference, by design:
+
+* ``Arap.hess_prod`` returns H p.  The reference passes (dhdX, p) to
+  functions taking (p, dhdX) (warp/fem/_arap.py:55-56 vs
+  func/_hess_prod.py:38-39,55), so its result is not H p.
+* Reductions are deterministic (fixed order) instead of float atomics.
+"""
+
+from __future__ import annotations
+
+from collections.abc import Sequence
+from typing import Any
+
+import numpy as np
+import torch
+
+from . import _lib
+from .common import FRACTION, LAMBDA, MU, default_name
+from .mesh import Mesh, Region
+from .runtime import DeviceMesh, as_dtype
+
+
+def _fraction(region: Region) -> np.ndarray:
+    f = region.cell_data.get(FRACTION.vtk)
+    if f is None:
+        return np.ones(region.n_cells)
+    return np.asarray(f, np.float64)
+
+
+class FemPotential:
+    """Base class: one material law over one tet region (a WarpPotentialFem)."""
+
+    material: int = -1
+    material_names: tuple[str, ...] = ()
+
+    def __init__(
+        self,
+        mesh: DeviceMesh,
+        materials: dict[str, torch.Tensor],
+        *,
+        name: str | None = None,
+        requires_grad: Sequence[str] = (),
+        region: Region | None = None,
+    ):
+        self.mesh = mesh
+        self.materials = materials
+        self.name = name if name is not None else default_name(self)
+        self.requires_grad = tuple(requires_grad)
+        self.region = region
+        self._energy = torch.zeros(1, dtype=torch.float64, device=mesh.device)
+
+    # ------------------------------------------------------------ construction
+
+    @classmethod
+    def from_region(
+        cls,
+        region: Region,
+        *,
+        requires_grad: Sequence[str] = (),
+        name: str | None = None,
+        dtype=torch.float32,
+        device=None,
+        chunks: int = 1,
+    ):
+        """Upload the region (dhdX -> inverse rest shapes, dV * Fraction) -- _base.py:94-111."""
+        dtype = as_dtype(dtype)
+        dV = _fraction(region) * region.dV
+        mesh = DeviceMesh(
+            region.cells, region.dminv, dV, region.mesh.n_points, dtype=dtype, device=device, chunks=chunks
+        )
+        materials = cls.materials_from_region(region, mesh)
+        return cls(mesh, materials, name=name, requires_grad=requires_grad, region=region)
+
+    @classmethod
+    def from_pyvista(cls, obj: Any, **kwargs):
+        return cls.from_region(Region.from_mesh(Mesh.from_pyvista(obj)), **kwargs)
+
+    from_mesh = from_pyvista
+
+    @classmethod
+    def materials_from_region(cls, region: Region, mesh: DeviceMesh) -> dict[str, torch.Tensor]:
+        raise NotImplementedError
+
+    @staticmethod
+    def _cell_tensor(region: Region, key: str, mesh: DeviceMesh) -> torch.Tensor:
+        if key not in region.cell_data:
+            raise KeyError(f"cell_data[{key!r}] is required")
+        arr = np.asarray(region.cell_data[key], np.float64).reshape(-1)
+        if arr.shape[0] != region.n_cells:
+            raise ValueError(f"cell_data[{key!r}] must have one value per cell")
+        return torch.as_tensor(arr, dtype=mesh.dtype, device=mesh.device).contiguous()
+
+    def set_materials(self, **values: torch.Tensor) -> None:
+        for k, v in values.items():
+            if k not in self.material_names:
+                raise KeyError(f"{type(self).__name__} has no material {k!r}")
+            v = torch.as_tensor(v, dtype=self.mesh.dtype, device=self.mesh.device).reshape(-1).contiguous()
+            if v.shape[0] != self.mesh.n_tets:
+                raise ValueError(f"{k} must have one value per tet")
+            self.materials[k] = v
+
+    # ------------------------------------------------------------ properties
+
+    @property
+    def n_points(self) -> int:
+        return self.mesh.n_verts
+
+    @property
+    def n_cells(self) -> int:
+        return self.mesh.n_tets
+
+    @property
+    def dtype(self) -> torch.dtype:
+        return self.mesh.dtype
+
+    @property
+    def device(self) -> torch.device:
+        return self.mesh.device
+
+    @property
+    def launch_dim(self) -> tuple[int, int]:
+        return (self.mesh.n_tets, 1)
+
+    def _mu(self):
+        return self.materials["mu"]
+
+    def _lam(self):
+        return self.materials.get("lambda")
+
+    # ------------------------------------------------------------ reference protocol
+
+    def _add_scalar(self, output: torch.Tensor, which: str, u, p=None) -> None:
+        if output.dtype == torch.float64 and output.is_contiguous() and output.numel() == 1:
+            kw = {which: output}
+            self.mesh.eval(self.material, self._mu(), self._lam(), u, p, accumulate=True, **kw)
+            return
+        self._energy.zero_()
+        kw = {which: self._energy}
+        self.mesh.eval(self.material, self._mu(), self._lam(), u, p, accumulate=True, **kw)
+        output += self._energy.to(output.dtype)
+
+    def fun(self, u: torch.Tensor, output: torch.Tensor) -> None:
+        self._add_scalar(output, "energy", u)
+
+    def grad(self, u: torch.Tensor, output: torch.Tensor) -> None:
+        self.mesh.eval(self.material, self._mu(), self._lam(), u, grad=output, accumulate=True)
+
+    def hess_diag(self, u: torch.Tensor, output: torch.Tensor) -> None:
+        self.mesh.eval(self.material, self._mu(), self._lam(), u, hess_diag=output, accumulate=True)
+
+    def hess_prod(self, u: torch.Tensor, p: torch.Tensor, output: torch.Tensor) -> None:
+        self.mesh.eval(self.material, self._mu(), self._lam(), u, p, hess_prod=output, accumulate=True)
+
+    def hess_quad(self, u: torch.Tensor, p: torch.Tensor, output: torch.Tensor) -> None:
+        self._add_scalar(output, "hess_quad", u, p)
+
+    # ------------------------------------------------------------ fused / adjoint
+
+    def fused(self, u, energy=None, grad=None, hess_diag=None, *, accumulate=True) -> None:
+        """Energy (float64 (1,)), gradient and Hessian diagonal in one element pass."""
+        self.mesh.eval(
+            self.material, self._mu(), self._lam(), u, energy=energy, grad=grad, hess_diag=hess_diag,
+            accumulate=accumulate,
+        )
+
+    def mixed_derivative_prod(self, u: torch.Tensor, p: torch.Tensor) -> dict[str, torch.Tensor]:
+        """Per-tet p^T d(grad)/d(mu) (and d/d(lambda) for SNH)."""
+        dmu = torch.empty(self.n_cells, dtype=self.dtype, device=self.device)
+        dlam = torch.empty_like(dmu) if self.material == _lib.APL_SNH else None
+        self.mesh.eval(self.material, self._mu(), self._lam(), u, p, dmu=dmu, dlam=dlam)
+        out = {"mu": dmu}
+        if dlam is not None:
+            out["lambda"] = dlam
+        return out
+
+
+WarpPotentialFem = FemPotential
+
+
+class StableNeoHookean(FemPotential):
+    """Stable Neo-Hookean (warp/fem/_stable_neo_hookean.py:17-154):
+    Psi = mu/2 (|F|^2 - 3) - mu (J - 1) + lam/2 (J - 1)^2."""
+
+    material = _lib.APL_SNH
+    material_names = ("mu", "lambda")
+
+    @classmethod
+    def materials_from_region(cls, region, mesh):
+        return {
+            "lambda": cls._cell_tensor(region, LAMBDA.vtk, mesh),
+            "mu": cls._cell_tensor(region, MU.vtk, mesh),
+        }
+
+
+class Arap(FemPotential):
+    """As-rigid-as-possible (warp/fem/_arap.py:17-125): Psi = mu/2 |F - R|^2."""
+
+    material = _lib.APL_ARAP
+    material_names = ("mu",)
+
+    @classmethod
+    def materials_from_region(cls, region, mesh):
+        return {"mu": cls._cell_tensor(region, MU.vtk, mesh)}

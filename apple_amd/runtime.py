@@ -1,0 +1,205 @@
+"""Device-side mesh handle and tensor plumbing around the C ABI.
+
+PyTorch-ROCm provides device memory and streams; every computation on the
+hot path runs in the HIP kernels of ``libapple_amd.so``.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+
+_DT = {torch.float32: _lib.APL_F32, torch.float64: _lib.APL_F64}
+
+
+def as_dtype(dtype) -> torch.dtype:
+    if dtype in (torch.float32, "float32", np.float32, "f32"):
+        return torch.float32
+    if dtype in (torch.float64, "float64", np.float64, "f64"):
+        return torch.float64
+    raise ValueError(f"unsupported dtype {dtype!r}: use float32 or float64")
+
+
+def ptr(t: torch.Tensor | None, *, dtype=None, device=None, shape=None, name="tensor") -> int | None:
+    """Raw device pointer of a contiguous tensor, after checking it."""
+    if t is None:
+        return None
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name} must be a torch.Tensor")
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be a device (cuda) tensor")
+    if device is not None and t.device != device:
+        raise ValueError(f"{name} is on {t.device}, expected {device}")
+    if dtype is not None and t.dtype != dtype:
+        raise TypeError(f"{name} has dtype {t.dtype}, expected {dtype}")
+    if shape is not None and tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name} has shape {tuple(t.shape)}, expected {tuple(shape)}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    return t.data_ptr()
+
+
+def current_stream(device: torch.device) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+class DeviceMesh:
+    """Owner of an ``apl_mesh`` handle: element data and the block partition.
+
+    ``cells`` (T,4) int, ``dminv`` (T,3,3) = inverse of dX/dr, ``dV`` (T,),
+    all host arrays; ``inst_tet_off`` / ``inst_vtx_off`` describe a batch of
+    independent meshes concatenated into one (tet and vertex ranges).
+    """
+
+    def __init__(
+        self,
+        cells,
+        dminv,
+        dV,
+        n_verts: int,
+        *,
+        dtype=torch.float32,
+        device=None,
+        inst_tet_off=None,
+        inst_vtx_off=None,
+        chunks: int = 1,
+    ):
+        self.dtype = as_dtype(dtype)
+        self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        if self.device.type != "cuda":
+            raise ValueError("apple_amd runs on a ROCm device; got " + str(self.device))
+        cells = np.ascontiguousarray(cells, dtype=np.int32)
+        dminv = np.ascontiguousarray(dminv, dtype=np.float64).reshape(-1, 9)
+        dV = np.ascontiguousarray(dV, dtype=np.float64).reshape(-1)
+        T = cells.shape[0]
+        if cells.shape != (T, 4) or dminv.shape[0] != T or dV.shape[0] != T:
+            raise ValueError("cells (T,4), dminv (T,3,3) and dV (T,) must agree on T")
+        self.n_tets = T
+        self.n_verts = int(n_verts)
+        if inst_tet_off is None:
+            self.n_inst = 1
+            self.inst_tet_off = np.array([0, T], np.int64)
+            self.inst_vtx_off = np.array([0, self.n_verts], np.int64)
+        else:
+            self.inst_tet_off = np.ascontiguousarray(inst_tet_off, dtype=np.int64)
+            self.inst_vtx_off = np.ascontiguousarray(inst_vtx_off, dtype=np.int64)
+            self.n_inst = self.inst_tet_off.size - 1
+        self._handle = ctypes.c_void_p()
+        L = _lib.lib()
+        _lib.check(
+            L.apl_mesh_create(
+                _DT[self.dtype],
+                T,
+                self.n_verts,
+                cells.ctypes.data,
+                dminv.ctypes.data,
+                dV.ctypes.data,
+                self.n_inst,
+                self.inst_tet_off.ctypes.data,
+                self.inst_vtx_off.ctypes.data,
+                int(chunks),
+                self.device.index if self.device.index is not None else torch.cuda.current_device(),
+                ctypes.byref(self._handle),
+            )
+        )
+        self._scratch = torch.zeros(max(self.n_inst, 1) * 2, dtype=torch.float64, device=self.device)
+
+    @property
+    def handle(self) -> int:
+        return self._handle.value
+
+    def info(self) -> dict:
+        inf = _lib.MeshInfo()
+        _lib.check(_lib.lib().apl_mesh_info(self._handle, ctypes.byref(inf)))
+        return {k: getattr(inf, k) for k, _ in _lib.MeshInfo._fields_}
+
+    def close(self) -> None:
+        if getattr(self, "_handle", None) is not None and self._handle.value:
+            _lib.lib().apl_mesh_destroy(self._handle)
+            self._handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------ evaluation
+
+    def vec(self, name="u"):
+        return dict(dtype=self.dtype, device=self.device, shape=(self.n_verts, 3), name=name)
+
+    def eval(
+        self,
+        material: int,
+        mu,
+        lam,
+        u,
+        p=None,
+        *,
+        energy=None,
+        grad=None,
+        hess_diag=None,
+        hess_prod=None,
+        hess_quad=None,
+        dmu=None,
+        dlam=None,
+        accumulate=False,
+    ) -> None:
+        """One fused element pass writing (or adding into) the given outputs."""
+        what = 0
+        if energy is not None:
+            what |= _lib.APL_ENERGY
+        if grad is not None:
+            what |= _lib.APL_GRAD
+        if hess_diag is not None:
+            what |= _lib.APL_HESS_DIAG
+        if hess_prod is not None:
+            what |= _lib.APL_HESS_PROD
+        if hess_quad is not None:
+            what |= _lib.APL_HESS_QUAD
+        if dmu is not None:
+            what |= _lib.APL_MIXED
+        if accumulate:
+            what |= _lib.APL_ACCUMULATE
+        tet = dict(dtype=self.dtype, device=self.device, shape=(self.n_tets,))
+        sc = dict(dtype=torch.float64, device=self.device, shape=(self.n_inst,))
+        _lib.check(
+            _lib.lib().apl_elem_eval(
+                self._handle,
+                int(material),
+                ptr(mu, name="mu", **tet),
+                ptr(lam, name="lambda", **tet) if material == _lib.APL_SNH else None,
+                ptr(u, **self.vec("u")),
+                ptr(p, **self.vec("p")),
+                what,
+                ptr(energy, name="energy", **sc),
+                ptr(grad, **self.vec("grad")),
+                ptr(hess_diag, **self.vec("hess_diag")),
+                ptr(hess_prod, **self.vec("hess_prod")),
+                ptr(hess_quad, name="hess_quad", **sc),
+                ptr(dmu, name="dmu", **tet),
+                ptr(dlam, name="dlam", **tet),
+                current_stream(self.device),
+            )
+        )
+
+    def ext_force(self, indices, force, u, *, energy=None, grad=None, accumulate=True) -> None:
+        n = int(indices.shape[0])
+        _lib.check(
+            _lib.lib().apl_ext_force_eval(
+                self._handle,
+                n,
+                ptr(indices, dtype=torch.int32, device=self.device, name="indices"),
+                ptr(force, dtype=self.dtype, device=self.device, shape=(n, 3), name="force"),
+                ptr(u, **self.vec("u")) if u is not None else None,
+                ptr(energy, dtype=torch.float64, device=self.device, shape=(1,), name="energy"),
+                ptr(grad, **self.vec("grad")),
+                1 if accumulate else 0,
+                current_stream(self.device),
+            )
+        )

@@ -1,0 +1,95 @@
+/* apple_amd -- C ABI of the MI355X (gfx950) FEM elasticity hot path.
+ *
+ * The drop-in boundary for liblaf/apple's element potentials and static
+ * solver.  Every array argument is a caller-owned device pointer (e.g. a
+ * PyTorch-ROCm tensor's data_ptr()) unless marked "host".  Real-valued arrays
+ * are float32 or float64 according to the dtype the mesh was created with;
+ * energies, quadratic forms and solver scalars are always float64.  Vertex
+ * vectors are (V,3) row-major, per-tet arrays (T,), both in the caller's
+ * order.  All work is enqueued on `stream` (a hipStream_t, NULL = default
+ * stream); a mesh or solver handle must be used by one stream at a time.
+ * Functions return 0 on success and -1 on error; apl_last_error() then
+ * describes the error (thread-local).  No entry point allocates device memory
+ * except the *_create functions.
+ */
+#ifndef APPLE_AMD_H
+#define APPLE_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct apl_mesh apl_mesh;
+typedef struct apl_solver apl_solver;
+
+enum { APL_F32 = 0, APL_F64 = 1 };
+/* materials: warp/fem/_stable_neo_hookean.py, warp/fem/_arap.py */
+enum { APL_SNH = 0, APL_ARAP = 1 };
+
+/* output selection for apl_elem_eval */
+enum {
+  APL_ENERGY = 1,      /* WarpPotentialFem.fun        (warp/fem/_base.py:151-158, 243-263) */
+  APL_GRAD = 2,        /* WarpPotentialFem.grad       (_base.py:160-167, 265-291) */
+  APL_HESS_DIAG = 4,   /* WarpPotentialFem.hess_diag  (_base.py:169-176, 293-324) */
+  APL_HESS_PROD = 8,   /* WarpPotentialFem.hess_prod  (_base.py:178-185, 326-351) */
+  APL_HESS_QUAD = 16,  /* WarpPotentialFem.hess_quad  (_base.py:187-194, 353-383) */
+  APL_MIXED = 32,      /* p^T d(grad)/d(mu), p^T d(grad)/d(lambda) per tet: the
+                          mixed_derivative_prod of the adjoint
+                          (exp/2025/08/27/inverse/src/30-inverse.py:96-97) */
+  APL_ACCUMULATE = 1 << 16 /* add into the outputs (the reference's += semantics,
+                              warp/model/_model.py:13-36) instead of overwriting */
+};
+
+/* solver result codes (liblaf-peach Result names as used by apple:
+ * tests/forward/test_static_simulation.py:85) */
+enum {
+  APL_RUNNING = 0,
+  APL_PRIMARY_SUCCESS = 1,
+  APL_MAX_STEPS_REACHED = 2,
+  APL_NON_FINITE = 3,
+  APL_STAGNATION = 4
+};
+
+const char* apl_last_error(void);
+int apl_version(void);
+
+/* Setup: WarpPotentialFem.from_region (warp/fem/_base.py:94-111) plus the
+ * block partition of the deterministic reduction.  cells (host, T x 4),
+ * dminv (host, T x 3 x 3: inverse of dX/dr, i.e. rows 1..3 of the region's
+ * dhdX, jax/fem/region/_region.py:84-108), dv (host, T: dV including
+ * Fraction).  A batch of independent meshes is one mesh whose instances own
+ * the tet ranges inst_tet_off[i]..inst_tet_off[i+1] and vertex ranges
+ * inst_vtx_off[i]..; pass n_inst = 1 and NULL offsets for a single mesh.
+ * chunks (>= 1) is the number of 256-tet chunks one block may span. */
+int apl_mesh_create(int dtype, int64_t n_tets, int64_t n_verts, const int32_t* cells, const double* dminv,
+                    const double* dv, int64_t n_inst, const int64_t* inst_tet_off, const int64_t* inst_vtx_off,
+                    int chunks, int device, apl_mesh** out);
+int apl_mesh_destroy(apl_mesh* mesh);
+
+typedef struct {
+  int64_t n_tets, n_verts, n_inst, n_blocks, n_partials, n_shared_verts, n_vertex_blocks;
+  int32_t nloc_max, chunks, dtype, elem_grid_snh, elem_grid_arap;
+  int64_t device_bytes;
+} apl_mesh_info_t;
+int apl_mesh_info(const apl_mesh* mesh, apl_mesh_info_t* info /* host */);
+
+/* One fused element pass (replaces the five launches of warp/fem/_base.py:151-194).
+ * u, p: (V,3); mu, lam: (T,) (lam ignored for ARAP); energy, hess_quad:
+ * (n_inst,) float64; grad, hess_diag, hess_prod: (V,3); dmu, dlam: (T,).
+ * Unused outputs may be NULL. */
+int apl_elem_eval(apl_mesh* mesh, int material, const void* mu, const void* lam, const void* u, const void* p,
+                  unsigned what, double* energy, void* grad, void* hess_diag, void* hess_prod, double* hess_quad,
+                  void* dmu, void* dlam, void* stream);
+
+/* ExternalForce (warp/potential/_ext_force.py:17-39): E -= f . u[idx],
+ * grad[idx] -= f for n entries with unique indices.  energy: (1,) float64. */
+int apl_ext_force_eval(apl_mesh* mesh, int64_t n, const int32_t* indices, const void* force, const void* u,
+                       double* energy, void* grad, int accumulate, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* APPLE_AMD_H */
