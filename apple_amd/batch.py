@@ -1,0 +1,102 @@
+"""Batched solves: many independent meshes of uneven sizes in one launch.
+
+The instances are concatenated into one device mesh whose element blocks
+never straddle an instance, so every launch of the PNCG (element passes,
+vector passes, per-instance scalar steps) covers the whole batch; an
+instance that has converged drops out of the following launches through its
+status flag.  Each instance keeps its own materials, loads, boundary
+conditions and per-instance FP64 solver scalars.
+"""
+
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+import torch
+
+from . import _lib
+from .mesh import Region
+from .runtime import DeviceMesh, as_dtype, normalize_device
+from .solver import DeviceSolver, PncgConfig, SolveStats, mask_bits
+
+
+@dataclasses.dataclass
+class Instance:
+    region: Region
+    mu: np.ndarray
+    lam: np.ndarray | None = None
+    f_ext: np.ndarray | None = None  # (V,3)
+    fixed_mask: np.ndarray | None = None  # (V,3) bool
+    x0: np.ndarray | None = None  # (V,3) initial guess incl. fixed values
+
+
+class _BatchPotential:
+    """Minimal potential view (materials + mesh) for the device solver."""
+
+    def __init__(self, mesh, material, mu, lam):
+        self.mesh = mesh
+        self.material = material
+        self.materials = {"mu": mu}
+        if lam is not None:
+            self.materials["lambda"] = lam
+
+
+class Batch:
+    def __init__(self, instances: list[Instance], material: int, *, dtype=torch.float32, device=None, chunks=1):
+        self.instances = instances
+        self.material = int(material)
+        self.dtype = as_dtype(dtype)
+        self.device = normalize_device(device)
+        nv = [inst.region.mesh.n_points for inst in instances]
+        nt = [inst.region.n_cells for inst in instances]
+        self.vtx_off = np.concatenate([[0], np.cumsum(nv)]).astype(np.int64)
+        self.tet_off = np.concatenate([[0], np.cumsum(nt)]).astype(np.int64)
+        cells = np.concatenate([inst.region.cells + self.vtx_off[i] for i, inst in enumerate(instances)])
+        dminv = np.concatenate([inst.region.dminv for inst in instances])
+        dV = np.concatenate([inst.region.dV for inst in instances])
+        self.mesh = DeviceMesh(cells, dminv, dV, int(self.vtx_off[-1]), dtype=self.dtype, device=self.device,
+                               inst_tet_off=self.tet_off, inst_vtx_off=self.vtx_off, chunks=chunks)
+        t = lambda a: torch.as_tensor(np.concatenate(a), dtype=self.dtype, device=self.device).contiguous()
+        self.mu = t([np.asarray(i.mu, np.float64) for i in instances])
+        self.lam = (
+            t([np.asarray(i.lam, np.float64) for i in instances]) if self.material == _lib.APL_SNH else None
+        )
+        V = int(self.vtx_off[-1])
+        f = np.zeros((V, 3))
+        fm = np.zeros((V, 3), bool)
+        x0 = np.zeros((V, 3))
+        for i, inst in enumerate(instances):
+            sl = slice(self.vtx_off[i], self.vtx_off[i + 1])
+            if inst.f_ext is not None:
+                f[sl] = inst.f_ext
+            if inst.fixed_mask is not None:
+                fm[sl] = inst.fixed_mask
+            if inst.x0 is not None:
+                x0[sl] = inst.x0
+        self.f_ext = torch.as_tensor(f, dtype=self.dtype, device=self.device).contiguous()
+        self.mask = mask_bits(fm, self.device)
+        self.x0 = torch.as_tensor(x0, dtype=self.dtype, device=self.device).contiguous()
+        self.potential = _BatchPotential(self.mesh, self.material, self.mu, self.lam)
+        self._solver = None
+
+    @property
+    def n_inst(self) -> int:
+        return len(self.instances)
+
+    @property
+    def n_tets(self) -> int:
+        return int(self.tet_off[-1])
+
+    def solver(self) -> DeviceSolver:
+        if self._solver is None:
+            self._solver = DeviceSolver(self.potential)
+        return self._solver
+
+    def solve(self, x: torch.Tensor | None = None, config: PncgConfig | None = None) -> tuple[torch.Tensor, list[SolveStats]]:
+        x = self.x0.clone() if x is None else x
+        stats = self.solver().solve(x, self.f_ext, self.mask, config)
+        return x, stats
+
+    def split(self, x: torch.Tensor) -> list[torch.Tensor]:
+        return [x[self.vtx_off[i] : self.vtx_off[i + 1]] for i in range(self.n_inst)]

@@ -224,16 +224,18 @@ int apl_elem_eval(apl_mesh* m, int material, const void* mu, const void* lam, co
   const unsigned w = to_internal(what);
   const int acc = (what & APL_ACCUMULATE) ? 1 : 0;
   if (w == 0) return 0;
-  if (mu == nullptr || u == nullptr || (material == APL_SNH && lam == nullptr)) {
+  if (m->L.T > 0 && (mu == nullptr || u == nullptr || (material == APL_SNH && lam == nullptr))) {
     set_error("mu, lam (SNH) and u must be non-NULL");
     return -1;
   }
-  if ((w & (W_HP | W_HQ | W_MIX)) && p == nullptr) {
+  if (m->L.T > 0 && (w & (W_HP | W_HQ | W_MIX)) && p == nullptr) {
     set_error("p must be non-NULL for hess_prod, hess_quad and mixed derivatives");
     return -1;
   }
-  if (((w & W_E) && !energy) || ((w & W_G) && !grad) || ((w & W_H) && !hess_diag) || ((w & W_HP) && !hess_prod) ||
-      ((w & W_HQ) && !hess_quad) || ((w & W_MIX) && (!dmu || (material == APL_SNH && !dlam)))) {
+  const bool has_v = m->L.V > 0, has_t = m->L.T > 0;
+  if (((w & W_E) && !energy) || ((w & W_G) && has_v && !grad) || ((w & W_H) && has_v && !hess_diag) ||
+      ((w & W_HP) && has_v && !hess_prod) || ((w & W_HQ) && !hess_quad) ||
+      ((w & W_MIX) && has_t && (!dmu || (material == APL_SNH && !dlam)))) {
     set_error("a requested output pointer is NULL");
     return -1;
   }

@@ -378,6 +378,9 @@ static size_t smem_bytes(const apl_mesh* m) {
 
 template <int MAT, unsigned WHAT, class R>
 static int grid_for(apl_mesh* m) {
+  static_assert(WHAT < 64, "pass variant index");
+  int& cached = m->grid_cache[MAT][WHAT];
+  if (cached > 0) return cached;
   const size_t smem = smem_bytes<MAT, WHAT, R>(m);
   int occ = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, elem_kernel<MAT, WHAT, R>, NTH, smem) != hipSuccess ||
@@ -388,7 +391,8 @@ static int grid_for(apl_mesh* m) {
   if (g > nb8) g = nb8;
   g = ((g + 7) / 8) * 8;
   if (g < 8) g = 8;
-  return (int)g;
+  cached = (int)g;
+  return cached;
 }
 
 template <int MAT, unsigned WHAT, class R>
@@ -447,12 +451,17 @@ int launch_elem(apl_mesh* m, int material, unsigned what, const void* mu, const 
   return -1;
 }
 
-int elem_grid(apl_mesh* m, int material, unsigned what) {
-  if (material == MAT_SNH && what == (W_E | W_G | W_H))
-    return m->dtype == DT_F32 ? grid_for<MAT_SNH, W_E | W_G | W_H, float>(m) : grid_for<MAT_SNH, W_E | W_G | W_H, double>(m);
-  if (material == MAT_ARAP && what == (W_E | W_G | W_H))
-    return m->dtype == DT_F32 ? grid_for<MAT_ARAP, W_E | W_G | W_H, float>(m) : grid_for<MAT_ARAP, W_E | W_G | W_H, double>(m);
+template <int MAT>
+static int grid_mat(apl_mesh* m, unsigned what) {
+#define APL_CASE(W) \
+  if (what == (W)) return m->dtype == DT_F32 ? grid_for<MAT, (W), float>(m) : grid_for<MAT, (W), double>(m);
+  APL_WHAT_LIST(APL_CASE)
+#undef APL_CASE
   return 0;
+}
+
+int elem_grid(apl_mesh* m, int material, unsigned what) {
+  return material == MAT_SNH ? grid_mat<MAT_SNH>(m, what) : grid_mat<MAT_ARAP>(m, what);
 }
 
 template <class R, int NCH>

@@ -65,6 +65,7 @@ struct apl_mesh {
   apl::Layout L;
   apl::DevLayout d;
   size_t device_bytes = 0;
+  int grid_cache[2][64] = {};  // persistent grid per (material, pass variant)
 };
 
 namespace apl {

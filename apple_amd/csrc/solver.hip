@@ -1,0 +1,910 @@
+// Device-resident PNCG static solver and PCG adjoint solver.
+//
+// PNCG restates the loop of liblaf-peach as apple states it in
+// benches/bench_pncg_branching_backends.py:254-330,393-456,606-679 (the
+// external peach package itself is not vendored): Jacobi preconditioner
+// P = 1/|diag H| with the mean of the positive entries for zeros, Dai-Kou
+// beta reset when non-finite or |beta| > beta_max, steepest-descent fallback
+// when g.p >= 0, alpha = -g.p / p^T H p (x overstep), halving backtracking,
+// stop at ||g|| <= rtol ||g_0|| (or atol).  The sufficient-decrease test is
+// Armijo OR the approximate Wolfe conditions of Hager & Zhang (2005): energy
+// differences stop resolving the decrease long before ||g|| reaches 1e-6
+// relative in FP32, the directional-derivative test does not.
+//
+// One iteration is six launches, all with device-side control (no host sync):
+//   A  element pass at the trial point x + alpha p: energy, gradient, Hdiag
+//   R  vertex pass: sums of shared-vertex partials, masks, FP64 partial dots
+//   S  per-instance scalar step: energy/dot totals, accept/reject, beta, gd
+//   D  vertex pass: commit x += alpha p, g_cur = g, p = -P g + beta p
+//   B  element pass: p^T H p (clamped per tet)
+//   S2 per-instance: alpha = -gd / pHp
+// A rejected trial skips D/B/S2 work (flags) and the next A re-evaluates at
+// the halved step.  K iterations are captured in one HIP graph; the host
+// checks the per-instance status once per graph launch.  Every instance of a
+// batch runs in the same launches; finished instances drop out via flags.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/apple_amd.h"
+#include "internal.h"
+#include "tetmath.h"
+
+namespace apl {
+
+enum : int { D_G2 = 0, D_GPG, D_GPG0, D_GPGC, D_GPGC0, D_GCPGC, D_GCPGC0, D_GTP, D_GCP, D_HSUM, D_HCNT, D_EXT, D_PP };
+static_assert(D_PP < N_DOTS, "dot slots");
+
+struct StatePtrs {
+  double *f, *gd, *alpha, *alpha_commit, *beta, *gnorm0, *gnorm, *pmean, *f_first, *pHp;
+  int32_t *status, *act_eval, *act_dir, *commit, *n_steps, *n_evals, *ls_step, *n_backtracks, *phase, *restarts;
+};
+constexpr int N_SD = 10, N_SI = 10;
+
+struct Params {
+  double rtol, atol, overstep, c1, wolfe_delta, wolfe_eps, beta_max;
+  int max_steps, max_backtracking;
+};
+
+template <int N>
+__device__ inline void block_reduce_n(double (&v)[N], double* sm, double* out) {
+#pragma unroll
+  for (int k = 0; k < N; ++k)
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < N; ++k) sm[w * N + k] = v[k];
+  __syncthreads();
+  if ((int)threadIdx.x < N) {
+    double s = 0.0;
+#pragma unroll
+    for (int ww = 0; ww < NTH / 64; ++ww) s += sm[ww * N + threadIdx.x];
+    out[threadIdx.x] = s;
+  }
+  __syncthreads();
+}
+
+template <class R>
+struct VecArgs {
+  const int32_t* vblk_off;
+  const int32_t* vblk_inst;
+  const int32_t* vtx_sh_off;
+  const int32_t* sh_src;
+  const R* partial;
+  long long n_partials;
+  const uint8_t* mask;
+  const R* fext;
+  R* x;
+  R* p;
+  R* gc;
+  R* gE;
+  R* hd;
+  double* vblk_dots;
+  StatePtrs st;
+};
+
+template <class R>
+__global__ __launch_bounds__(NTH) void fill_kernel(R* a, long long n, R v) {
+  for (long long i = (long long)blockIdx.x * NTH + threadIdx.x; i < n; i += (long long)gridDim.x * NTH) a[i] = v;
+}
+
+__global__ void pncg_init_kernel(StatePtrs st, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  st.f[i] = 0.0;
+  st.gd[i] = 0.0;
+  st.alpha[i] = 0.0;
+  st.alpha_commit[i] = 0.0;
+  st.beta[i] = 0.0;
+  st.gnorm0[i] = 0.0;
+  st.gnorm[i] = 0.0;
+  st.pmean[i] = 1.0;
+  st.f_first[i] = 0.0;
+  st.pHp[i] = 0.0;
+  st.status[i] = APL_RUNNING;
+  st.act_eval[i] = 1;
+  st.act_dir[i] = 0;
+  st.commit[i] = 0;
+  st.n_steps[i] = 0;
+  st.n_evals[i] = 0;
+  st.ls_step[i] = 0;
+  st.n_backtracks[i] = 0;
+  st.phase[i] = 0;
+  st.restarts[i] = 0;
+}
+
+// R: shared-vertex sums + FP64 partial dots of one vertex block
+template <class R>
+__global__ __launch_bounds__(NTH) void pncg_reduce_kernel(VecArgs<R> A) {
+  __shared__ double sm[(NTH / 64) * N_DOTS];
+  const int vb = blockIdx.x;
+  const int inst = A.vblk_inst[vb];
+  if (A.st.act_eval[inst] == 0) return;
+  const int v0 = A.vblk_off[vb], v1 = A.vblk_off[vb + 1];
+  const long long v = (long long)v0 + threadIdx.x;
+  const double alpha = A.st.alpha[inst];
+  double d[N_DOTS];
+#pragma unroll
+  for (int k = 0; k < N_DOTS; ++k) d[k] = 0.0;
+  if (v < v1) {
+    R ge[3], h[3];
+    const int o0 = A.vtx_sh_off[v], o1 = A.vtx_sh_off[v + 1];
+    if (o1 > o0) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        ge[c] = R(0);
+        h[c] = R(0);
+      }
+      for (int k = o0; k < o1; ++k) {
+        const long long pidx = A.sh_src[k];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          ge[c] += A.partial[c * A.n_partials + pidx];
+          h[c] += A.partial[(3 + c) * A.n_partials + pidx];
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        A.gE[3 * v + c] = ge[c];
+        A.hd[3 * v + c] = h[c];
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        ge[c] = A.gE[3 * v + c];
+        h[c] = A.hd[3 * v + c];
+      }
+    }
+    const unsigned msk = A.mask != nullptr ? A.mask[v] : 0u;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double f = A.fext != nullptr ? (double)A.fext[3 * v + c] : 0.0;
+      const double pc = (double)A.p[3 * v + c];
+      d[D_EXT] -= f * ((double)A.x[3 * v + c] + alpha * pc);
+      if (msk & (1u << c)) continue;
+      const double gt = (double)(ge[c] - (A.fext != nullptr ? A.fext[3 * v + c] : R(0)));
+      const double gcv = (double)A.gc[3 * v + c];
+      const double hh = (double)h[c];
+      d[D_G2] += gt * gt;
+      if (hh > 0.0) {
+        const double P = 1.0 / hh;
+        d[D_GPG] += gt * gt * P;
+        d[D_GPGC] += gt * gcv * P;
+        d[D_GCPGC] += gcv * gcv * P;
+        d[D_HSUM] += hh;
+        d[D_HCNT] += 1.0;
+      } else {
+        d[D_GPG0] += gt * gt;
+        d[D_GPGC0] += gt * gcv;
+        d[D_GCPGC0] += gcv * gcv;
+      }
+      d[D_GTP] += gt * pc;
+      d[D_GCP] += gcv * pc;
+      d[D_PP] += pc * pc;
+    }
+  }
+  block_reduce_n<N_DOTS>(d, sm, A.vblk_dots + (long long)vb * N_DOTS);
+}
+
+// S: per-instance decision
+__global__ __launch_bounds__(NTH) void pncg_scalar_kernel(StatePtrs st, Params P, const double* vblk_dots,
+                                                          const int64_t* inst_vblk_off, const double* blk_red,
+                                                          const int64_t* inst_blk_off) {
+  __shared__ double sm[(NTH / 64) * (N_DOTS + 1)];
+  __shared__ double tot[N_DOTS + 1];
+  const int i = blockIdx.x;
+  if (st.act_eval[i] == 0) return;
+  double acc[N_DOTS + 1];
+#pragma unroll
+  for (int k = 0; k <= N_DOTS; ++k) acc[k] = 0.0;
+  for (long long vb = inst_vblk_off[i] + threadIdx.x; vb < inst_vblk_off[i + 1]; vb += NTH)
+#pragma unroll
+    for (int k = 0; k < N_DOTS; ++k) acc[k] += vblk_dots[vb * N_DOTS + k];
+  for (long long b = inst_blk_off[i] + threadIdx.x; b < inst_blk_off[i + 1]; b += NTH) acc[N_DOTS] += blk_red[2 * b];
+  block_reduce_n<N_DOTS + 1>(acc, sm, tot);
+  if (threadIdx.x != 0) return;
+  const double fE = tot[N_DOTS];
+  const double ft = fE + tot[D_EXT];
+  const double gn = sqrt(tot[D_G2]);
+  const int phase = st.phase[i];
+  st.n_evals[i] += 1;
+  bool accept;
+  if (phase != 1) {
+    accept = true;
+  } else {
+    const double f0 = st.f[i], a = st.alpha[i], gd = st.gd[i];
+    const bool fin = isfinite(ft);
+    const bool armijo = fin && ft <= f0 + P.c1 * a * gd;
+    const bool wolfe = P.wolfe_delta > 0.0 && fin && ft <= f0 + P.wolfe_eps * fabs(f0) &&
+                       tot[D_GTP] <= (2.0 * P.wolfe_delta - 1.0) * gd;
+    accept = armijo || wolfe;
+  }
+  if (accept) {
+    st.alpha_commit[i] = phase == 1 ? st.alpha[i] : 0.0;
+    st.f[i] = ft;
+    st.gnorm[i] = gn;
+    st.commit[i] = 1;
+    if (phase == 0) {
+      st.gnorm0[i] = gn;
+      st.f_first[i] = ft;
+    }
+    if (phase == 1) {
+      st.n_steps[i] += 1;
+      st.restarts[i] = 0;
+    }
+    int status = APL_RUNNING;
+    if (!isfinite(gn) || !isfinite(ft))
+      status = APL_NON_FINITE;
+    else if (gn <= P.rtol * st.gnorm0[i] || gn <= P.atol)
+      status = APL_PRIMARY_SUCCESS;
+    else if (st.n_steps[i] >= P.max_steps)
+      status = APL_MAX_STEPS_REACHED;
+    st.status[i] = status;
+    if (status == APL_RUNNING) {
+      const double pm = tot[D_HCNT] > 0.0 ? tot[D_HSUM] / tot[D_HCNT] : 1.0;
+      st.pmean[i] = pm;
+      const double gPg = tot[D_GPG] + tot[D_GPG0] / pm;
+      double beta = 0.0;
+      if (phase == 1) {
+        const double gPgc = tot[D_GPGC] + tot[D_GPGC0] / pm;
+        const double gcPgc = tot[D_GCPGC] + tot[D_GCPGC0] / pm;
+        const double yp = tot[D_GTP] - tot[D_GCP];
+        if (fabs(yp) > 1e-300) {
+          const double gPy = gPg - gPgc;
+          const double yPy = gPg - 2.0 * gPgc + gcPgc;
+          beta = gPy / yp - (yPy / yp) * (tot[D_GTP] / yp);
+        } else {
+          beta = INFINITY;
+        }
+        if (!isfinite(beta) || fabs(beta) > P.beta_max) beta = 0.0;
+      }
+      double gd = -gPg + beta * tot[D_GTP];
+      if (!(isfinite(gd) && gd < 0.0)) {
+        beta = 0.0;
+        gd = -gPg;
+      }
+      st.beta[i] = beta;
+      st.gd[i] = gd;
+      st.act_dir[i] = 1;
+    } else {
+      st.act_dir[i] = 0;
+      st.act_eval[i] = 0;
+    }
+    st.ls_step[i] = 0;
+    st.phase[i] = 1;
+  } else {
+    st.commit[i] = 0;
+    st.act_dir[i] = 0;
+    st.ls_step[i] += 1;
+    st.n_backtracks[i] += 1;
+    const double a = st.alpha[i] * 0.5;
+    if (st.ls_step[i] > P.max_backtracking || !(a > 0.0)) {
+      if (st.restarts[i] >= 1) {
+        st.status[i] = APL_STAGNATION;
+        st.act_eval[i] = 0;
+      } else {
+        // restart from x with steepest descent: re-evaluate at alpha = 0
+        st.restarts[i] += 1;
+        st.phase[i] = 2;
+        st.alpha[i] = 0.0;
+        st.ls_step[i] = 0;
+      }
+    } else {
+      st.alpha[i] = a;
+    }
+  }
+}
+
+// D: commit and new direction
+template <class R>
+__global__ __launch_bounds__(NTH) void pncg_commit_kernel(VecArgs<R> A) {
+  const int vb = blockIdx.x;
+  const int inst = A.vblk_inst[vb];
+  if (A.st.commit[inst] == 0) return;
+  const long long v = (long long)A.vblk_off[vb] + threadIdx.x;
+  if (v >= A.vblk_off[vb + 1]) return;
+  const R a = (R)A.st.alpha_commit[inst];
+  const R beta = (R)A.st.beta[inst];
+  const bool dir = A.st.act_dir[inst] != 0;
+  const R pm = (R)A.st.pmean[inst];
+  const unsigned msk = A.mask != nullptr ? A.mask[v] : 0u;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const R pc = A.p[3 * v + c];
+    A.x[3 * v + c] += a * pc;
+    const bool fixed = (msk & (1u << c)) != 0;
+    const R gt = fixed ? R(0) : A.gE[3 * v + c] - (A.fext != nullptr ? A.fext[3 * v + c] : R(0));
+    A.gc[3 * v + c] = gt;
+    if (dir) {
+      const R h = A.hd[3 * v + c];
+      const R Pc = fixed ? R(0) : (h > R(0) ? R(1) / h : R(1) / pm);
+      A.p[3 * v + c] = -Pc * gt + beta * pc;
+    }
+  }
+}
+
+// S2: alpha from the quadratic form; clears the per-iteration flags
+__global__ __launch_bounds__(NTH) void pncg_alpha_kernel(StatePtrs st, Params P, const double* blk_red,
+                                                         const int64_t* inst_blk_off) {
+  __shared__ double red[NTH / 64];
+  const int i = blockIdx.x;
+  const bool dir = st.act_dir[i] != 0;
+  if (dir) {
+    double q = 0.0;
+    for (long long b = inst_blk_off[i] + threadIdx.x; b < inst_blk_off[i + 1]; b += NTH) q += blk_red[2 * b + 1];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) q += __shfl_down(q, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = q;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double pHp = 0.0;
+      for (int w = 0; w < NTH / 64; ++w) pHp += red[w];
+      st.pHp[i] = pHp;
+      double a = -st.gd[i] / pHp;
+      if (isnan(a)) a = 0.0;
+      else if (a == -INFINITY) a = 0.0;
+      else if (a == INFINITY) a = 1.0;
+      if (!(a > 0.0 && isfinite(a))) a = 1.0;
+      st.alpha[i] = a * P.overstep;
+    }
+  }
+  if (threadIdx.x == 0) {
+    st.commit[i] = 0;
+    st.act_dir[i] = 0;
+  }
+}
+
+
+// ------------------------------------------------------------------ PCG (adjoint)
+// State reuse: gd = r.z, gnorm0 = ||b||, gnorm = ||r||, alpha = a, beta,
+// pHp = d.Hd, pmean = mean positive diagonal, f = ||r|| / ||b||.
+
+template <class R>
+struct CgArgs {
+  const int32_t* vblk_off;
+  const int32_t* vblk_inst;
+  const int32_t* vtx_sh_off;
+  const int32_t* sh_src;
+  const R* partial;
+  long long n_partials;
+  const uint8_t* mask;
+  const R* rhs;
+  R* y;
+  R* r;
+  R* z;
+  R* d;
+  R* q;
+  R* hd;
+  double* vblk_dots;
+  StatePtrs st;
+};
+
+enum CgStage : int { CG_DIAG = 0, CG_PRECOND = 1, CG_Q = 2, CG_UPDATE = 3, CG_DIR = 4 };
+
+template <class R, int STAGE>
+__global__ __launch_bounds__(NTH) void cg_vec_kernel(CgArgs<R> A) {
+  __shared__ double sm[(NTH / 64) * N_DOTS];
+  const int vb = blockIdx.x;
+  const int inst = A.vblk_inst[vb];
+  if (STAGE != CG_DIAG && STAGE != CG_PRECOND && A.st.act_eval[inst] == 0) return;
+  const long long v = (long long)A.vblk_off[vb] + threadIdx.x;
+  const bool in = v < A.vblk_off[vb + 1];
+  double dd[N_DOTS];
+#pragma unroll
+  for (int k = 0; k < N_DOTS; ++k) dd[k] = 0.0;
+  const unsigned msk = (in && A.mask != nullptr) ? A.mask[v] : 0u;
+  if (in) {
+    if constexpr (STAGE == CG_DIAG || STAGE == CG_Q) {
+      R* out = STAGE == CG_DIAG ? A.hd : A.q;
+      R s3[3];
+      const int o0 = A.vtx_sh_off[v], o1 = A.vtx_sh_off[v + 1];
+      if (o1 > o0) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) s3[c] = R(0);
+        for (int k = o0; k < o1; ++k) {
+          const long long pidx = A.sh_src[k];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) s3[c] += A.partial[c * A.n_partials + pidx];
+        }
+      } else {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) s3[c] = out[3 * v + c];
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const bool fixed = (msk & (1u << c)) != 0;
+        if constexpr (STAGE == CG_DIAG) {
+          out[3 * v + c] = s3[c];
+          const R b = fixed ? R(0) : A.rhs[3 * v + c];
+          A.r[3 * v + c] = b;
+          A.y[3 * v + c] = R(0);
+          A.d[3 * v + c] = R(0);
+          A.q[3 * v + c] = R(0);
+          if (!fixed) {
+            dd[D_G2] += (double)b * (double)b;
+            if (s3[c] > R(0)) {
+              dd[D_HSUM] += (double)s3[c];
+              dd[D_HCNT] += 1.0;
+            }
+          }
+        } else {
+          const R qv = fixed ? R(0) : s3[c];
+          out[3 * v + c] = qv;
+          dd[D_GTP] += (double)A.d[3 * v + c] * (double)qv;
+        }
+      }
+    } else if constexpr (STAGE == CG_PRECOND || STAGE == CG_UPDATE) {
+      const R a = (R)A.st.alpha[inst];
+      const R pm = (R)A.st.pmean[inst];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const bool fixed = (msk & (1u << c)) != 0;
+        R rv = A.r[3 * v + c];
+        if constexpr (STAGE == CG_UPDATE) {
+          A.y[3 * v + c] += a * A.d[3 * v + c];
+          rv -= a * A.q[3 * v + c];
+          A.r[3 * v + c] = rv;
+        }
+        const R h = A.hd[3 * v + c];
+        const R zv = fixed ? R(0) : (h > R(0) ? rv / h : rv / pm);
+        A.z[3 * v + c] = zv;
+        if constexpr (STAGE == CG_PRECOND) A.d[3 * v + c] = zv;
+        dd[D_GPG] += (double)rv * (double)zv;
+        dd[D_G2] += (double)rv * (double)rv;
+      }
+    } else {  // CG_DIR
+      const R beta = (R)A.st.beta[inst];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) A.d[3 * v + c] = A.z[3 * v + c] + beta * A.d[3 * v + c];
+    }
+  }
+  if constexpr (STAGE != CG_DIR) block_reduce_n<N_DOTS>(dd, sm, A.vblk_dots + (long long)vb * N_DOTS);
+}
+
+// per-instance CG scalars; stage 0: after DIAG, 1: after PRECOND, 2: after Q, 3: after UPDATE
+__global__ __launch_bounds__(NTH) void cg_scalar_kernel(StatePtrs st, int stage, double rtol, int max_iters,
+                                                        const double* vblk_dots, const int64_t* inst_vblk_off) {
+  __shared__ double sm[(NTH / 64) * N_DOTS];
+  __shared__ double tot[N_DOTS];
+  const int i = blockIdx.x;
+  if (stage >= 2 && st.act_eval[i] == 0) return;
+  double acc[N_DOTS];
+#pragma unroll
+  for (int k = 0; k < N_DOTS; ++k) acc[k] = 0.0;
+  for (long long vb = inst_vblk_off[i] + threadIdx.x; vb < inst_vblk_off[i + 1]; vb += NTH)
+#pragma unroll
+    for (int k = 0; k < N_DOTS; ++k) acc[k] += vblk_dots[vb * N_DOTS + k];
+  block_reduce_n<N_DOTS>(acc, sm, tot);
+  if (threadIdx.x != 0) return;
+  if (stage == 0) {
+    st.pmean[i] = tot[D_HCNT] > 0.0 ? tot[D_HSUM] / tot[D_HCNT] : 1.0;
+    const double bn = sqrt(tot[D_G2]);
+    st.gnorm0[i] = bn;
+    st.gnorm[i] = bn;
+    st.n_steps[i] = 0;
+    st.f[i] = bn > 0.0 ? 1.0 : 0.0;
+    st.status[i] = bn > 0.0 ? APL_RUNNING : APL_PRIMARY_SUCCESS;
+    st.act_eval[i] = bn > 0.0 ? 1 : 0;
+    st.alpha[i] = 0.0;
+    st.beta[i] = 0.0;
+  } else if (stage == 1) {
+    st.gd[i] = tot[D_GPG];
+  } else if (stage == 2) {
+    const double dq = tot[D_GTP];
+    st.pHp[i] = dq;
+    if (!(dq > 0.0) || !isfinite(dq)) {
+      st.status[i] = APL_NON_FINITE;
+      st.act_eval[i] = 0;
+      st.alpha[i] = 0.0;
+    } else {
+      st.alpha[i] = st.gd[i] / dq;
+    }
+  } else {
+    const double rz = tot[D_GPG];
+    const double rn = sqrt(tot[D_G2]);
+    st.gnorm[i] = rn;
+    st.n_steps[i] += 1;
+    st.f[i] = st.gnorm0[i] > 0.0 ? rn / st.gnorm0[i] : 0.0;
+    if (rn <= rtol * st.gnorm0[i]) {
+      st.status[i] = APL_PRIMARY_SUCCESS;
+      st.act_eval[i] = 0;
+    } else if (!isfinite(rn)) {
+      st.status[i] = APL_NON_FINITE;
+      st.act_eval[i] = 0;
+    } else if (st.n_steps[i] >= max_iters) {
+      st.status[i] = APL_MAX_STEPS_REACHED;
+      st.act_eval[i] = 0;
+    } else {
+      st.beta[i] = rz / st.gd[i];
+      st.gd[i] = rz;
+    }
+  }
+}
+
+}  // namespace apl
+
+using namespace apl;
+
+struct apl_solver {
+  apl_mesh* mesh = nullptr;
+  int material = 0;
+  void* vecs = nullptr;  // p, gc, gE, hd, r, z, d, q, xs (9 x V x 3)
+  double* sd = nullptr;
+  int32_t* si = nullptr;
+  StatePtrs st{};
+  hipStream_t cap = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  std::vector<char> gkey;
+  int32_t* host_status = nullptr;
+};
+
+namespace apl {
+
+template <class R>
+static VecArgs<R> vec_args(apl_solver* s, void* x, const void* fext, const uint8_t* mask) {
+  apl_mesh* m = s->mesh;
+  const size_t nv = 3 * (size_t)m->L.V;
+  R* base = (R*)s->vecs;
+  VecArgs<R> A;
+  A.vblk_off = m->d.vblk_off;
+  A.vblk_inst = m->d.vblk_inst;
+  A.vtx_sh_off = m->d.vtx_sh_off;
+  A.sh_src = m->d.sh_src;
+  A.partial = (const R*)m->d.partial;
+  A.n_partials = m->L.n_partials;
+  A.mask = mask;
+  A.fext = (const R*)fext;
+  A.x = (R*)x;
+  A.p = base;
+  A.gc = base + nv;
+  A.gE = base + 2 * nv;
+  A.hd = base + 3 * nv;
+  A.vblk_dots = m->d.vblk_dots;
+  A.st = s->st;
+  return A;
+}
+
+static Params to_params(const apl_pncg_params* p) {
+  Params P;
+  P.rtol = p->rtol;
+  P.atol = p->atol;
+  P.overstep = p->overstep;
+  P.c1 = p->c1;
+  P.wolfe_delta = p->wolfe_delta;
+  P.wolfe_eps = p->wolfe_eps;
+  P.beta_max = p->beta_max;
+  P.max_steps = p->max_steps;
+  P.max_backtracking = p->max_backtracking;
+  return P;
+}
+
+template <class R>
+static int pncg_iteration(apl_solver* s, const void* mu, const void* lam, void* x, const void* fext,
+                          const uint8_t* mask, const Params& P, int grid, hipStream_t st) {
+  apl_mesh* m = s->mesh;
+  VecArgs<R> A = vec_args<R>(s, x, fext, mask);
+  const unsigned nvb = (unsigned)m->L.vblk_inst.size();
+  const unsigned ni = (unsigned)m->L.n_inst;
+  if (launch_elem(m, s->material, W_E | W_G | W_H, mu, lam, x, A.p, s->st.alpha, s->st.act_eval, A.gE, A.hd, 0,
+                  grid, st))
+    return -1;
+  if (nvb) hipLaunchKernelGGL((pncg_reduce_kernel<R>), dim3(nvb), dim3(NTH), 0, st, A);
+  hipLaunchKernelGGL(pncg_scalar_kernel, dim3(ni), dim3(NTH), 0, st, s->st, P, m->d.vblk_dots, m->d.inst_vblk_off,
+                     m->d.blk_red, m->d.inst_blk_off);
+  if (nvb) hipLaunchKernelGGL((pncg_commit_kernel<R>), dim3(nvb), dim3(NTH), 0, st, A);
+  if (launch_elem(m, s->material, W_HQ, mu, lam, x, A.p, nullptr, s->st.act_dir, nullptr, nullptr, 0, grid, st))
+    return -1;
+  hipLaunchKernelGGL(pncg_alpha_kernel, dim3(ni), dim3(NTH), 0, st, s->st, P, m->d.blk_red, m->d.inst_blk_off);
+  APL_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+template <class R>
+static int fill(R* a, long long n, R v, hipStream_t st) {
+  if (n <= 0) return 0;
+  long long g = (n + NTH - 1) / NTH;
+  if (g > 4096) g = 4096;
+  hipLaunchKernelGGL((fill_kernel<R>), dim3((unsigned)g), dim3(NTH), 0, st, a, n, v);
+  APL_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+template <class R>
+static int pncg_solve_t(apl_solver* s, const void* mu, const void* lam, void* x, const void* fext,
+                        const uint8_t* mask, const apl_pncg_params* prm, apl_pncg_stats* stats, hipStream_t st) {
+  apl_mesh* m = s->mesh;
+  const int n = (int)m->L.n_inst;
+  const Params P = to_params(prm);
+  const int K = prm->iters_per_launch > 0 ? prm->iters_per_launch : 16;
+  // reset state and work vectors (p and g_cur must start at zero)
+  hipLaunchKernelGGL(pncg_init_kernel, dim3((n + 255) / 256), dim3(256), 0, st, s->st, n);
+  if (fill<R>((R*)s->vecs, 4 * 3 * (long long)m->L.V, R(0), st)) return -1;
+  APL_CHECK_HIP(hipGetLastError());
+  const int grid = prm->grid_override;
+  // occupancy-derived grids are queried (and cached) outside any capture
+  (void)elem_grid(m, s->material, W_E | W_G | W_H);
+  (void)elem_grid(m, s->material, W_HQ);
+  // cached graph of K iterations for these pointers and parameters
+  bool use_graph = prm->use_graph != 0;
+  if (use_graph) {
+    std::vector<char> key(sizeof(void*) * 5 + sizeof(Params) + 2 * sizeof(int));
+    char* k = key.data();
+    const void* ptrs[5] = {mu, lam, x, fext, mask};
+    std::memcpy(k, ptrs, sizeof ptrs);
+    k += sizeof ptrs;
+    std::memcpy(k, &P, sizeof P);
+    k += sizeof P;
+    std::memcpy(k, &K, sizeof K);
+    k += sizeof K;
+    std::memcpy(k, &grid, sizeof grid);
+    if (s->gexec == nullptr || key != s->gkey) {
+      if (s->gexec) {
+        (void)hipGraphExecDestroy(s->gexec);
+        s->gexec = nullptr;
+      }
+      if (s->graph) {
+        (void)hipGraphDestroy(s->graph);
+        s->graph = nullptr;
+      }
+      if (s->cap == nullptr) APL_CHECK_HIP(hipStreamCreateWithFlags(&s->cap, hipStreamNonBlocking));
+      APL_CHECK_HIP(hipStreamBeginCapture(s->cap, hipStreamCaptureModeThreadLocal));
+      int rc = 0;
+      for (int it = 0; it < K && rc == 0; ++it) rc = pncg_iteration<R>(s, mu, lam, x, fext, mask, P, grid, s->cap);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(s->cap, &g);
+      if (rc != 0 || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        if (rc == 0) set_error(std::string("graph capture failed: ") + hipGetErrorString(e));
+        return -1;
+      }
+      s->graph = g;
+      APL_CHECK_HIP(hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0));
+      s->gkey = key;
+    }
+  }
+  const long long guard = ((long long)prm->max_steps + 2) * ((long long)prm->max_backtracking + 2) * 2 + 8;
+  long long done = 0;
+  while (true) {
+    if (use_graph) {
+      APL_CHECK_HIP(hipGraphLaunch(s->gexec, st));
+    } else {
+      for (int it = 0; it < K; ++it)
+        if (pncg_iteration<R>(s, mu, lam, x, fext, mask, P, grid, st)) return -1;
+    }
+    done += K;
+    APL_CHECK_HIP(hipMemcpyAsync(s->host_status, s->st.status, n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    APL_CHECK_HIP(hipStreamSynchronize(st));
+    bool running = false;
+    for (int i = 0; i < n; ++i) running = running || s->host_status[i] == APL_RUNNING;
+    if (!running || done > guard) break;
+  }
+  if (stats != nullptr) {
+    std::vector<double> hd((size_t)N_SD * n);
+    std::vector<int32_t> hi((size_t)N_SI * n);
+    APL_CHECK_HIP(hipMemcpyAsync(hd.data(), s->sd, hd.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    APL_CHECK_HIP(hipMemcpyAsync(hi.data(), s->si, hi.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    APL_CHECK_HIP(hipStreamSynchronize(st));
+    for (int i = 0; i < n; ++i) {
+      apl_pncg_stats& o = stats[i];
+      o.f = hd[0 * n + i];
+      o.alpha = hd[3 * n + i];
+      o.beta = hd[4 * n + i];
+      o.grad_norm_first = hd[5 * n + i];
+      o.grad_norm = hd[6 * n + i];
+      o.f_first = hd[8 * n + i];
+      o.status = hi[0 * n + i];
+      o.n_steps = hi[4 * n + i];
+      o.n_evals = hi[5 * n + i];
+      o.n_backtracks = hi[7 * n + i];
+      if (o.status == APL_RUNNING) o.status = APL_MAX_STEPS_REACHED;
+    }
+  }
+  return 0;
+}
+
+
+template <class R>
+static CgArgs<R> cg_args(apl_solver* s, const void* rhs, void* y, const uint8_t* mask) {
+  apl_mesh* m = s->mesh;
+  const size_t nv = 3 * (size_t)m->L.V;
+  R* base = (R*)s->vecs;
+  CgArgs<R> A;
+  A.vblk_off = m->d.vblk_off;
+  A.vblk_inst = m->d.vblk_inst;
+  A.vtx_sh_off = m->d.vtx_sh_off;
+  A.sh_src = m->d.sh_src;
+  A.partial = (const R*)m->d.partial;
+  A.n_partials = m->L.n_partials;
+  A.mask = mask;
+  A.rhs = (const R*)rhs;
+  A.y = (R*)y;
+  A.r = base + 4 * nv;
+  A.z = base + 5 * nv;
+  A.d = base + 6 * nv;
+  A.q = base + 7 * nv;
+  A.hd = base + 8 * nv;
+  A.vblk_dots = m->d.vblk_dots;
+  A.st = s->st;
+  return A;
+}
+
+template <class R>
+static int cg_iteration(apl_solver* s, const void* mu, const void* lam, const void* x, CgArgs<R>& A, double rtol,
+                        int max_iters, hipStream_t st) {
+  apl_mesh* m = s->mesh;
+  const unsigned nvb = (unsigned)m->L.vblk_inst.size();
+  const unsigned ni = (unsigned)m->L.n_inst;
+  if (launch_elem(m, s->material, W_HP, mu, lam, x, A.d, nullptr, s->st.act_eval, A.q, nullptr, 0, 0, st)) return -1;
+  if (nvb) hipLaunchKernelGGL((cg_vec_kernel<R, CG_Q>), dim3(nvb), dim3(NTH), 0, st, A);
+  hipLaunchKernelGGL(cg_scalar_kernel, dim3(ni), dim3(NTH), 0, st, s->st, 2, rtol, max_iters, m->d.vblk_dots,
+                     m->d.inst_vblk_off);
+  if (nvb) hipLaunchKernelGGL((cg_vec_kernel<R, CG_UPDATE>), dim3(nvb), dim3(NTH), 0, st, A);
+  hipLaunchKernelGGL(cg_scalar_kernel, dim3(ni), dim3(NTH), 0, st, s->st, 3, rtol, max_iters, m->d.vblk_dots,
+                     m->d.inst_vblk_off);
+  if (nvb) hipLaunchKernelGGL((cg_vec_kernel<R, CG_DIR>), dim3(nvb), dim3(NTH), 0, st, A);
+  APL_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+template <class R>
+static int cg_solve_t(apl_solver* s, const void* mu, const void* lam, const void* x, const void* rhs, void* y,
+                      const uint8_t* mask, double rtol, int max_iters, apl_pncg_stats* stats, hipStream_t st) {
+  apl_mesh* m = s->mesh;
+  const int n = (int)m->L.n_inst;
+  const unsigned nvb = (unsigned)m->L.vblk_inst.size();
+  CgArgs<R> A = cg_args<R>(s, rhs, y, mask);
+  hipLaunchKernelGGL(pncg_init_kernel, dim3((n + 255) / 256), dim3(256), 0, st, s->st, n);
+  // zero the diagonal buffer (vertices with no tet keep 0) and run the Hdiag pass at x
+  if (fill<R>(A.hd, 3 * (long long)m->L.V, R(0), st)) return -1;
+  if (launch_elem(m, s->material, W_H, mu, lam, x, nullptr, nullptr, nullptr, nullptr, A.hd, 0, 0, st)) return -1;
+  if (nvb) hipLaunchKernelGGL((cg_vec_kernel<R, CG_DIAG>), dim3(nvb), dim3(NTH), 0, st, A);
+  hipLaunchKernelGGL(cg_scalar_kernel, dim3(n), dim3(NTH), 0, st, s->st, 0, rtol, max_iters, m->d.vblk_dots,
+                     m->d.inst_vblk_off);
+  if (nvb) hipLaunchKernelGGL((cg_vec_kernel<R, CG_PRECOND>), dim3(nvb), dim3(NTH), 0, st, A);
+  hipLaunchKernelGGL(cg_scalar_kernel, dim3(n), dim3(NTH), 0, st, s->st, 1, rtol, max_iters, m->d.vblk_dots,
+                     m->d.inst_vblk_off);
+  APL_CHECK_HIP(hipGetLastError());
+  (void)elem_grid(m, s->material, W_HP);
+  const int K = 16;
+  long long done = 0;
+  while (true) {
+    for (int it = 0; it < K; ++it)
+      if (cg_iteration<R>(s, mu, lam, x, A, rtol, max_iters, st)) return -1;
+    done += K;
+    APL_CHECK_HIP(hipMemcpyAsync(s->host_status, s->st.status, n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    APL_CHECK_HIP(hipStreamSynchronize(st));
+    bool running = false;
+    for (int i = 0; i < n; ++i) running = running || s->host_status[i] == APL_RUNNING;
+    if (!running || done > (long long)max_iters + K) break;
+  }
+  if (stats != nullptr) {
+    std::vector<double> hd((size_t)N_SD * n);
+    std::vector<int32_t> hi((size_t)N_SI * n);
+    APL_CHECK_HIP(hipMemcpyAsync(hd.data(), s->sd, hd.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    APL_CHECK_HIP(hipMemcpyAsync(hi.data(), s->si, hi.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    APL_CHECK_HIP(hipStreamSynchronize(st));
+    for (int i = 0; i < n; ++i) {
+      apl_pncg_stats& o = stats[i];
+      o.f = hd[0 * n + i];
+      o.alpha = hd[2 * n + i];
+      o.beta = hd[4 * n + i];
+      o.grad_norm_first = hd[5 * n + i];
+      o.grad_norm = hd[6 * n + i];
+      o.f_first = 1.0;
+      o.status = hi[0 * n + i] == APL_RUNNING ? APL_MAX_STEPS_REACHED : hi[0 * n + i];
+      o.n_steps = hi[4 * n + i];
+      o.n_evals = hi[4 * n + i];
+      o.n_backtracks = 0;
+    }
+  }
+  return 0;
+}
+
+}  // namespace apl
+
+extern "C" {
+
+int apl_solver_create(apl_mesh* m, int material, apl_solver** out) {
+  if (m == nullptr || out == nullptr) {
+    set_error("NULL argument");
+    return -1;
+  }
+  if (material != APL_SNH && material != APL_ARAP) {
+    set_error("material must be APL_SNH or APL_ARAP");
+    return -1;
+  }
+  *out = nullptr;
+  APL_CHECK_HIP(hipSetDevice(m->device));
+  apl_solver* s = new apl_solver();
+  s->mesh = m;
+  s->material = material;
+  const int n = (int)m->L.n_inst;
+  const size_t vb = 9 * 3 * (size_t)std::max<int64_t>(m->L.V, 1) * real_size(m);
+  if (hipMalloc(&s->vecs, vb) != hipSuccess || hipMalloc((void**)&s->sd, N_SD * (size_t)n * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&s->si, N_SI * (size_t)n * sizeof(int32_t)) != hipSuccess ||
+      hipHostMalloc((void**)&s->host_status, (size_t)n * sizeof(int32_t), 0) != hipSuccess) {
+    set_error("out of device memory creating the solver");
+    apl_solver_destroy(s);
+    return -1;
+  }
+  APL_CHECK_HIP(hipMemset(s->vecs, 0, vb));
+  double* d = s->sd;
+  s->st.f = d + 0 * n;
+  s->st.gd = d + 1 * n;
+  s->st.alpha = d + 2 * n;
+  s->st.alpha_commit = d + 3 * n;
+  s->st.beta = d + 4 * n;
+  s->st.gnorm0 = d + 5 * n;
+  s->st.gnorm = d + 6 * n;
+  s->st.pmean = d + 7 * n;
+  s->st.f_first = d + 8 * n;
+  s->st.pHp = d + 9 * n;
+  int32_t* ii = s->si;
+  s->st.status = ii + 0 * n;
+  s->st.act_eval = ii + 1 * n;
+  s->st.act_dir = ii + 2 * n;
+  s->st.commit = ii + 3 * n;
+  s->st.n_steps = ii + 4 * n;
+  s->st.n_evals = ii + 5 * n;
+  s->st.ls_step = ii + 6 * n;
+  s->st.n_backtracks = ii + 7 * n;
+  s->st.phase = ii + 8 * n;
+  s->st.restarts = ii + 9 * n;
+  *out = s;
+  return 0;
+}
+
+int apl_solver_destroy(apl_solver* s) {
+  if (s == nullptr) return 0;
+  if (s->mesh) (void)hipSetDevice(s->mesh->device);
+  if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+  if (s->graph) (void)hipGraphDestroy(s->graph);
+  if (s->cap) (void)hipStreamDestroy(s->cap);
+  if (s->vecs) (void)hipFree(s->vecs);
+  if (s->sd) (void)hipFree(s->sd);
+  if (s->si) (void)hipFree(s->si);
+  if (s->host_status) (void)hipHostFree(s->host_status);
+  delete s;
+  return 0;
+}
+
+int apl_pncg_solve(apl_solver* s, const void* mu, const void* lam, void* x, const void* f_ext,
+                   const uint8_t* fixed_mask, const apl_pncg_params* params, apl_pncg_stats* stats, void* stream) {
+  if (s == nullptr || params == nullptr || (s->mesh->L.V > 0 && x == nullptr) ||
+      (s->mesh->L.T > 0 && (mu == nullptr || (s->material == APL_SNH && lam == nullptr)))) {
+    set_error("NULL argument");
+    return -1;
+  }
+  if (params->max_steps < 0 || params->max_backtracking < 0 || !(params->rtol >= 0) || !(params->atol >= 0)) {
+    set_error("invalid PNCG parameters");
+    return -1;
+  }
+  APL_CHECK_HIP(hipSetDevice(s->mesh->device));
+  if (s->mesh->dtype == DT_F32)
+    return pncg_solve_t<float>(s, mu, lam, x, f_ext, fixed_mask, params, stats, (hipStream_t)stream);
+  return pncg_solve_t<double>(s, mu, lam, x, f_ext, fixed_mask, params, stats, (hipStream_t)stream);
+}
+
+int apl_cg_solve(apl_solver* s, const void* mu, const void* lam, const void* x, const void* rhs, void* y,
+                 const uint8_t* fixed_mask, double rtol, int max_iters, apl_pncg_stats* stats, void* stream) {
+  if (s == nullptr || (s->mesh->L.V > 0 && (x == nullptr || rhs == nullptr || y == nullptr)) ||
+      (s->mesh->L.T > 0 && (mu == nullptr || (s->material == APL_SNH && lam == nullptr)))) {
+    set_error("NULL argument");
+    return -1;
+  }
+  if (!(rtol >= 0) || max_iters < 0) {
+    set_error("invalid CG parameters");
+    return -1;
+  }
+  APL_CHECK_HIP(hipSetDevice(s->mesh->device));
+  if (s->mesh->dtype == DT_F32)
+    return cg_solve_t<float>(s, mu, lam, x, rhs, y, fixed_mask, rtol, max_iters, stats, (hipStream_t)stream);
+  return cg_solve_t<double>(s, mu, lam, x, rhs, y, fixed_mask, rtol, max_iters, stats, (hipStream_t)stream);
+}
+
+}  // extern "C"

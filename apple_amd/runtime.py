@@ -43,6 +43,16 @@ def ptr(t: torch.Tensor | None, *, dtype=None, device=None, shape=None, name="te
     return t.data_ptr()
 
 
+def normalize_device(device) -> torch.device:
+    """torch.device with an explicit index (``cuda`` -> ``cuda:<current>``)."""
+    dev = torch.device(device) if device is not None else torch.device("cuda")
+    if dev.type != "cuda":
+        raise ValueError("apple_amd runs on a ROCm device; got " + str(dev))
+    if dev.index is None:
+        dev = torch.device("cuda", torch.cuda.current_device())
+    return dev
+
+
 def current_stream(device: torch.device) -> int:
     return torch.cuda.current_stream(device).cuda_stream
 
@@ -69,9 +79,7 @@ class DeviceMesh:
         chunks: int = 1,
     ):
         self.dtype = as_dtype(dtype)
-        self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
-        if self.device.type != "cuda":
-            raise ValueError("apple_amd runs on a ROCm device; got " + str(self.device))
+        self.device = normalize_device(device)
         cells = np.ascontiguousarray(cells, dtype=np.int32)
         dminv = np.ascontiguousarray(dminv, dtype=np.float64).reshape(-1, 9)
         dV = np.ascontiguousarray(dV, dtype=np.float64).reshape(-1)
@@ -102,7 +110,7 @@ class DeviceMesh:
                 self.inst_tet_off.ctypes.data,
                 self.inst_vtx_off.ctypes.data,
                 int(chunks),
-                self.device.index if self.device.index is not None else torch.cuda.current_device(),
+                self.device.index,
                 ctypes.byref(self._handle),
             )
         )

@@ -88,6 +88,51 @@ int apl_elem_eval(apl_mesh* mesh, int material, const void* mu, const void* lam,
 int apl_ext_force_eval(apl_mesh* mesh, int64_t n, const int32_t* indices, const void* force, const void* u,
                        double* energy, void* grad, int accumulate, void* stream);
 
+/* ---------------------------------------------------------------- solvers */
+
+/* PNCG parameters.  Defaults used by the Python layer: rtol 1e-6, atol 0,
+ * max_steps 1500 (forward/_forward.py:26-27), max_backtracking 8, overstep 1,
+ * c1 1e-4, wolfe_delta 0.1, wolfe_eps 1e-6 (FP32) / 1e-12 (FP64), beta_max 10
+ * (benches/bench_pncg_branching_backends.py:288-289), iters_per_launch 16,
+ * use_graph 1, grid_override 0 (occupancy-derived persistent grid). */
+typedef struct {
+  double rtol, atol;
+  int32_t max_steps, max_backtracking;
+  double overstep, c1, wolfe_delta, wolfe_eps, beta_max;
+  int32_t iters_per_launch, use_graph, grid_override, pad;
+} apl_pncg_params;
+
+/* Per-instance result: status (APL_* code), accepted steps, energy
+ * evaluations, backtracks, final energy, gradient norms, last step, last beta. */
+typedef struct {
+  int32_t status, n_steps, n_evals, n_backtracks;
+  double f, grad_norm, grad_norm_first, alpha, beta, f_first;
+} apl_pncg_stats;
+
+/* A solver owns the work vectors and per-instance state for one mesh and
+ * material (the Forward/PNCG pair of forward/_forward.py:18-56). */
+int apl_solver_create(apl_mesh* mesh, int material, apl_solver** out);
+int apl_solver_destroy(apl_solver* solver);
+
+/* Forward.step(): minimise E(x) = sum_t Psi_t dV_t - f_ext . x over the free
+ * DOFs, independently per instance.  x (V,3) holds the initial guess with
+ * the fixed DOFs at their prescribed values (DofMap.to_full,
+ * forward/dof_map/_dof_map.py:39-43) and is overwritten by the solution.
+ * f_ext (V,3) or NULL; fixed_mask (V,) uint8, bit c set = component c fixed
+ * (DofMapBuilder, forward/dof_map/_builder.py:34-64), or NULL.  stats: host
+ * array of n_inst entries or NULL.  Synchronises `stream` once per
+ * iters_per_launch iterations to test for termination. */
+int apl_pncg_solve(apl_solver* solver, const void* mu, const void* lam, void* x, const void* f_ext,
+                   const uint8_t* fixed_mask, const apl_pncg_params* params, apl_pncg_stats* stats, void* stream);
+
+/* Adjoint linear solve (exp/2025/08/27/inverse/src/30-inverse.py:85-93):
+ * Jacobi-preconditioned CG on H(x) y = rhs over the free DOFs, H applied by
+ * the unclamped Hessian-vector pass; y (V,3) is overwritten (zero at fixed
+ * DOFs).  Stops at ||r|| <= rtol ||rhs|| per instance; stats as above with
+ * f = final relative residual.  Reports APL_NON_FINITE if d^T H d <= 0. */
+int apl_cg_solve(apl_solver* solver, const void* mu, const void* lam, const void* x, const void* rhs, void* y,
+                 const uint8_t* fixed_mask, double rtol, int max_iters, apl_pncg_stats* stats, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

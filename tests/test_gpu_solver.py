@@ -1,0 +1,216 @@
+"""PNCG / PCG on the GPU vs the oracle (end states), through the C ABI.
+
+* The reference's patch test (tests/forward/test_static_simulation.py:58-93)
+  through ModelBuilder / Forward.step(): interior vertex at
+  (0.05, -0.025, 0.0375) within 1e-8 (FP64) and PRIMARY_SUCCESS.
+* Gravity-loaded cubes (SNH and ARAP, z = 0 pinned, seeded noisy start):
+  equilibrium positions within 1e-5 of the bounding-box diagonal of the
+  Newton-polished FP64 oracle equilibrium; the oracle gradient there is ~0.
+* Graph replay == eager launches, bitwise; batched == one-by-one, bitwise.
+* PCG: H y = b vs a sparse direct solve of the oracle's exact Hessian.
+* Failure paths: max_steps exhaustion, non-finite materials.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import oracle.fem as of
+import oracle.pncg as op
+from apple_amd import batch, fem, forward, potential
+from apple_amd.mesh import Mesh, Region, cube
+from apple_amd.solver import DeviceSolver, PncgConfig, Result, mask_bits
+
+pytestmark = pytest.mark.gpu
+
+PATCH_POINTS = np.array(
+    [[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0], [0.25, 0.25, 0.25]]
+)
+PATCH_CELLS = np.array([[0, 1, 2, 4], [0, 1, 4, 3], [0, 4, 2, 3], [4, 1, 2, 3]], np.int32)
+
+
+@pytest.mark.parametrize("dtype,atol", [(torch.float64, 1e-8), (torch.float32, 2e-6)])
+@pytest.mark.parametrize("cls", [fem.Arap, fem.StableNeoHookean])
+def test_patch_test_known_answer(dtype, atol, cls):
+    print(f"[solver] patch {cls.__name__} {dtype}", flush=True)
+    mesh = Mesh(PATCH_POINTS.copy(), PATCH_CELLS.copy())
+    mesh.cell_data["mu"] = np.ones(4)
+    mesh.cell_data["lambda"] = np.ones(4)
+    builder = forward.ModelBuilder()
+    builder.add_vertices(mesh)
+    fixed_mask = np.zeros((5, 3), bool)
+    fixed_value = np.zeros((5, 3))
+    fixed_mask[:4] = True
+    fixed_value[3] = [0.2, -0.1, 0.15]
+    mesh.point_data["FixedMask"] = fixed_mask
+    mesh.point_data["FixedValue"] = fixed_value
+    builder.add_fixed(mesh)
+    builder.add_potential(cls.from_pyvista(mesh, dtype=dtype, device="cuda"))
+    model = builder.finalize()
+    fwd = forward.Forward(model, forward.PNCG(rtol=1e-9 if dtype == torch.float64 else 1e-5))
+    e0 = float(fwd.problem.fun(fwd.state))
+    sol = fwd.step()
+    e1 = float(fwd.problem.fun(fwd.state))
+    assert sol.result.name == "PRIMARY_SUCCESS"
+    assert model.n_free == 3
+    assert np.isfinite(e0) and np.isfinite(e1) and e1 < e0
+    u = fwd.state.u.double().cpu().numpy()
+    np.testing.assert_allclose(u[:4], fixed_value[:4], atol=1e-7)
+    np.testing.assert_allclose(u[4], [0.05, -0.025, 0.0375], atol=atol)
+
+
+def _gravity_case(n, material, seed=0, noise_scale=0.1):
+    m = cube(n)
+    rng = np.random.default_rng(seed)
+    mu = 1e4 * np.exp(0.5 * rng.standard_normal(m.n_cells))
+    lam = 10.0 * mu
+    region = Region.from_mesh(m)
+    f = np.zeros((m.n_points, 3))
+    np.add.at(f[:, 2], m.cells.reshape(-1), np.repeat(-9.8 * 1e3 * region.dV / 4.0, 4))
+    fm = np.zeros((m.n_points, 3), bool)
+    fm[m.points[:, 2] == 0.0] = True
+    h = 1.0 / n
+    x0 = noise_scale * h * np.random.default_rng(seed + 100).standard_normal((m.n_points, 3))
+    x0[fm] = 0.0
+    return m, region, mu, lam, f, fm, x0
+
+
+def _oracle_equilibrium(region, material, mu, lam, f, fm, x_start):
+    mat = of.SNH if material == "snh" else of.ARAP
+    pot = of.FemPotential(mat, region.cells, region.dhdX, region.dV, mu, lam if mat == of.SNH else None)
+    ext = of.ExternalForce(np.arange(f.shape[0]), f)
+    dof = of.DofMap(fm, np.zeros_like(f))
+    prob = op.Problem(of.Model([pot, ext]), dof)
+    x = op.newton(prob, dof.to_free(x_start), tol=1e-10, max_steps=30)
+    gn = np.linalg.norm(prob.grad(x))
+    return dof.to_full(x), gn
+
+
+@pytest.mark.parametrize("material", ["snh", "arap"])
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_gravity_equilibrium_matches_oracle(material, dtype):
+    print(f"[solver] gravity {material} {dtype}", flush=True)
+    m, region, mu, lam, f, fm, x0 = _gravity_case(6, material)
+    cls = fem.StableNeoHookean if material == "snh" else fem.Arap
+    m.cell_data["mu"] = mu
+    m.cell_data["lambda"] = lam
+    pot = cls.from_region(region, dtype=dtype, device="cuda")
+    solver = DeviceSolver(pot)
+    x = torch.as_tensor(x0, dtype=dtype, device="cuda").contiguous()
+    ft = torch.as_tensor(f, dtype=dtype, device="cuda").contiguous()
+    mask = mask_bits(fm, x.device)
+    st = solver.solve(x, ft, mask, PncgConfig(rtol=1e-6, max_steps=5000))[0]
+    assert st.result == Result.PRIMARY_SUCCESS, st
+    assert st.relative_grad_norm <= 1e-6
+    xg = x.double().cpu().numpy()
+    x_ref, gn = _oracle_equilibrium(region, material, mu, lam, f, fm, xg)
+    assert gn <= 1e-6 * max(st.grad_norm_first, 1.0)
+    diag = np.sqrt(3.0)
+    err = np.abs(xg - x_ref).max() / diag
+    print(f"  steps {st.n_steps} evals {st.n_evals} rel|g| {st.relative_grad_norm:.2e} pos err/diag {err:.2e}")
+    assert err <= 1e-5
+
+
+def test_graph_replay_equals_eager_launches():
+    m, region, mu, lam, f, fm, x0 = _gravity_case(5, "snh", seed=2)
+    m.cell_data["mu"] = mu
+    m.cell_data["lambda"] = lam
+    pot = fem.StableNeoHookean.from_region(region, dtype=torch.float32, device="cuda")
+    solver = DeviceSolver(pot)
+    ft = torch.as_tensor(f, dtype=torch.float32, device="cuda")
+    mask = mask_bits(fm, "cuda")
+    xs, sts = [], []
+    for graph in (True, False, True):
+        x = torch.as_tensor(x0, dtype=torch.float32, device="cuda").contiguous()
+        sts.append(solver.solve(x, ft, mask, PncgConfig(use_graph=graph, iters_per_launch=8))[0])
+        xs.append(x.cpu())
+    assert torch.equal(xs[0], xs[1]) and torch.equal(xs[0], xs[2])
+    assert sts[0].n_steps == sts[1].n_steps == sts[2].n_steps
+
+
+def test_batched_equals_one_by_one():
+    insts = []
+    for k, n in enumerate((3, 5, 4)):
+        m, region, mu, lam, f, fm, x0 = _gravity_case(n, "snh", seed=10 + k)
+        insts.append(batch.Instance(region, mu, lam, f, fm, x0))
+    b = batch.Batch(insts, 0, dtype=torch.float32, device="cuda")
+    xb, stb = b.solve(config=PncgConfig(rtol=1e-6))
+    parts = b.split(xb)
+    for i, inst in enumerate(insts):
+        single = batch.Batch([inst], 0, dtype=torch.float32, device="cuda")
+        xs, sts = single.solve(config=PncgConfig(rtol=1e-6))
+        assert stb[i].result == Result.PRIMARY_SUCCESS
+        assert stb[i].n_steps == sts[0].n_steps
+        assert torch.equal(parts[i].cpu(), xs.cpu())
+
+
+@pytest.mark.parametrize("material", ["snh", "arap"])
+def test_cg_matches_sparse_direct_solve(material):
+    m, region, mu, lam, f, fm, x0 = _gravity_case(4, material, seed=5)
+    m.cell_data["mu"] = mu
+    m.cell_data["lambda"] = lam
+    cls = fem.StableNeoHookean if material == "snh" else fem.Arap
+    pot = cls.from_region(region, dtype=torch.float64, device="cuda")
+    solver = DeviceSolver(pot)
+    x = torch.as_tensor(x0, dtype=torch.float64, device="cuda").contiguous()
+    ft = torch.as_tensor(f, dtype=torch.float64, device="cuda")
+    mask = mask_bits(fm, "cuda")
+    solver.solve(x, ft, mask, PncgConfig(rtol=1e-8))
+    rng = np.random.default_rng(1)
+    b = rng.standard_normal((m.n_points, 3))
+    bt = torch.as_tensor(b, dtype=torch.float64, device="cuda")
+    y = torch.zeros_like(bt)
+    st = solver.cg(x, bt, y, mask, rtol=1e-10, max_iters=5000)[0]
+    assert st.result == Result.PRIMARY_SUCCESS, st
+    mat = of.SNH if material == "snh" else of.ARAP
+    opot = of.FemPotential(mat, region.cells, region.dhdX, region.dV, mu, lam if mat == of.SNH else None)
+    dof = of.DofMap(fm, np.zeros_like(f))
+    prob = op.Problem(of.Model([opot]), dof)
+    H = prob.hessian(dof.to_free(x.cpu().numpy()))
+    import scipy.sparse.linalg as spla
+
+    y_ref = dof.to_full_grad(spla.spsolve(H.tocsc(), dof.to_free(b)))
+    np.testing.assert_allclose(y.cpu().numpy(), y_ref, rtol=1e-7, atol=1e-9 * np.abs(y_ref).max())
+
+
+def test_failure_codes():
+    m, region, mu, lam, f, fm, x0 = _gravity_case(3, "snh", seed=7)
+    m.cell_data["mu"] = mu
+    m.cell_data["lambda"] = lam
+    pot = fem.StableNeoHookean.from_region(region, dtype=torch.float32, device="cuda")
+    solver = DeviceSolver(pot)
+    ft = torch.as_tensor(f, dtype=torch.float32, device="cuda")
+    mask = mask_bits(fm, "cuda")
+    x = torch.as_tensor(x0, dtype=torch.float32, device="cuda").contiguous()
+    st = solver.solve(x, ft, mask, PncgConfig(max_steps=3, rtol=1e-12))[0]
+    assert st.result == Result.MAX_STEPS_REACHED and st.n_steps == 3
+    bad = pot.materials["mu"].clone()
+    bad[5] = float("nan")
+    pot.set_materials(mu=bad)
+    x = torch.as_tensor(x0, dtype=torch.float32, device="cuda").contiguous()
+    st = solver.solve(x, ft, mask, PncgConfig())[0]
+    assert st.result == Result.NON_FINITE
+
+
+def test_host_driven_pncg_for_general_models():
+    """Two FEM potentials on one mesh: the host-driven PNCG path, same equilibrium."""
+    m, region, mu, lam, f, fm, x0 = _gravity_case(4, "snh", seed=3)
+    m.cell_data["mu"] = 0.5 * mu
+    m.cell_data["lambda"] = 0.5 * lam
+    builder = forward.ModelBuilder()
+    builder.add_vertices(m)
+    m.point_data["FixedMask"] = fm
+    m.point_data["FixedValue"] = np.zeros_like(f)
+    builder.add_fixed(m)
+    a = fem.StableNeoHookean.from_region(region, dtype=torch.float64, device="cuda")
+    b2 = fem.StableNeoHookean.from_region(region, dtype=torch.float64, device="cuda")
+    builder.add_potential(a)
+    builder.add_potential(b2)
+    builder.add_potential(potential.ExternalForce(np.arange(m.n_points), f, m.n_points, dtype=torch.float64, device="cuda"))
+    model = builder.finalize()
+    assert model.fused_parts() is None
+    fwd = forward.Forward(model, forward.PNCG(rtol=1e-8, max_steps=5000))
+    sol = fwd.step()
+    assert sol.success
+    x_ref, gn = _oracle_equilibrium(region, "snh", mu, lam, f, fm, fwd.state.u.cpu().numpy())
+    assert np.abs(fwd.state.u.cpu().numpy() - x_ref).max() <= 1e-7
