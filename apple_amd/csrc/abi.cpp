@@ -270,4 +270,47 @@ int apl_elem_eval(apl_mesh* m, int material, const void* mu, const void* lam, co
   return 0;
 }
 
+int apl_bench_elem_pass(apl_mesh* m, int material, unsigned what, const void* mu, const void* lam, const void* u,
+                        const void* p, double* energy, void* out_a, void* out_b, int iters, double* ms_kernel,
+                        double* ms_pass, void* stream) {
+  if (m == nullptr || iters < 1 || ms_kernel == nullptr || ms_pass == nullptr) {
+    set_error("invalid argument");
+    return -1;
+  }
+  APL_CHECK_HIP(hipSetDevice(m->device));
+  const unsigned w = to_internal(what);
+  if (w != (W_E | W_G | W_H) && w != W_HQ && w != W_HP) {
+    set_error("bench pass must be ENERGY|GRAD|HESS_DIAG, HESS_QUAD or HESS_PROD");
+    return -1;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  std::vector<hipEvent_t> ev(3 * (size_t)iters, nullptr);
+  for (auto& e : ev) APL_CHECK_HIP(hipEventCreate(&e));
+  int rc = 0;
+  for (int it = 0; it < iters && rc == 0; ++it) {
+    APL_CHECK_HIP(hipEventRecord(ev[3 * it], s));
+    rc = launch_elem(m, material, w, mu, lam, u, p, nullptr, nullptr, out_a, out_b, 0, 0, s);
+    APL_CHECK_HIP(hipEventRecord(ev[3 * it + 1], s));
+    if (rc == 0 && (w & (W_G | W_H | W_HP))) rc = launch_shared_reduce(m, w, out_a, out_b, 0, nullptr, s);
+    if (rc == 0 && (w & (W_E | W_HQ)))
+      rc = launch_inst_reduce(m, (w & W_E) ? energy : nullptr, (w & W_HQ) ? energy : nullptr, 0, nullptr, s);
+    APL_CHECK_HIP(hipEventRecord(ev[3 * it + 2], s));
+  }
+  if (rc == 0) {
+    APL_CHECK_HIP(hipEventSynchronize(ev[3 * iters - 1]));
+    double k = 0, t = 0;
+    for (int it = 0; it < iters; ++it) {
+      float a = 0, b = 0;
+      APL_CHECK_HIP(hipEventElapsedTime(&a, ev[3 * it], ev[3 * it + 1]));
+      APL_CHECK_HIP(hipEventElapsedTime(&b, ev[3 * it], ev[3 * it + 2]));
+      k += a;
+      t += b;
+    }
+    *ms_kernel = k / iters;
+    *ms_pass = t / iters;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  return rc;
+}
+
 }  // extern "C"

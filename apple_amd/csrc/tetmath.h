@@ -235,6 +235,19 @@ __host__ __device__ inline void swap_cols(M3<R>& V, R (&e)[3], int i, int j) {
   }
 }
 
+// sigma and sigma - 1 from the eigenvalue e of F^T F - I and the QR value q:
+// near sigma = 1, sigma - 1 = e / (1 + sqrt(1 + e)) (no cancellation).
+template <class R>
+__host__ __device__ inline void sigma_from(R e, R q, R& s, R& sm1) {
+  if (fabs(e) < R(0.25) && q > R(0)) {
+    sm1 = e / (R(1) + sqrt(R(1) + e));
+    s = R(1) + sm1;
+  } else {
+    s = q;
+    sm1 = q - R(1);
+  }
+}
+
 template <class R>
 struct Svd {
   M3<R> U, V;
@@ -299,20 +312,37 @@ __host__ __device__ inline Svd<R> svd_rv(const M3<R>& G) {
   set_col(r.U, 0, u0);
   set_col(r.U, 1, u1);
   set_col(r.U, 2, u2);
-  const R sq[3] = {n0, n1, dot(u2, f2)};
+  sigma_from(e[0], n0, r.s[0], r.sm1[0]);
+  sigma_from(e[1], n1, r.s[1], r.sm1[1]);
+  sigma_from(e[2], dot(u2, f2), r.s[2], r.sm1[2]);
+  return r;
+}
+
+// Rotation-variant SVD computed in precision C and rounded to R.  The FP32
+// element kernels run the ARAP SVD in FP64: the Jacobi/QR steps are the
+// accuracy bottleneck of FP32 ARAP (stress errors up to 1e-4 relative near
+// repeated singular values), while the kernel's memory traffic -- its
+// bound -- does not change.
+template <class R, class C>
+__host__ __device__ inline Svd<R> svd_rv_in(const M3<R>& G) {
+  M3<C> Gc;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) Gc.m[i][j] = (C)G.m[i][j];
+  const Svd<C> Sc = svd_rv(Gc);
+  Svd<R> S;
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    // near sigma = 1 take sigma - 1 = e / (1 + sqrt(1 + e)) (no cancellation)
-    if (fabs(e[i]) < R(0.25) && sq[i] > R(0)) {
-      R sm1 = e[i] / (R(1) + sqrt(R(1) + e[i]));
-      r.sm1[i] = sm1;
-      r.s[i] = R(1) + sm1;
-    } else {
-      r.s[i] = sq[i];
-      r.sm1[i] = sq[i] - R(1);
+    S.s[i] = (R)Sc.s[i];
+    S.sm1[i] = (R)Sc.sm1[i];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      S.U.m[i][j] = (R)Sc.U.m[i][j];
+      S.V.m[i][j] = (R)Sc.V.m[i][j];
     }
   }
-  return r;
+  return S;
 }
 
 // U diag(d) V^T
@@ -486,7 +516,7 @@ __host__ __device__ inline void tet_eval(const TetGeom<R>& g, R mu, R lam, const
       }
     }
   } else {  // ARAP
-    const Svd<R> S = svd_rv(G);
+    const Svd<R> S = svd_rv_in<R, double>(G);
     if constexpr ((WHAT & W_E) != 0)
       o.e = g.dV * R(0.5) * mu * (S.sm1[0] * S.sm1[0] + S.sm1[1] * S.sm1[1] + S.sm1[2] * S.sm1[2]);
     if constexpr ((WHAT & W_G) != 0) {

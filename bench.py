@@ -1,0 +1,338 @@
+"""Benchmark of the FEM elasticity hot path on one MI355X.
+
+Headline (``value``): Mtet-evals/s of pass A -- energy + gradient + Hessian
+diagonal of stable Neo-Hookean over a 64^3 x 5 tet cube (1,310,720 tets,
+274,625 vertices, FP32, per-tet mu lognormal(median 1e4, sigma 0.5),
+lambda = 10 mu), one step = one full pass including the deterministic
+vertex reduction and the FP64 energy sum, inputs resident in HBM.
+
+Also reported on the same line: the dominant kernel's roofline (algorithmic
+bytes per launch / measured kernel time vs 8 TB/s), the CPU baseline (the C
+restatement of the reference's separate fun/grad/hess_diag passes, timed on
+the host), PNCG solves/s at 1e-6 relative gradient norm, the ARAP pass, and a
+batched-vs-plain solve comparison.
+
+Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--quick]
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mtet-evals/s (energy+grad+Hdiag) & PNCG solves/s, batched+adjoint vs plain"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_cube_problem(n, material, dtype, device, seed_mu=1, seed_noise=0, noise_frac=0.1):
+    """Config-2 style cube: lognormal mu, lambda = 10 mu, gravity, z = 0 pinned, noisy start."""
+    import torch
+
+    from apple_amd import fem
+    from apple_amd.mesh import Region, cube
+
+    m = cube(n)
+    rng = np.random.default_rng(seed_mu)
+    mu = 1e4 * np.exp(0.5 * rng.standard_normal(m.n_cells))
+    m.cell_data["mu"] = mu
+    m.cell_data["lambda"] = 10.0 * mu
+    region = Region.from_mesh(m)
+    cls = fem.StableNeoHookean if material == "snh" else fem.Arap
+    pot = cls.from_region(region, dtype=dtype, device=device)
+    f = np.zeros((m.n_points, 3))
+    np.add.at(f[:, 2], m.cells.reshape(-1), np.repeat(-9.8 * 1e3 * region.dV / 4.0, 4))
+    fixed = np.zeros((m.n_points, 3), bool)
+    fixed[m.points[:, 2] == 0.0] = True
+    x0 = noise_frac / n * np.random.default_rng(seed_noise).standard_normal((m.n_points, 3))
+    x0[fixed] = 0.0
+    t = lambda a: torch.as_tensor(a, dtype=dtype, device=device).contiguous()
+    return dict(mesh=m, region=region, pot=pot, f=t(f), fixed=fixed, x0=t(x0), mu=mu)
+
+
+def bench_pass(pot, u, what, iters):
+    """Mean element-kernel ms and full-pass ms over `iters` launches (HIP events)."""
+    import torch
+
+    from apple_amd import _lib
+    from apple_amd.runtime import current_stream, ptr
+
+    m = pot.mesh
+    V = m.n_verts
+    a = torch.empty(V, 3, dtype=m.dtype, device=m.device)
+    b = torch.empty_like(a)
+    e = torch.zeros(1, dtype=torch.float64, device=m.device)
+    lam = pot.materials.get("lambda")
+    p = u if what != _lib.APL_ENERGY | _lib.APL_GRAD | _lib.APL_HESS_DIAG else None
+    k = ctypes.c_double()
+    tp = ctypes.c_double()
+    _lib.check(
+        _lib.lib().apl_bench_elem_pass(
+            m._handle, int(pot.material), what, ptr(pot.materials["mu"]), ptr(lam) if lam is not None else None,
+            ptr(u), ptr(p), ptr(e), ptr(a), ptr(b), int(iters), ctypes.byref(k), ctypes.byref(tp),
+            current_stream(m.device),
+        )
+    )
+    return k.value, tp.value
+
+
+def cpu_baseline(prob, target_s=12.0, threads=1):
+    """C restatement of the reference passes, timed on the host (bounded sample)."""
+    from oracle import c_oracle
+
+    region = prob["region"]
+    m = prob["mesh"]
+    cp = c_oracle.SnhProblem(m.cells, region.dhdX, region.dV, prob["mu"], 10.0 * prob["mu"])
+    used = c_oracle.set_threads(threads)
+    u = np.ascontiguousarray(prob["x0"].double().cpu().numpy())
+    g = np.zeros_like(u)
+    h = np.zeros_like(u)
+    cp.pass_a(u, g, h)  # warm
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        cp.pass_a(u, g, h)
+        n += 1
+        if time.perf_counter() - t0 >= target_s:
+            break
+    dt = time.perf_counter() - t0
+    return dict(
+        value=round(n * m.n_cells / dt / 1e6, 4),
+        unit="Mtet-evals/s",
+        cores=int(used),
+        kind="port",
+        sample=f"{n} passes of fun+grad+hess_diag (separate passes, atomic scatter, FP64) over the same "
+        f"{m.n_cells}-tet mesh, {dt:.1f} s, {used} thread(s)",
+    )
+
+
+def load_traffic(name):
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=64, help="cube resolution (cells per side)")
+    ap.add_argument("--quick", action="store_true", help="pass timings only (no solves, no CPU baseline)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--profile-pass", action="store_true",
+                    help="only run pass-A launches and a reference copy (for rocprofv3 counter runs)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (sharding is described in DESIGN.md)"}))
+        sys.exit(2)
+
+    import torch
+
+    from apple_amd import _lib
+    from apple_amd.solver import DeviceSolver, PncgConfig, mask_bits
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    dtype = torch.float32
+    n = args.n
+    log(f"[bench] building {n}^3 x 5 cube (SNH, FP32)")
+    prob = make_cube_problem(n, "snh", dtype, dev)
+    pot = prob["pot"]
+    T, V = pot.mesh.n_tets, pot.mesh.n_verts
+    info = pot.mesh.info()
+    what_a = _lib.APL_ENERGY | _lib.APL_GRAD | _lib.APL_HESS_DIAG
+    u = prob["x0"]
+
+    if args.profile_pass:
+        bench_pass(pot, u, what_a, 20)
+        a = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device=dev)  # 256 MiB
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("[bench] profile-pass done")
+        return
+
+    # ---------------------------------------------------------------- pass A (headline)
+    log(f"[bench] warmup {args.warmup}, timed {args.steps} steps of pass A")
+    bench_pass(pot, u, what_a, max(args.warmup, 1))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ms_kernel, ms_pass = bench_pass(pot, u, what_a, args.steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall / args.steps * 1e3
+    value = T / (ms_per_step / 1e3) / 1e6
+    bytes_a = 64 * T + 36 * V
+    achieved = bytes_a / (ms_kernel / 1e3) / 1e9
+    traffic = load_traffic("traffic_pass_a_snh64.json")
+    roofline = {
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic.get("bytes_per_launch") if (traffic and n == 64) else None,
+        "kernel": "elem_kernel<SNH, E|G|H, float>",
+        "kernel_ms": round(ms_kernel, 5),
+        "pass_ms": round(ms_pass, 5),
+        "algorithmic_bytes_per_launch": bytes_a,
+    }
+    log(f"[bench] pass A: kernel {ms_kernel*1e3:.1f} us, pass {ms_pass*1e3:.1f} us, wall/step {ms_per_step*1e3:.1f} us,"
+        f" {value:.0f} Mtet-evals/s, {achieved:.0f} GB/s ({achieved/HBM_PEAK_GBS:.1%} of HBM)")
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mtet-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"pass A (energy+grad+Hdiag), stable Neo-Hookean, {n}^3 cube x 5 tets, "
+            "mu lognormal(1e4, 0.5), lambda = 10 mu, seeded noisy displacement",
+            "n_tets": T,
+            "n_verts": V,
+            "n_blocks": info["n_blocks"],
+            "parallelism": "single GPU",
+        },
+        "roofline": roofline,
+    }
+
+    # ---------------------------------------------------------------- other passes
+    k_hq, p_hq = bench_pass(pot, u, _lib.APL_HESS_QUAD, 20)
+    out["pass_b_hess_quad"] = {
+        "kernel_ms": round(k_hq, 5),
+        "hbm_frac": round((64 * T + 24 * V) / (k_hq / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+    }
+    k_hp, p_hp = bench_pass(pot, u, _lib.APL_HESS_PROD, 20)
+    out["pass_c_hess_prod"] = {
+        "kernel_ms": round(k_hp, 5),
+        "hbm_frac": round((64 * T + 36 * V) / (k_hp / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+    }
+    if not args.quick:
+        log("[bench] ARAP pass A")
+        aprob = make_cube_problem(n, "arap", dtype, dev)
+        ka, pa = bench_pass(aprob["pot"], aprob["x0"], what_a, 20)
+        out["arap_pass_a"] = {
+            "mtet_evals_per_s": round(T / (pa / 1e3) / 1e6, 1),
+            "kernel_ms": round(ka, 5),
+            "hbm_frac": round((60 * T + 36 * V) / (ka / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+        }
+        del aprob
+
+    # ---------------------------------------------------------------- PNCG solve
+    if not (args.quick or args.no_solve):
+        log("[bench] PNCG solve (SNH, rtol 1e-6)")
+        solver = DeviceSolver(pot)
+        mask = mask_bits(prob["fixed"], dev)
+        cfg = PncgConfig(rtol=1e-6, max_steps=20000)
+        xw = prob["x0"].clone()
+        solver.solve(xw, prob["f"], mask, PncgConfig(rtol=1e-6, max_steps=16))  # graph warmup
+        x = prob["x0"].clone()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = solver.solve(x, prob["f"], mask, cfg)[0]
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        out["pncg"] = {
+            "solves_per_s": round(1.0 / dt, 3),
+            "solve_s": round(dt, 4),
+            "result": st.result.name,
+            "steps": st.n_steps,
+            "evals": st.n_evals,
+            "relative_grad_norm": st.relative_grad_norm,
+            "ms_per_iteration": round(dt / max(st.n_evals, 1) * 1e3, 4),
+        }
+        log(f"[bench] PNCG: {st.result.name} in {st.n_steps} steps, {dt:.3f} s")
+
+    # ---------------------------------------------------------------- batched vs plain
+    if not (args.quick or args.no_batch):
+        out["batched"] = bench_batched(dev, dtype)
+
+    # ---------------------------------------------------------------- CPU baseline
+    if not (args.quick or args.no_cpu):
+        log("[bench] CPU baseline (C restatement, 1 thread)")
+        out["cpu_baseline"] = cpu_baseline(prob, threads=1)
+    print(json.dumps(out), flush=True)
+
+
+def bench_batched(dev, dtype, n_meshes=32, lo=6, hi=16, seed=2):
+    """Batched PNCG over uneven meshes vs the same solves one instance at a time."""
+    import torch
+
+    from apple_amd import batch
+    from apple_amd.mesh import Region, cube
+    from apple_amd.solver import PncgConfig
+
+    rng = np.random.default_rng(seed)
+    mrng = np.random.default_rng(seed + 1)
+    insts = []
+    for k in range(n_meshes):
+        n = int(rng.integers(lo, hi + 1))
+        m = cube(n)
+        region = Region.from_mesh(m)
+        mu_i = mrng.uniform(1e3, 1e5)
+        nu = mrng.uniform(0.3, 0.45)
+        mu = np.full(m.n_cells, mu_i)
+        lam = 2 * mu * nu / (1 - 2 * nu)
+        f = np.zeros((m.n_points, 3))
+        np.add.at(f[:, 2], m.cells.reshape(-1), np.repeat(-9.8e3 * region.dV / 4, 4))
+        fixed = np.zeros((m.n_points, 3), bool)
+        fixed[m.points[:, 2] == 0] = True
+        x0 = 0.1 / n * np.random.default_rng(100 + k).standard_normal((m.n_points, 3))
+        x0[fixed] = 0
+        insts.append(batch.Instance(region, mu, lam, f, fixed, x0))
+    cfg = PncgConfig(rtol=1e-6, max_steps=20000)
+    b = batch.Batch(insts, 0, dtype=dtype, device=dev)
+    b.solve(config=PncgConfig(rtol=1e-6, max_steps=16))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    xb, stb = b.solve(config=cfg)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    singles = [batch.Batch([inst], 0, dtype=dtype, device=dev) for inst in insts]
+    for s in singles:
+        s.solve(config=PncgConfig(rtol=1e-6, max_steps=16))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ok = 0
+    for s in singles:
+        _, st = s.solve(config=cfg)
+        ok += st[0].success
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    return {
+        "workload": f"{n_meshes} cubes, n ~ U{{{lo}..{hi}}}, {b.n_tets} tets total, SNH FP32, PNCG to 1e-6",
+        "batched_solves_per_s": round(n_meshes / t_batch, 3),
+        "plain_solves_per_s": round(n_meshes / t_plain, 3),
+        "speedup": round(t_plain / t_batch, 3),
+        "batched_all_converged": all(s.success for s in stb),
+        "plain_all_converged": ok == n_meshes,
+    }
+
+
+if __name__ == "__main__":
+    main()
