@@ -98,15 +98,18 @@ int apl_mesh_create(int dtype, int64_t n_tets, int64_t n_verts, const int32_t* c
     inst_vtx_off = one_v;
   }
   APL_CHECK_HIP(hipSetDevice(device));
+  (void)hipGetLastError();  // drop errors left by unrelated earlier HIP calls
   apl_mesh* m = new apl_mesh();
   m->dtype = dtype;
   m->device = device;
   int ncu = 0;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
     m->n_cu = ncu;
+  // LDS budget (64 KiB per block): FP64 blocks hold at most 512 slots and one chunk
   const int nloc_cap = dtype == APL_F64 ? 512 : 1024;
-  std::string err = build_layout(cells, n_tets, n_verts, n_inst, inst_tet_off, inst_vtx_off, NTH,
-                                 chunks < 1 ? 1 : chunks, nloc_cap, m->L);
+  if (chunks < 1 || dtype == APL_F64) chunks = 1;
+  std::string err = build_layout(cells, n_tets, n_verts, n_inst, inst_tet_off, inst_vtx_off, NTH, chunks, nloc_cap,
+                                 m->L);
   if (!err.empty()) {
     set_error(err);
     delete m;
@@ -240,6 +243,7 @@ int apl_elem_eval(apl_mesh* m, int material, const void* mu, const void* lam, co
     return -1;
   }
   APL_CHECK_HIP(hipSetDevice(m->device));
+  (void)hipGetLastError();  // drop errors left by unrelated earlier HIP calls
   hipStream_t s = (hipStream_t)stream;
   const unsigned egh = w & (W_E | W_G | W_H);
   if (egh) {
@@ -278,6 +282,7 @@ int apl_bench_elem_pass(apl_mesh* m, int material, unsigned what, const void* mu
     return -1;
   }
   APL_CHECK_HIP(hipSetDevice(m->device));
+  (void)hipGetLastError();  // drop errors left by unrelated earlier HIP calls
   const unsigned w = to_internal(what);
   if (w != (W_E | W_G | W_H) && w != W_HQ && w != W_HP) {
     set_error("bench pass must be ENERGY|GRAD|HESS_DIAG, HESS_QUAD or HESS_PROD");

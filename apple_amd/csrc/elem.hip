@@ -27,7 +27,7 @@
 
 namespace apl {
 
-template <class R>
+template <class R, class X>
 struct ElemArgs {
   const R* dminv;
   const R* dV;
@@ -42,7 +42,7 @@ struct ElemArgs {
   const uint16_t* loc_inc;
   const R* mu;
   const R* lam;
-  const R* x;
+  const X* x;  // displacements (X = R, or double when the solver keeps them in FP64)
   const R* p;
   const double* alpha;
   const int32_t* active;
@@ -54,6 +54,7 @@ struct ElemArgs {
   long long n_partials;
   int nblocks;
   int nloc_max;
+  int chunks;
   int accumulate;
 };
 
@@ -72,8 +73,53 @@ __device__ inline double block_sum(double v, double* red) {
   return r;
 }
 
-template <int MAT, unsigned WHAT, class R>
-__global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R> A) {
+// Two block sums in one fixed-order tree (result valid in thread 0).
+__device__ inline void block_sum2(double& a, double& b, double* red) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_down(a, off, 64);
+    b += __shfl_down(b, off, 64);
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+    red[2 * w] = a;
+    red[2 * w + 1] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    a = 0.0;
+    b = 0.0;
+#pragma unroll
+    for (int i = 0; i < NTH / 64; ++i) {
+      a += red[2 * i];
+      b += red[2 * i + 1];
+    }
+  }
+  __syncthreads();
+}
+
+// Per-tet record, loaded into registers ahead of the block's LDS staging.
+template <class R>
+struct TetRec {
+  R B[9];
+  R dV, mu, lam;
+  ushort4 sl;
+};
+
+template <int MAT, class R, class X>
+__device__ inline void load_rec(const ElemArgs<R, X>& A, long long t, TetRec<R>& r) {
+#pragma unroll
+  for (int kk = 0; kk < 9; ++kk) r.B[kk] = A.dminv[kk * A.T + t];
+  r.dV = A.dV[t];
+  r.mu = A.mu[t];
+  r.lam = (MAT == MAT_SNH) ? A.lam[t] : R(0);
+  r.sl = A.slot[t];
+}
+
+// SPT_: local slots reduced per thread (nloc_max <= SPT_ * NTH)
+template <int MAT, unsigned WHAT, class R, class X, int SPT_>
+__global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R, X> A) {
+  constexpr int SPT = SPT_;
   constexpr bool HAS_G = (WHAT & W_G) != 0;
   constexpr bool HAS_H = (WHAT & W_H) != 0;
   constexpr bool HAS_HP = (WHAT & W_HP) != 0;
@@ -86,9 +132,12 @@ __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R> A) {
 
   extern __shared__ double smem_d[];
   double* red = smem_d;
-  R* u_lds = reinterpret_cast<R*>(smem_d + 8);
-  R* v_lds = u_lds + 3 * A.nloc_max;
+  X* u_lds = reinterpret_cast<X*>(smem_d + 8);
+  R* v_lds = reinterpret_cast<R*>(u_lds + 3 * A.nloc_max);
   R* contrib = v_lds + (NEEDV ? 3 * A.nloc_max : 0);
+  // the block's incidence CSR (sorted by local slot) staged next to contrib
+  uint16_t* inc_lds = reinterpret_cast<uint16_t*>(contrib + (SCATTER ? NCH * 4 * NTH : 0));
+  uint16_t* off_lds = inc_lds + (SCATTER ? 4 * NTH * A.chunks : 0);
 
   const int tid = threadIdx.x;
   const int pid = blockIdx.x;
@@ -104,16 +153,21 @@ __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R> A) {
     const int sb = A.blk_loc_off[b];
     const int nloc = A.blk_loc_off[b + 1] - sb;
     const int nown = A.blk_nown[b];
-    const R alpha = A.alpha != nullptr ? (R)A.alpha[inst] : R(0);
+    const X alpha = A.alpha != nullptr ? (X)A.alpha[inst] : X(0);
+
+    // this thread's first tet record, loaded before the staging barrier so the
+    // two global-load latencies overlap
+    TetRec<R> rec;
+    if (tid < nt) load_rec<MAT>(A, (long long)t0 + tid, rec);
 
     // stage the block's vertex window (u = x + alpha p for a trial point)
     for (int s = tid; s < nloc; s += NTH) {
       const long long gid = A.loc_gid[sb + s];
-      R ux = A.x[3 * gid], uy = A.x[3 * gid + 1], uz = A.x[3 * gid + 2];
+      X ux = A.x[3 * gid], uy = A.x[3 * gid + 1], uz = A.x[3 * gid + 2];
       if (A.alpha != nullptr) {
-        ux += alpha * A.p[3 * gid];
-        uy += alpha * A.p[3 * gid + 1];
-        uz += alpha * A.p[3 * gid + 2];
+        ux += alpha * (X)A.p[3 * gid];
+        uy += alpha * (X)A.p[3 * gid + 1];
+        uz += alpha * (X)A.p[3 * gid + 2];
       }
       u_lds[3 * s] = ux;
       u_lds[3 * s + 1] = uy;
@@ -124,10 +178,14 @@ __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R> A) {
         v_lds[3 * s + 2] = A.p[3 * gid + 2];
       }
     }
+    if constexpr (SCATTER) {
+      const uint16_t* inc_g = A.loc_inc + 4LL * t0;
+      for (int k = tid; k < 4 * nt; k += NTH) inc_lds[k] = inc_g[k];
+      const uint16_t* off_g = A.loc_inc_off + sb + b;
+      for (int k = tid; k <= nloc; k += NTH) off_lds[k] = off_g[k];
+    }
     __syncthreads();
 
-    const uint16_t* inc_off = A.loc_inc_off + sb + b;
-    const uint16_t* inc = A.loc_inc + 4LL * t0;
     R acc[SPT][NCHA];
     int cur[SPT], endi[SPT];
 #pragma unroll
@@ -136,8 +194,8 @@ __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R> A) {
       cur[k] = 0;
       endi[k] = 0;
       if (SCATTER && s < nloc) {
-        cur[k] = inc_off[s];
-        endi[k] = inc_off[s + 1];
+        cur[k] = off_lds[s];
+        endi[k] = off_lds[s + 1];
       }
 #pragma unroll
       for (int ch = 0; ch < NCHA; ++ch) acc[k][ch] = R(0);
@@ -148,23 +206,24 @@ __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R> A) {
       const int tl = c * NTH + tid;
       if (tl < nt) {
         const long long t = t0 + tl;
-        R B[9];
-#pragma unroll
-        for (int kk = 0; kk < 9; ++kk) B[kk] = A.dminv[kk * A.T + t];
-        const TetGeom<R> g = make_geom(B, A.dV[t]);
-        const ushort4 sl = A.slot[t];
-        const int sidx[4] = {sl.x, sl.y, sl.z, sl.w};
+        if (c > 0) load_rec<MAT>(A, t, rec);
+        const TetGeom<R> g = make_geom(rec.B, rec.dV);
+        const int sidx[4] = {rec.sl.x, rec.sl.y, rec.sl.z, rec.sl.w};
+        // edge differences u_a - u_0 in the storage precision X, then R: F - I
+        // depends only on them, and forming them before rounding keeps F
+        // accurate relative to the strain rather than to |u|.
         V3<R> u[4], v[4];
+        const X u0x = u_lds[3 * sidx[0]], u0y = u_lds[3 * sidx[0] + 1], u0z = u_lds[3 * sidx[0] + 2];
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
-          u[a] = v3(u_lds[3 * sidx[a]], u_lds[3 * sidx[a] + 1], u_lds[3 * sidx[a] + 2]);
+          u[a] = v3((R)(u_lds[3 * sidx[a]] - u0x), (R)(u_lds[3 * sidx[a] + 1] - u0y), (R)(u_lds[3 * sidx[a] + 2] - u0z));
           if constexpr (NEEDV)
             v[a] = v3(v_lds[3 * sidx[a]], v_lds[3 * sidx[a] + 1], v_lds[3 * sidx[a] + 2]);
           else
             v[a] = v3(R(0), R(0), R(0));
         }
-        const R mu = A.mu[t];
-        const R lam = (MAT == MAT_SNH) ? A.lam[t] : R(0);
+        const R mu = rec.mu;
+        const R lam = rec.lam;
         TetOut<R> o;
         tet_eval<MAT, WHAT, R>(g, mu, lam, u, v, o);
         if constexpr ((WHAT & W_E) != 0) e_acc += (double)o.e;
@@ -210,18 +269,31 @@ __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R> A) {
       }
       if constexpr (SCATTER) {
         __syncthreads();
-        const int lim = (c + 1) * 4 * NTH;
-        const int base = c * 4 * NTH;
+        if (nchunks == 1) {
+          // whole incidence list of every slot lies in this chunk
 #pragma unroll
-        for (int k = 0; k < SPT; ++k) {
-          while (cur[k] < endi[k]) {
-            const int id = inc[cur[k]];
-            if (id >= lim) break;
-            const int j = id - base;
-            const int tlc = j >> 2, a = j & 3;
+          for (int k = 0; k < SPT; ++k) {
+            for (int q = cur[k]; q < endi[k]; ++q) {
+              const int id = inc_lds[q];
+              const int tlc = id >> 2, a = id & 3;
 #pragma unroll
-            for (int ch = 0; ch < NCH; ++ch) acc[k][ch] += contrib[(ch * 4 + a) * NTH + tlc];
-            ++cur[k];
+              for (int ch = 0; ch < NCH; ++ch) acc[k][ch] += contrib[(ch * 4 + a) * NTH + tlc];
+            }
+          }
+        } else {
+          const int lim = (c + 1) * 4 * NTH;
+          const int base = c * 4 * NTH;
+#pragma unroll
+          for (int k = 0; k < SPT; ++k) {
+            while (cur[k] < endi[k]) {
+              const int id = inc_lds[cur[k]];
+              if (id >= lim) break;
+              const int j = id - base;
+              const int tlc = j >> 2, a = j & 3;
+#pragma unroll
+              for (int ch = 0; ch < NCH; ++ch) acc[k][ch] += contrib[(ch * 4 + a) * NTH + tlc];
+              ++cur[k];
+            }
           }
         }
         __syncthreads();
@@ -252,17 +324,17 @@ __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R> A) {
           }
         } else {
           const long long pidx = (long long)A.blk_sh_base[b] + (s - nown);
+          R* dst = A.partial + pidx * NCH;
 #pragma unroll
-          for (int ch = 0; ch < NCH; ++ch) A.partial[ch * A.n_partials + pidx] = acc[k][ch];
+          for (int ch = 0; ch < NCH; ++ch) dst[ch] = acc[k][ch];
         }
       }
     }
     if constexpr (RED) {
-      const double e = block_sum(e_acc, red);
-      const double q = block_sum(q_acc, red);
+      block_sum2(e_acc, q_acc, red);
       if (tid == 0) {
-        A.blk_red[2 * b] = e;
-        A.blk_red[2 * b + 1] = q;
+        A.blk_red[2 * b] = e_acc;
+        A.blk_red[2 * b + 1] = q_acc;
       }
     }
     __syncthreads();
@@ -291,9 +363,9 @@ __global__ __launch_bounds__(NTH) void shared_reduce_kernel(
 #pragma unroll
   for (int ch = 0; ch < NCH; ++ch) acc[ch] = R(0);
   for (int k = sh_off[i]; k < sh_off[i + 1]; ++k) {
-    const long long pidx = sh_src[k];
+    const R* src = partial + (long long)sh_src[k] * NCH;
 #pragma unroll
-    for (int ch = 0; ch < NCH; ++ch) acc[ch] += partial[ch * n_partials + pidx];
+    for (int ch = 0; ch < NCH; ++ch) acc[ch] += src[ch];
   }
 #pragma unroll
   for (int grp = 0; grp < NCH / 3; ++grp) {
@@ -311,33 +383,60 @@ __global__ __launch_bounds__(NTH) void shared_reduce_kernel(
 }
 
 // Per-instance sums of the block energies and quadratic forms, fixed order.
-__global__ __launch_bounds__(NTH) void inst_reduce_kernel(const double* blk_red, const int64_t* inst_blk_off,
-                                                          const int32_t* active, double* out_e,
-                                                          double* out_q, int accumulate) {
-  __shared__ double red[8];
+constexpr int RED_TH = 1024;
+__global__ __launch_bounds__(RED_TH) void inst_reduce_kernel(const double* blk_red, const int64_t* inst_blk_off,
+                                                             const int32_t* active, double* out_e,
+                                                             double* out_q, int accumulate) {
+  __shared__ double red[2 * (RED_TH / 64)];
   const int i = blockIdx.x;
   if (active != nullptr && active[i] == 0) return;
   const long long b0 = inst_blk_off[i], b1 = inst_blk_off[i + 1];
-  double e = 0.0, q = 0.0;
-  for (long long b = b0 + threadIdx.x; b < b1; b += NTH) {
-    e += blk_red[2 * b];
-    q += blk_red[2 * b + 1];
+  const double2* br = reinterpret_cast<const double2*>(blk_red);
+  double e[4] = {0, 0, 0, 0}, q[4] = {0, 0, 0, 0};
+  long long b = b0 + threadIdx.x;
+  for (; b + 3 * RED_TH < b1; b += 4 * RED_TH) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double2 v = br[b + u * RED_TH];
+      e[u] += v.x;
+      q[u] += v.y;
+    }
   }
-  e = block_sum(e, red);
-  q = block_sum(q, red);
+  for (int u = 0; b < b1; b += RED_TH, ++u) {
+    const double2 v = br[b];
+    e[u & 3] += v.x;
+    q[u & 3] += v.y;
+  }
+  double ee = (e[0] + e[1]) + (e[2] + e[3]);
+  double qq = (q[0] + q[1]) + (q[2] + q[3]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    ee += __shfl_down(ee, off, 64);
+    qq += __shfl_down(qq, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[2 * (threadIdx.x >> 6)] = ee;
+    red[2 * (threadIdx.x >> 6) + 1] = qq;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
-    if (out_e != nullptr) out_e[i] = accumulate ? out_e[i] + e : e;
-    if (out_q != nullptr) out_q[i] = accumulate ? out_q[i] + q : q;
+    double se = 0.0, sq = 0.0;
+    for (int w = 0; w < RED_TH / 64; ++w) {
+      se += red[2 * w];
+      sq += red[2 * w + 1];
+    }
+    if (out_e != nullptr) out_e[i] = accumulate ? out_e[i] + se : se;
+    if (out_q != nullptr) out_q[i] = accumulate ? out_q[i] + sq : sq;
   }
 }
 
 // ------------------------------------------------------------------ launchers
 
-template <class R>
-static ElemArgs<R> make_args(apl_mesh* m, const void* mu, const void* lam, const void* x, const void* p,
-                             const double* alpha, const int32_t* active, void* out_a, void* out_b,
-                             int accumulate) {
-  ElemArgs<R> A;
+template <class R, class X>
+static ElemArgs<R, X> make_args(apl_mesh* m, const void* mu, const void* lam, const void* x, const void* p,
+                                const double* alpha, const int32_t* active, void* out_a, void* out_b,
+                                int accumulate) {
+  ElemArgs<R, X> A;
   A.dminv = (const R*)m->d.dminv;
   A.dV = (const R*)m->d.dV;
   A.slot = (const ushort4*)m->d.slot;
@@ -351,7 +450,7 @@ static ElemArgs<R> make_args(apl_mesh* m, const void* mu, const void* lam, const
   A.loc_inc = m->d.loc_inc;
   A.mu = (const R*)mu;
   A.lam = (const R*)lam;
-  A.x = (const R*)x;
+  A.x = (const X*)x;
   A.p = (const R*)p;
   A.alpha = alpha;
   A.active = active;
@@ -363,27 +462,29 @@ static ElemArgs<R> make_args(apl_mesh* m, const void* mu, const void* lam, const
   A.n_partials = m->L.n_partials;
   A.nblocks = (int)m->L.nblocks;
   A.nloc_max = m->L.nloc_max;
+  A.chunks = m->L.chunks;
   A.accumulate = accumulate;
   return A;
 }
 
-template <int MAT, unsigned WHAT, class R>
+template <int MAT, unsigned WHAT, class R, class X>
 static size_t smem_bytes(const apl_mesh* m) {
   constexpr bool NEEDV = (WHAT & (W_HP | W_HQ | W_MIX)) != 0;
   constexpr int NCH = ((WHAT & W_G) ? 3 : 0) + ((WHAT & W_H) ? 3 : 0) + ((WHAT & W_HP) ? 3 : 0);
-  size_t n = 8 * sizeof(double) + sizeof(R) * 3 * (size_t)m->L.nloc_max * (NEEDV ? 2 : 1);
+  size_t n = 8 * sizeof(double) + 3 * (size_t)m->L.nloc_max * (sizeof(X) + (NEEDV ? sizeof(R) : 0));
   n += sizeof(R) * (size_t)NCH * 4 * NTH;
+  if (NCH > 0) n += sizeof(uint16_t) * ((size_t)4 * NTH * m->L.chunks + (size_t)m->L.nloc_max + 1);
   return n;
 }
 
-template <int MAT, unsigned WHAT, class R>
-static int grid_for(apl_mesh* m) {
+template <int MAT, unsigned WHAT, class R, class X, int SPT>
+static int grid_for_spt(apl_mesh* m) {
   static_assert(WHAT < 64, "pass variant index");
-  int& cached = m->grid_cache[MAT][WHAT];
+  int& cached = m->grid_cache[MAT][WHAT + (sizeof(X) != sizeof(R) ? 64 : 0)];
   if (cached > 0) return cached;
-  const size_t smem = smem_bytes<MAT, WHAT, R>(m);
+  const size_t smem = smem_bytes<MAT, WHAT, R, X>(m);
   int occ = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, elem_kernel<MAT, WHAT, R>, NTH, smem) != hipSuccess ||
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, elem_kernel<MAT, WHAT, R, X, SPT>, NTH, smem) != hipSuccess ||
       occ < 1)
     occ = 1;
   long long g = (long long)occ * m->n_cu;
@@ -395,19 +496,39 @@ static int grid_for(apl_mesh* m) {
   return cached;
 }
 
-template <int MAT, unsigned WHAT, class R>
+// slots per thread for this mesh's largest block window
+inline int spt_for(const apl_mesh* m) { return m->L.nloc_max <= NTH ? 1 : (m->L.nloc_max <= 2 * NTH ? 2 : 4); }
+
+template <int MAT, unsigned WHAT, class R, class X>
+static int grid_for(apl_mesh* m) {
+  constexpr bool SCATTER = (WHAT & (W_G | W_H | W_HP)) != 0;
+  if (!SCATTER) return grid_for_spt<MAT, WHAT, R, X, 1>(m);
+  const int spt = spt_for(m);
+  if (spt == 1) return grid_for_spt<MAT, WHAT, R, X, 1>(m);
+  if (spt == 2) return grid_for_spt<MAT, WHAT, R, X, 2>(m);
+  return grid_for_spt<MAT, WHAT, R, X, 4>(m);
+}
+
+template <int MAT, unsigned WHAT, class R, class X>
 static int launch_one(apl_mesh* m, const void* mu, const void* lam, const void* x, const void* p,
                       const double* alpha, const int32_t* active, void* out_a, void* out_b,
                       int accumulate, int grid_override, hipStream_t s) {
   if (m->L.nblocks == 0) return 0;
-  const size_t smem = smem_bytes<MAT, WHAT, R>(m);
+  const size_t smem = smem_bytes<MAT, WHAT, R, X>(m);
   if (smem > 65536) {
     set_error("element block needs more than 64 KiB of LDS; rebuild the mesh with a smaller nloc_cap");
     return -1;
   }
-  const int grid = grid_override > 0 ? ((grid_override + 7) / 8) * 8 : grid_for<MAT, WHAT, R>(m);
-  ElemArgs<R> A = make_args<R>(m, mu, lam, x, p, alpha, active, out_a, out_b, accumulate);
-  hipLaunchKernelGGL((elem_kernel<MAT, WHAT, R>), dim3(grid), dim3(NTH), smem, s, A);
+  const int grid = grid_override > 0 ? ((grid_override + 7) / 8) * 8 : grid_for<MAT, WHAT, R, X>(m);
+  ElemArgs<R, X> A = make_args<R, X>(m, mu, lam, x, p, alpha, active, out_a, out_b, accumulate);
+  constexpr bool SCATTER = (WHAT & (W_G | W_H | W_HP)) != 0;
+  const int spt = SCATTER ? spt_for(m) : 1;
+  if (spt == 1)
+    hipLaunchKernelGGL((elem_kernel<MAT, WHAT, R, X, 1>), dim3(grid), dim3(NTH), smem, s, A);
+  else if (spt == 2)
+    hipLaunchKernelGGL((elem_kernel<MAT, WHAT, R, X, SCATTER ? 2 : 1>), dim3(grid), dim3(NTH), smem, s, A);
+  else
+    hipLaunchKernelGGL((elem_kernel<MAT, WHAT, R, X, SCATTER ? 4 : 1>), dim3(grid), dim3(NTH), smem, s, A);
   APL_CHECK_HIP(hipGetLastError());
   return 0;
 }
@@ -425,9 +546,18 @@ static int launch_one(apl_mesh* m, const void* mu, const void* lam, const void* 
 template <int MAT, class R>
 static int dispatch_what(apl_mesh* m, unsigned what, const void* mu, const void* lam, const void* x,
                          const void* p, const double* alpha, const int32_t* active, void* out_a,
-                         void* out_b, int accumulate, int grid_override, hipStream_t s) {
+                         void* out_b, int accumulate, int grid_override, int x_f64, hipStream_t s) {
+  if (x_f64 && sizeof(R) == 4) {
+    // solver passes over FP64 displacements
+    if (what == (W_E | W_G | W_H))
+      return launch_one<MAT, W_E | W_G | W_H, R, double>(m, mu, lam, x, p, alpha, active, out_a, out_b, accumulate, grid_override, s);
+    if (what == W_HQ)
+      return launch_one<MAT, W_HQ, R, double>(m, mu, lam, x, p, alpha, active, out_a, out_b, accumulate, grid_override, s);
+    set_error("FP64 displacements are supported for the solver passes only");
+    return -1;
+  }
 #define APL_CASE(W) \
-  if (what == (W)) return launch_one<MAT, (W), R>(m, mu, lam, x, p, alpha, active, out_a, out_b, accumulate, grid_override, s);
+  if (what == (W)) return launch_one<MAT, (W), R, R>(m, mu, lam, x, p, alpha, active, out_a, out_b, accumulate, grid_override, s);
   APL_WHAT_LIST(APL_CASE)
 #undef APL_CASE
   set_error("unsupported combination of outputs for one element pass");
@@ -436,32 +566,37 @@ static int dispatch_what(apl_mesh* m, unsigned what, const void* mu, const void*
 
 int launch_elem(apl_mesh* m, int material, unsigned what, const void* mu, const void* lam, const void* x,
                 const void* p, const double* alpha, const int32_t* active, void* out_a, void* out_b,
-                int accumulate, int grid_override, hipStream_t s) {
+                int accumulate, int grid_override, hipStream_t s, int x_f64) {
   if (material == MAT_SNH) {
     if (m->dtype == DT_F32)
-      return dispatch_what<MAT_SNH, float>(m, what, mu, lam, x, p, alpha, active, out_a, out_b, accumulate, grid_override, s);
-    return dispatch_what<MAT_SNH, double>(m, what, mu, lam, x, p, alpha, active, out_a, out_b, accumulate, grid_override, s);
+      return dispatch_what<MAT_SNH, float>(m, what, mu, lam, x, p, alpha, active, out_a, out_b, accumulate, grid_override, x_f64, s);
+    return dispatch_what<MAT_SNH, double>(m, what, mu, lam, x, p, alpha, active, out_a, out_b, accumulate, grid_override, 0, s);
   }
   if (material == MAT_ARAP) {
     if (m->dtype == DT_F32)
-      return dispatch_what<MAT_ARAP, float>(m, what, mu, lam, x, p, alpha, active, out_a, out_b, accumulate, grid_override, s);
-    return dispatch_what<MAT_ARAP, double>(m, what, mu, lam, x, p, alpha, active, out_a, out_b, accumulate, grid_override, s);
+      return dispatch_what<MAT_ARAP, float>(m, what, mu, lam, x, p, alpha, active, out_a, out_b, accumulate, grid_override, x_f64, s);
+    return dispatch_what<MAT_ARAP, double>(m, what, mu, lam, x, p, alpha, active, out_a, out_b, accumulate, grid_override, 0, s);
   }
   set_error("unknown material");
   return -1;
 }
 
 template <int MAT>
-static int grid_mat(apl_mesh* m, unsigned what) {
+static int grid_mat(apl_mesh* m, unsigned what, int x_f64) {
+  if (x_f64 && m->dtype == DT_F32) {
+    if (what == (W_E | W_G | W_H)) return grid_for<MAT, W_E | W_G | W_H, float, double>(m);
+    if (what == W_HQ) return grid_for<MAT, W_HQ, float, double>(m);
+    return 0;
+  }
 #define APL_CASE(W) \
-  if (what == (W)) return m->dtype == DT_F32 ? grid_for<MAT, (W), float>(m) : grid_for<MAT, (W), double>(m);
+  if (what == (W)) return m->dtype == DT_F32 ? grid_for<MAT, (W), float, float>(m) : grid_for<MAT, (W), double, double>(m);
   APL_WHAT_LIST(APL_CASE)
 #undef APL_CASE
   return 0;
 }
 
-int elem_grid(apl_mesh* m, int material, unsigned what) {
-  return material == MAT_SNH ? grid_mat<MAT_SNH>(m, what) : grid_mat<MAT_ARAP>(m, what);
+int elem_grid(apl_mesh* m, int material, unsigned what, int x_f64) {
+  return material == MAT_SNH ? grid_mat<MAT_SNH>(m, what, x_f64) : grid_mat<MAT_ARAP>(m, what, x_f64);
 }
 
 template <class R, int NCH>
@@ -492,7 +627,7 @@ int launch_shared_reduce(apl_mesh* m, unsigned what, void* out_a, void* out_b, i
 
 int launch_inst_reduce(apl_mesh* m, double* out_e, double* out_q, int accumulate, const int32_t* active,
                        hipStream_t s) {
-  hipLaunchKernelGGL(inst_reduce_kernel, dim3((unsigned)m->L.n_inst), dim3(NTH), 0, s, m->d.blk_red,
+  hipLaunchKernelGGL(inst_reduce_kernel, dim3((unsigned)m->L.n_inst), dim3(RED_TH), 0, s, m->d.blk_red,
                      m->d.inst_blk_off, active, out_e, out_q, accumulate);
   APL_CHECK_HIP(hipGetLastError());
   return 0;

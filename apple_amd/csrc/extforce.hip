@@ -107,6 +107,7 @@ extern "C" int apl_ext_force_eval(apl_mesh* m, int64_t n, const int32_t* indices
     return -1;
   }
   APL_CHECK_HIP(hipSetDevice(m->device));
+  (void)hipGetLastError();  // drop errors left by unrelated earlier HIP calls
   if (m->dtype == DT_F32)
     return ext_eval<float>(m, n, indices, force, u, energy, grad, accumulate, (hipStream_t)stream);
   return ext_eval<double>(m, n, indices, force, u, energy, grad, accumulate, (hipStream_t)stream);

@@ -12,7 +12,6 @@
 namespace apl {
 
 constexpr int NTH = 256;       // threads per element / vector block
-constexpr int SPT = 4;         // local slots reduced per thread (nloc_cap / NTH)
 constexpr int MAX_CH = 6;      // scatter channels per incidence (grad + hdiag)
 constexpr int N_DOTS = 16;     // per vertex-block partial dot products
 
@@ -65,7 +64,7 @@ struct apl_mesh {
   apl::Layout L;
   apl::DevLayout d;
   size_t device_bytes = 0;
-  int grid_cache[2][64] = {};  // persistent grid per (material, pass variant)
+  int grid_cache[2][128] = {};  // persistent grid per (material, pass variant, FP64 displacements)
 };
 
 namespace apl {
@@ -74,15 +73,16 @@ inline size_t real_size(const apl_mesh* m) { return m->dtype == DT_F64 ? 8 : 4; 
 
 // element pass: per-block partials (vertex sums of owned slots are written
 // directly into out_a/out_b; shared slots into the partial workspace)
+// x_f64: x holds FP64 displacements while the mesh is FP32 (solver passes)
 int launch_elem(apl_mesh* m, int material, unsigned what, const void* mu, const void* lam,
                 const void* x, const void* p, const double* alpha, const int32_t* active,
-                void* out_a, void* out_b, int accumulate, int grid_override, hipStream_t s);
+                void* out_a, void* out_b, int accumulate, int grid_override, hipStream_t s, int x_f64 = 0);
 // vertex sums of shared vertices from the partials
 int launch_shared_reduce(apl_mesh* m, unsigned what, void* out_a, void* out_b, int accumulate,
                          const int32_t* active, hipStream_t s);
 // per-instance sums of the block energies / quadratic forms
 int launch_inst_reduce(apl_mesh* m, double* out_e, double* out_q, int accumulate,
                        const int32_t* active, hipStream_t s);
-int elem_grid(apl_mesh* m, int material, unsigned what);
+int elem_grid(apl_mesh* m, int material, unsigned what, int x_f64 = 0);
 
 }  // namespace apl

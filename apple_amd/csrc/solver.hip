@@ -24,6 +24,7 @@
 // batch runs in the same launches; finished instances drop out via flags.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -78,7 +79,7 @@ struct VecArgs {
   long long n_partials;
   const uint8_t* mask;
   const R* fext;
-  R* x;
+  double* x;  // iterate, always FP64 (the line-search step is taken in FP64)
   R* p;
   R* gc;
   R* gE;
@@ -90,6 +91,12 @@ struct VecArgs {
 template <class R>
 __global__ __launch_bounds__(NTH) void fill_kernel(R* a, long long n, R v) {
   for (long long i = (long long)blockIdx.x * NTH + threadIdx.x; i < n; i += (long long)gridDim.x * NTH) a[i] = v;
+}
+
+template <class A, class B>
+__global__ __launch_bounds__(NTH) void convert_kernel(const A* src, B* dst, long long n) {
+  for (long long i = (long long)blockIdx.x * NTH + threadIdx.x; i < n; i += (long long)gridDim.x * NTH)
+    dst[i] = (B)src[i];
 }
 
 __global__ void pncg_init_kernel(StatePtrs st, int n) {
@@ -140,11 +147,11 @@ __global__ __launch_bounds__(NTH) void pncg_reduce_kernel(VecArgs<R> A) {
         h[c] = R(0);
       }
       for (int k = o0; k < o1; ++k) {
-        const long long pidx = A.sh_src[k];
+        const R* src = A.partial + (long long)A.sh_src[k] * 6;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          ge[c] += A.partial[c * A.n_partials + pidx];
-          h[c] += A.partial[(3 + c) * A.n_partials + pidx];
+          ge[c] += src[c];
+          h[c] += src[3 + c];
         }
       }
 #pragma unroll
@@ -164,7 +171,7 @@ __global__ __launch_bounds__(NTH) void pncg_reduce_kernel(VecArgs<R> A) {
     for (int c = 0; c < 3; ++c) {
       const double f = A.fext != nullptr ? (double)A.fext[3 * v + c] : 0.0;
       const double pc = (double)A.p[3 * v + c];
-      d[D_EXT] -= f * ((double)A.x[3 * v + c] + alpha * pc);
+      d[D_EXT] -= f * (A.x[3 * v + c] + alpha * pc);
       if (msk & (1u << c)) continue;
       const double gt = (double)(ge[c] - (A.fext != nullptr ? A.fext[3 * v + c] : R(0)));
       const double gcv = (double)A.gc[3 * v + c];
@@ -307,7 +314,7 @@ __global__ __launch_bounds__(NTH) void pncg_commit_kernel(VecArgs<R> A) {
   if (A.st.commit[inst] == 0) return;
   const long long v = (long long)A.vblk_off[vb] + threadIdx.x;
   if (v >= A.vblk_off[vb + 1]) return;
-  const R a = (R)A.st.alpha_commit[inst];
+  const double a = A.st.alpha_commit[inst];
   const R beta = (R)A.st.beta[inst];
   const bool dir = A.st.act_dir[inst] != 0;
   const R pm = (R)A.st.pmean[inst];
@@ -315,7 +322,7 @@ __global__ __launch_bounds__(NTH) void pncg_commit_kernel(VecArgs<R> A) {
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const R pc = A.p[3 * v + c];
-    A.x[3 * v + c] += a * pc;
+    A.x[3 * v + c] += a * (double)pc;
     const bool fixed = (msk & (1u << c)) != 0;
     const R gt = fixed ? R(0) : A.gE[3 * v + c] - (A.fext != nullptr ? A.fext[3 * v + c] : R(0));
     A.gc[3 * v + c] = gt;
@@ -406,9 +413,9 @@ __global__ __launch_bounds__(NTH) void cg_vec_kernel(CgArgs<R> A) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) s3[c] = R(0);
         for (int k = o0; k < o1; ++k) {
-          const long long pidx = A.sh_src[k];
+          const R* src = A.partial + (long long)A.sh_src[k] * 3;
 #pragma unroll
-          for (int c = 0; c < 3; ++c) s3[c] += A.partial[c * A.n_partials + pidx];
+          for (int c = 0; c < 3; ++c) s3[c] += src[c];
         }
       } else {
 #pragma unroll
@@ -532,7 +539,8 @@ using namespace apl;
 struct apl_solver {
   apl_mesh* mesh = nullptr;
   int material = 0;
-  void* vecs = nullptr;  // p, gc, gE, hd, r, z, d, q, xs (9 x V x 3)
+  void* vecs = nullptr;  // p, gc, gE, hd, r, z, d, q, hd (9 x V x 3)
+  void* x64 = nullptr;   // FP64 iterate of FP32 meshes (V x 3)
   double* sd = nullptr;
   int32_t* si = nullptr;
   StatePtrs st{};
@@ -546,7 +554,7 @@ struct apl_solver {
 namespace apl {
 
 template <class R>
-static VecArgs<R> vec_args(apl_solver* s, void* x, const void* fext, const uint8_t* mask) {
+static VecArgs<R> vec_args(apl_solver* s, double* x, const void* fext, const uint8_t* mask) {
   apl_mesh* m = s->mesh;
   const size_t nv = 3 * (size_t)m->L.V;
   R* base = (R*)s->vecs;
@@ -559,7 +567,7 @@ static VecArgs<R> vec_args(apl_solver* s, void* x, const void* fext, const uint8
   A.n_partials = m->L.n_partials;
   A.mask = mask;
   A.fext = (const R*)fext;
-  A.x = (R*)x;
+  A.x = x;
   A.p = base;
   A.gc = base + nv;
   A.gE = base + 2 * nv;
@@ -584,20 +592,22 @@ static Params to_params(const apl_pncg_params* p) {
 }
 
 template <class R>
-static int pncg_iteration(apl_solver* s, const void* mu, const void* lam, void* x, const void* fext,
+static int pncg_iteration(apl_solver* s, const void* mu, const void* lam, double* x, const void* fext,
                           const uint8_t* mask, const Params& P, int grid, hipStream_t st) {
+  const int x_f64 = sizeof(R) == 4 ? 1 : 0;
   apl_mesh* m = s->mesh;
   VecArgs<R> A = vec_args<R>(s, x, fext, mask);
   const unsigned nvb = (unsigned)m->L.vblk_inst.size();
   const unsigned ni = (unsigned)m->L.n_inst;
   if (launch_elem(m, s->material, W_E | W_G | W_H, mu, lam, x, A.p, s->st.alpha, s->st.act_eval, A.gE, A.hd, 0,
-                  grid, st))
+                  grid, st, x_f64))
     return -1;
   if (nvb) hipLaunchKernelGGL((pncg_reduce_kernel<R>), dim3(nvb), dim3(NTH), 0, st, A);
   hipLaunchKernelGGL(pncg_scalar_kernel, dim3(ni), dim3(NTH), 0, st, s->st, P, m->d.vblk_dots, m->d.inst_vblk_off,
                      m->d.blk_red, m->d.inst_blk_off);
   if (nvb) hipLaunchKernelGGL((pncg_commit_kernel<R>), dim3(nvb), dim3(NTH), 0, st, A);
-  if (launch_elem(m, s->material, W_HQ, mu, lam, x, A.p, nullptr, s->st.act_dir, nullptr, nullptr, 0, grid, st))
+  if (launch_elem(m, s->material, W_HQ, mu, lam, x, A.p, nullptr, s->st.act_dir, nullptr, nullptr, 0, grid, st,
+                  x_f64))
     return -1;
   hipLaunchKernelGGL(pncg_alpha_kernel, dim3(ni), dim3(NTH), 0, st, s->st, P, m->d.blk_red, m->d.inst_blk_off);
   APL_CHECK_HIP(hipGetLastError());
@@ -625,10 +635,19 @@ static int pncg_solve_t(apl_solver* s, const void* mu, const void* lam, void* x,
   hipLaunchKernelGGL(pncg_init_kernel, dim3((n + 255) / 256), dim3(256), 0, st, s->st, n);
   if (fill<R>((R*)s->vecs, 4 * 3 * (long long)m->L.V, R(0), st)) return -1;
   APL_CHECK_HIP(hipGetLastError());
+  // the iterate lives in FP64; FP32 meshes keep a private FP64 copy of x
+  const long long nx = 3 * (long long)m->L.V;
+  const int x_f64 = sizeof(R) == 4 ? 1 : 0;
+  double* xs = x_f64 ? (double*)s->x64 : (double*)x;
+  if (x_f64 && nx > 0) {
+    hipLaunchKernelGGL((convert_kernel<R, double>), dim3((unsigned)std::min<long long>((nx + NTH - 1) / NTH, 4096)),
+                       dim3(NTH), 0, st, (const R*)x, xs, nx);
+    APL_CHECK_HIP(hipGetLastError());
+  }
   const int grid = prm->grid_override;
   // occupancy-derived grids are queried (and cached) outside any capture
-  (void)elem_grid(m, s->material, W_E | W_G | W_H);
-  (void)elem_grid(m, s->material, W_HQ);
+  (void)elem_grid(m, s->material, W_E | W_G | W_H, x_f64);
+  (void)elem_grid(m, s->material, W_HQ, x_f64);
   // cached graph of K iterations for these pointers and parameters
   bool use_graph = prm->use_graph != 0;
   if (use_graph) {
@@ -654,7 +673,7 @@ static int pncg_solve_t(apl_solver* s, const void* mu, const void* lam, void* x,
       if (s->cap == nullptr) APL_CHECK_HIP(hipStreamCreateWithFlags(&s->cap, hipStreamNonBlocking));
       APL_CHECK_HIP(hipStreamBeginCapture(s->cap, hipStreamCaptureModeThreadLocal));
       int rc = 0;
-      for (int it = 0; it < K && rc == 0; ++it) rc = pncg_iteration<R>(s, mu, lam, x, fext, mask, P, grid, s->cap);
+      for (int it = 0; it < K && rc == 0; ++it) rc = pncg_iteration<R>(s, mu, lam, xs, fext, mask, P, grid, s->cap);
       hipGraph_t g = nullptr;
       hipError_t e = hipStreamEndCapture(s->cap, &g);
       if (rc != 0 || e != hipSuccess) {
@@ -674,7 +693,7 @@ static int pncg_solve_t(apl_solver* s, const void* mu, const void* lam, void* x,
       APL_CHECK_HIP(hipGraphLaunch(s->gexec, st));
     } else {
       for (int it = 0; it < K; ++it)
-        if (pncg_iteration<R>(s, mu, lam, x, fext, mask, P, grid, st)) return -1;
+        if (pncg_iteration<R>(s, mu, lam, xs, fext, mask, P, grid, st)) return -1;
     }
     done += K;
     APL_CHECK_HIP(hipMemcpyAsync(s->host_status, s->st.status, n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
@@ -682,6 +701,11 @@ static int pncg_solve_t(apl_solver* s, const void* mu, const void* lam, void* x,
     bool running = false;
     for (int i = 0; i < n; ++i) running = running || s->host_status[i] == APL_RUNNING;
     if (!running || done > guard) break;
+  }
+  if (x_f64 && nx > 0) {
+    hipLaunchKernelGGL((convert_kernel<double, R>), dim3((unsigned)std::min<long long>((nx + NTH - 1) / NTH, 4096)),
+                       dim3(NTH), 0, st, (const double*)xs, (R*)x, nx);
+    APL_CHECK_HIP(hipGetLastError());
   }
   if (stats != nullptr) {
     std::vector<double> hd((size_t)N_SD * n);
@@ -820,12 +844,15 @@ int apl_solver_create(apl_mesh* m, int material, apl_solver** out) {
   }
   *out = nullptr;
   APL_CHECK_HIP(hipSetDevice(m->device));
+  (void)hipGetLastError();  // drop errors left by unrelated earlier HIP calls
   apl_solver* s = new apl_solver();
   s->mesh = m;
   s->material = material;
   const int n = (int)m->L.n_inst;
   const size_t vb = 9 * 3 * (size_t)std::max<int64_t>(m->L.V, 1) * real_size(m);
-  if (hipMalloc(&s->vecs, vb) != hipSuccess || hipMalloc((void**)&s->sd, N_SD * (size_t)n * sizeof(double)) != hipSuccess ||
+  const size_t xb = m->dtype == DT_F32 ? 3 * (size_t)std::max<int64_t>(m->L.V, 1) * sizeof(double) : 0;
+  if (hipMalloc(&s->vecs, vb) != hipSuccess || (xb && hipMalloc(&s->x64, xb) != hipSuccess) ||
+      hipMalloc((void**)&s->sd, N_SD * (size_t)n * sizeof(double)) != hipSuccess ||
       hipMalloc((void**)&s->si, N_SI * (size_t)n * sizeof(int32_t)) != hipSuccess ||
       hipHostMalloc((void**)&s->host_status, (size_t)n * sizeof(int32_t), 0) != hipSuccess) {
     set_error("out of device memory creating the solver");
@@ -866,6 +893,7 @@ int apl_solver_destroy(apl_solver* s) {
   if (s->graph) (void)hipGraphDestroy(s->graph);
   if (s->cap) (void)hipStreamDestroy(s->cap);
   if (s->vecs) (void)hipFree(s->vecs);
+  if (s->x64) (void)hipFree(s->x64);
   if (s->sd) (void)hipFree(s->sd);
   if (s->si) (void)hipFree(s->si);
   if (s->host_status) (void)hipHostFree(s->host_status);
@@ -885,6 +913,7 @@ int apl_pncg_solve(apl_solver* s, const void* mu, const void* lam, void* x, cons
     return -1;
   }
   APL_CHECK_HIP(hipSetDevice(s->mesh->device));
+  (void)hipGetLastError();  // drop errors left by unrelated earlier HIP calls
   if (s->mesh->dtype == DT_F32)
     return pncg_solve_t<float>(s, mu, lam, x, f_ext, fixed_mask, params, stats, (hipStream_t)stream);
   return pncg_solve_t<double>(s, mu, lam, x, f_ext, fixed_mask, params, stats, (hipStream_t)stream);
@@ -902,6 +931,7 @@ int apl_cg_solve(apl_solver* s, const void* mu, const void* lam, const void* x, 
     return -1;
   }
   APL_CHECK_HIP(hipSetDevice(s->mesh->device));
+  (void)hipGetLastError();  // drop errors left by unrelated earlier HIP calls
   if (s->mesh->dtype == DT_F32)
     return cg_solve_t<float>(s, mu, lam, x, rhs, y, fixed_mask, rtol, max_iters, stats, (hipStream_t)stream);
   return cg_solve_t<double>(s, mu, lam, x, rhs, y, fixed_mask, rtol, max_iters, stats, (hipStream_t)stream);

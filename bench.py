@@ -157,6 +157,8 @@ def main():
     pot = prob["pot"]
     T, V = pot.mesh.n_tets, pot.mesh.n_verts
     info = pot.mesh.info()
+    log(f"[bench] partition: {info['n_blocks']} blocks, nloc_max {info['nloc_max']}, {info['n_partials']} partials, "
+        f"{info['n_shared_verts']} shared verts of {V}, grid {info['elem_grid_snh']}")
     what_a = _lib.APL_ENERGY | _lib.APL_GRAD | _lib.APL_HESS_DIAG
     u = prob["x0"]
 
@@ -216,6 +218,8 @@ def main():
             "n_tets": T,
             "n_verts": V,
             "n_blocks": info["n_blocks"],
+            "nloc_max": info["nloc_max"],
+            "n_partials": info["n_partials"],
             "parallelism": "single GPU",
         },
         "roofline": roofline,

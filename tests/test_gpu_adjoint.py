@@ -1,0 +1,142 @@
+"""Adjoint gradient of the equilibrium w.r.t. per-tet materials, on the GPU.
+
+Checked three ways: against the oracle's adjoint (sparse direct solve of the
+exact FP64 Hessian at the same equilibrium), against FP64 central finite
+differences of the loss through GPU equilibrium solves, and FP32 against FP64
+(the error is stated in the assertion).  lambda tied to mu (lambda = 10 mu)
+goes through autograd.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import oracle.fem as of
+import oracle.pncg as op
+from apple_amd import adjoint, batch, fem
+from apple_amd.mesh import Region, cube
+from apple_amd.solver import PncgConfig, mask_bits
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(n, seed, dtype):
+    m = cube(n)
+    rng = np.random.default_rng(seed)
+    mu = 1e4 * np.exp(0.5 * rng.standard_normal(m.n_cells))
+    m.cell_data["mu"] = mu
+    m.cell_data["lambda"] = 10 * mu
+    region = Region.from_mesh(m)
+    pot = fem.StableNeoHookean.from_region(region, dtype=dtype, device="cuda")
+    f = np.zeros((m.n_points, 3))
+    np.add.at(f[:, 2], m.cells.reshape(-1), np.repeat(-9.8e3 * region.dV / 4, 4))
+    fixed = np.zeros((m.n_points, 3), bool)
+    fixed[m.points[:, 2] == 0] = True
+    t = lambda a: torch.as_tensor(a, dtype=dtype, device="cuda").contiguous()
+    return m, region, pot, mu, t(f), fixed, mask_bits(fixed, "cuda")
+
+
+def _target(m, seed):
+    rng = np.random.default_rng(seed)
+    return 0.01 * rng.standard_normal((m.n_points, 3))
+
+
+def test_adjoint_matches_oracle_adjoint():
+    m, region, pot, mu, f, fixed, mask = _problem(3, 1, torch.float64)
+    x_t = torch.as_tensor(_target(m, 2), dtype=torch.float64, device="cuda")
+    mu_t = torch.as_tensor(mu, dtype=torch.float64, device="cuda").requires_grad_(True)
+    lam_t = (10 * mu_t)
+    x0 = torch.zeros(m.n_points, 3, dtype=torch.float64, device="cuda")
+    x = adjoint.equilibrium(pot, mu_t, lam_t, f, mask, x0, PncgConfig(rtol=1e-11, max_steps=20000),
+                            adjoint.AdjointConfig(rtol=1e-12))
+    loss = ((x - x_t) ** 2).sum()
+    loss.backward()
+    g = mu_t.grad.cpu().numpy()
+    # oracle: same equilibrium (Newton-polished), sparse adjoint
+    opot = of.FemPotential(of.SNH, m.cells, region.dhdX, region.dV, mu, 10 * mu)
+    ext = of.ExternalForce(np.arange(m.n_points), f.cpu().numpy())
+    dof = of.DofMap(fixed, np.zeros((m.n_points, 3)))
+    prob = op.Problem(of.Model([opot, ext]), dof)
+    xe = op.newton(prob, dof.to_free(x.detach().cpu().numpy()), tol=1e-11)
+    dLdx = dof.to_free(2 * (dof.to_full(xe) - x_t.cpu().numpy()))
+    dmu, dlam = op.adjoint_grad(prob, opot, xe, dLdx)
+    ref = dmu + 10 * dlam
+    np.testing.assert_allclose(g, ref, rtol=1e-6, atol=1e-9 * np.abs(ref).max())
+
+
+def test_adjoint_matches_central_differences():
+    m, region, pot, mu, f, fixed, mask = _problem(4, 3, torch.float64)
+    x_t = torch.as_tensor(_target(m, 4), dtype=torch.float64, device="cuda")
+    cfg = PncgConfig(rtol=1e-12, max_steps=50000)
+    x0 = torch.zeros(m.n_points, 3, dtype=torch.float64, device="cuda")
+    mu_t = torch.as_tensor(mu, dtype=torch.float64, device="cuda").requires_grad_(True)
+    x = adjoint.equilibrium(pot, mu_t, 10 * mu_t, f, mask, x0, cfg, adjoint.AdjointConfig(rtol=1e-12))
+    ((x - x_t) ** 2).sum().backward()
+    g = mu_t.grad.cpu().numpy()
+    x_eq = x.detach()
+
+    def loss_at(mu_np):
+        mt = torch.as_tensor(mu_np, dtype=torch.float64, device="cuda")
+        with torch.no_grad():
+            xx = adjoint.equilibrium(pot, mt, 10 * mt, f, mask, x_eq, cfg)
+        return float(((xx - x_t) ** 2).sum())
+
+    rng = np.random.default_rng(5)
+    for t in rng.choice(m.n_cells, 6, replace=False):
+        h = 1e-4 * mu[t]
+        mp = mu.copy()
+        mp[t] += h
+        mm = mu.copy()
+        mm[t] -= h
+        fd = (loss_at(mp) - loss_at(mm)) / (2 * h)
+        assert g[t] == pytest.approx(fd, rel=1e-4, abs=1e-6 * np.abs(g).max())
+
+
+def test_fp32_adjoint_close_to_fp64():
+    grads = {}
+    for dtype in (torch.float64, torch.float32):
+        m, region, pot, mu, f, fixed, mask = _problem(6, 7, dtype)
+        x_t = torch.as_tensor(_target(m, 8), dtype=dtype, device="cuda")
+        mu_t = torch.as_tensor(mu, dtype=dtype, device="cuda").requires_grad_(True)
+        x0 = torch.zeros(m.n_points, 3, dtype=dtype, device="cuda")
+        rtol = 1e-10 if dtype == torch.float64 else 1e-6
+        x = adjoint.equilibrium(pot, mu_t, 10 * mu_t, f, mask, x0, PncgConfig(rtol=rtol, max_steps=20000),
+                                adjoint.AdjointConfig(rtol=1e-10 if dtype == torch.float64 else 1e-6))
+        ((x - x_t) ** 2).sum().backward()
+        grads[dtype] = mu_t.grad.double().cpu().numpy()
+    err = np.linalg.norm(grads[torch.float32] - grads[torch.float64]) / np.linalg.norm(grads[torch.float64])
+    print(f"FP32 adjoint relative error vs FP64: {err:.2e}")
+    assert err <= 1e-3
+
+
+def test_batched_adjoint_equals_single():
+    insts, mus, targets = [], [], []
+    for k, n in enumerate((3, 4)):
+        m = cube(n)
+        rng = np.random.default_rng(20 + k)
+        mu = 1e4 * np.exp(0.5 * rng.standard_normal(m.n_cells))
+        region = Region.from_mesh(m)
+        f = np.zeros((m.n_points, 3))
+        np.add.at(f[:, 2], m.cells.reshape(-1), np.repeat(-9.8e3 * region.dV / 4, 4))
+        fixed = np.zeros((m.n_points, 3), bool)
+        fixed[m.points[:, 2] == 0] = True
+        insts.append(batch.Instance(region, mu, 10 * mu, f, fixed, np.zeros((m.n_points, 3))))
+        mus.append(mu)
+        targets.append(_target(m, 30 + k))
+    cfg = PncgConfig(rtol=1e-10, max_steps=20000)
+    acfg = adjoint.AdjointConfig(rtol=1e-10)
+
+    def run(b, mu_np, tgt):
+        mu_t = torch.as_tensor(mu_np, dtype=torch.float64, device="cuda").requires_grad_(True)
+        x = adjoint.equilibrium(b.potential, mu_t, 10 * mu_t, b.f_ext, b.mask, b.x0, cfg, acfg)
+        ((x - torch.as_tensor(tgt, dtype=torch.float64, device="cuda")) ** 2).sum().backward()
+        return mu_t.grad.cpu().numpy()
+
+    bb = batch.Batch(insts, 0, dtype=torch.float64, device="cuda")
+    g_b = run(bb, np.concatenate(mus), np.concatenate(targets))
+    off = 0
+    for inst, mu, tgt in zip(insts, mus, targets):
+        single = batch.Batch([inst], 0, dtype=torch.float64, device="cuda")
+        g_s = run(single, mu, tgt)
+        np.testing.assert_allclose(g_b[off : off + mu.size], g_s, rtol=1e-6, atol=1e-12)
+        off += mu.size
