@@ -36,10 +36,10 @@ static int alloc_bytes(apl_mesh* m, void** dst, size_t n) {
 
 static void free_mesh(apl_mesh* m) {
   DevLayout& d = m->d;
-  void* ptrs[] = {d.dminv,       d.dV,          d.slot,        d.blk_tet_off, d.blk_loc_off,   d.blk_nown,
-                  d.blk_sh_base, d.blk_inst,    d.inst_blk_off, d.loc_gid,    d.loc_inc_off,   d.loc_inc,
-                  d.sh_vtx,      d.sh_off,      d.sh_src,       d.vtx_sh_off, d.vblk_off,      d.vblk_inst,
-                  d.inst_vblk_off, d.inst_vtx_off, d.partial,   d.blk_red,    d.vblk_dots};
+  void* ptrs[] = {d.dminv,        d.dV,           d.slot,        d.blk_tet_off,  d.blk_loc_off,
+                  d.blk_inst,     d.inst_blk_off, d.loc_gid,     d.loc_inc_off,  d.loc_inc,
+                  d.vtx_slot_off, d.vtx_slot_src, d.vblk_off,    d.vblk_inst,    d.inst_vblk_off,
+                  d.inst_vtx_off, d.partial,      d.blk_red,     d.vblk_dots};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete m;
@@ -145,27 +145,18 @@ int apl_mesh_create(int dtype, int64_t n_tets, int64_t n_verts, const int32_t* c
   rc |= upload(m, &d.slot, L.slot.data(), L.slot.size());
   rc |= upload(m, &d.blk_tet_off, L.blk_tet_off.data(), L.blk_tet_off.size());
   rc |= upload(m, &d.blk_loc_off, L.blk_loc_off.data(), L.blk_loc_off.size());
-  rc |= upload(m, &d.blk_nown, L.blk_nown.data(), L.blk_nown.size());
-  rc |= upload(m, &d.blk_sh_base, L.blk_sh_base.data(), L.blk_sh_base.size());
   rc |= upload(m, &d.blk_inst, L.blk_inst.data(), L.blk_inst.size());
   rc |= upload(m, &d.inst_blk_off, L.inst_blk_off.data(), L.inst_blk_off.size());
   rc |= upload(m, &d.loc_gid, L.loc_gid.data(), L.loc_gid.size());
   rc |= upload(m, &d.loc_inc_off, L.loc_inc_off.data(), L.loc_inc_off.size());
   rc |= upload(m, &d.loc_inc, L.loc_inc.data(), L.loc_inc.size());
-  rc |= upload(m, &d.sh_vtx, L.sh_vtx.data(), L.sh_vtx.size());
-  rc |= upload(m, &d.sh_off, L.sh_off.data(), L.sh_off.size());
-  rc |= upload(m, &d.sh_src, L.sh_src.data(), L.sh_src.size());
-  {
-    std::vector<int32_t> vso((size_t)n_verts + 1, 0);
-    for (int64_t i = 0; i < L.n_shared_vtx; ++i) vso[L.sh_vtx[i] + 1] = L.sh_off[i + 1] - L.sh_off[i];
-    for (int64_t v = 0; v < n_verts; ++v) vso[v + 1] += vso[v];
-    rc |= upload(m, &d.vtx_sh_off, vso.data(), vso.size());
-  }
+  rc |= upload(m, &d.vtx_slot_off, L.vtx_slot_off.data(), L.vtx_slot_off.size());
+  rc |= upload(m, &d.vtx_slot_src, L.vtx_slot_src.data(), L.vtx_slot_src.size());
   rc |= upload(m, &d.vblk_off, L.vblk_off.data(), L.vblk_off.size());
   rc |= upload(m, &d.vblk_inst, L.vblk_inst.data(), L.vblk_inst.size());
   rc |= upload(m, &d.inst_vblk_off, L.inst_vblk_off.data(), L.inst_vblk_off.size());
   rc |= upload(m, &d.inst_vtx_off, inst_vtx_off, (size_t)n_inst + 1);
-  rc |= alloc_bytes(m, &d.partial, (size_t)MAX_CH * std::max<int64_t>(L.n_partials, 1) * rs);
+  rc |= alloc_bytes(m, &d.partial, (size_t)MAX_CH * std::max<int64_t>(L.n_slots, 1) * rs);
   rc |= alloc_bytes(m, (void**)&d.blk_red, 2 * (size_t)std::max<int64_t>(L.nblocks, 1) * sizeof(double));
   rc |= alloc_bytes(m, (void**)&d.vblk_dots,
                     (size_t)N_DOTS * std::max<size_t>(L.vblk_inst.size(), 1) * sizeof(double));
@@ -178,6 +169,7 @@ int apl_mesh_create(int dtype, int64_t n_tets, int64_t n_verts, const int32_t* c
   L.loc_inc = std::vector<uint16_t>();
   L.loc_inc_off = std::vector<uint16_t>();
   L.loc_gid = std::vector<int32_t>();
+  L.vtx_slot_src = std::vector<int32_t>();
   *out = m;
   return 0;
 }
@@ -201,7 +193,7 @@ int apl_mesh_info(const apl_mesh* mesh, apl_mesh_info_t* info) {
   info->n_verts = L.V;
   info->n_inst = L.n_inst;
   info->n_blocks = L.nblocks;
-  info->n_partials = L.n_partials;
+  info->n_partials = L.n_slots;
   info->n_shared_verts = L.n_shared_vtx;
   info->n_vertex_blocks = (int64_t)L.vblk_inst.size();
   info->nloc_max = L.nloc_max;
@@ -256,13 +248,13 @@ int apl_elem_eval(apl_mesh* m, int material, const void* mu, const void* lam, co
         if (egh & b) passes.push_back(b);
     for (unsigned pw : passes) {
       if (launch_elem(m, material, pw, mu, lam, u, nullptr, nullptr, nullptr, grad, hess_diag, acc, 0, s)) return -1;
-      if ((pw & (W_G | W_H)) && launch_shared_reduce(m, pw, grad, hess_diag, acc, nullptr, s)) return -1;
+      if ((pw & (W_G | W_H)) && launch_vertex_reduce(m, pw, grad, hess_diag, acc, nullptr, s)) return -1;
       if ((pw & W_E) && launch_inst_reduce(m, energy, nullptr, acc, nullptr, s)) return -1;
     }
   }
   if (w & W_HP) {
     if (launch_elem(m, material, W_HP, mu, lam, u, p, nullptr, nullptr, hess_prod, nullptr, acc, 0, s)) return -1;
-    if (launch_shared_reduce(m, W_HP, hess_prod, nullptr, acc, nullptr, s)) return -1;
+    if (launch_vertex_reduce(m, W_HP, hess_prod, nullptr, acc, nullptr, s)) return -1;
   }
   if (w & W_HQ) {
     if (launch_elem(m, material, W_HQ, mu, lam, u, p, nullptr, nullptr, nullptr, nullptr, acc, 0, s)) return -1;
@@ -296,7 +288,7 @@ int apl_bench_elem_pass(apl_mesh* m, int material, unsigned what, const void* mu
     APL_CHECK_HIP(hipEventRecord(ev[3 * it], s));
     rc = launch_elem(m, material, w, mu, lam, u, p, nullptr, nullptr, out_a, out_b, 0, 0, s);
     APL_CHECK_HIP(hipEventRecord(ev[3 * it + 1], s));
-    if (rc == 0 && (w & (W_G | W_H | W_HP))) rc = launch_shared_reduce(m, w, out_a, out_b, 0, nullptr, s);
+    if (rc == 0 && (w & (W_G | W_H | W_HP))) rc = launch_vertex_reduce(m, w, out_a, out_b, 0, nullptr, s);
     if (rc == 0 && (w & (W_E | W_HQ)))
       rc = launch_inst_reduce(m, (w & W_E) ? energy : nullptr, (w & W_HQ) ? energy : nullptr, 0, nullptr, s);
     APL_CHECK_HIP(hipEventRecord(ev[3 * it + 2], s));

@@ -9,16 +9,18 @@
 //
 //  * each workgroup (256 threads) owns a run of consecutive tets (a block) and
 //    stages the displacement of every vertex the block touches into LDS once
-//    (slots listed in loc_gid), so the 4 corner gathers per tet read LDS;
+//    (padded to 4 lanes, one wide LDS read per corner);
 //  * per-tet element data are SoA in HBM (9 x T inverse rest shapes, dV, mu,
-//    lambda, 4 x uint16 local corner slots): every load is coalesced;
+//    lambda, 4 x uint16 local corner slots), addressed by 32-bit offsets from
+//    per-block uniform base pointers: every load is coalesced and needs no
+//    per-lane 64-bit address arithmetic;
 //  * per-corner contributions go to LDS in [channel][corner][thread] order
-//    (bank-conflict free writes) and each local vertex sums its incidences
-//    over an ascending CSR list -- a fixed order, so results are bitwise
-//    reproducible; owned vertices are written once, shared vertices leave a
-//    partial that the vertex pass sums in ascending block order;
-//  * energies and quadratic forms are summed per thread in FP64, then by a
-//    fixed wave-shuffle tree and a per-instance pass over the block partials;
+//    (bank-conflict free writes); each local vertex sums its incidences over
+//    an ascending CSR list (a fixed order: bitwise reproducible) and writes
+//    one partial per slot into a slot-major buffer -- a contiguous run per
+//    block; the vertex pass adds the partials of every vertex in block order;
+//  * energies and quadratic forms are summed per thread, then in FP64 by a
+//    fixed wave-shuffle tree and a per-instance pass over the block sums;
 //  * the grid is persistent and XCD-aware: workgroup i runs on XCD i % 8, and
 //    XCD x walks the contiguous x-th eighth of the blocks, so neighbouring
 //    blocks (which share vertices) hit the same L2.
@@ -34,8 +36,6 @@ struct ElemArgs {
   const ushort4* slot;
   const int32_t* blk_tet_off;
   const int32_t* blk_loc_off;
-  const int32_t* blk_nown;
-  const int32_t* blk_sh_base;
   const int32_t* blk_inst;
   const int32_t* loc_gid;
   const uint16_t* loc_inc_off;
@@ -46,12 +46,11 @@ struct ElemArgs {
   const R* p;
   const double* alpha;
   const int32_t* active;
-  R* out_a;
+  R* out_a;  // per-tet outputs of the mixed pass
   R* out_b;
-  R* partial;
+  R* partial;  // slot-major partial sums (scatter passes)
   double* blk_red;
-  long long T;
-  long long n_partials;
+  int T;
   int nblocks;
   int nloc_max;
   int chunks;
@@ -98,6 +97,18 @@ __device__ inline void block_sum2(double& a, double& b, double* red) {
   __syncthreads();
 }
 
+// Element i of a uniform base pointer through a 32-bit byte offset: maps to
+// the scalar-base + vector-offset load/store form (no per-lane 64-bit
+// address arithmetic).  Mesh sizes are limited so that offsets fit 32 bits.
+template <class T>
+__device__ __forceinline__ const T& at(const T* base, unsigned i) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * (unsigned)sizeof(T));
+}
+template <class T>
+__device__ __forceinline__ T& at(T* base, unsigned i) {
+  return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + i * (unsigned)sizeof(T));
+}
+
 // Per-tet record, loaded into registers ahead of the block's LDS staging.
 template <class R>
 struct TetRec {
@@ -106,20 +117,26 @@ struct TetRec {
   ushort4 sl;
 };
 
+// Loads from a per-block uniform base with a 32-bit lane offset.
 template <int MAT, class R, class X>
-__device__ inline void load_rec(const ElemArgs<R, X>& A, long long t, TetRec<R>& r) {
+__device__ inline void load_rec(const ElemArgs<R, X>& A, int t0, unsigned tl, TetRec<R>& r) {
 #pragma unroll
-  for (int kk = 0; kk < 9; ++kk) r.B[kk] = A.dminv[kk * A.T + t];
-  r.dV = A.dV[t];
-  r.mu = A.mu[t];
-  r.lam = (MAT == MAT_SNH) ? A.lam[t] : R(0);
-  r.sl = A.slot[t];
+  for (int kk = 0; kk < 9; ++kk) r.B[kk] = at(A.dminv + ((size_t)kk * A.T + t0), tl);
+  r.dV = at(A.dV + t0, tl);
+  r.mu = at(A.mu + t0, tl);
+  r.lam = (MAT == MAT_SNH) ? at(A.lam + t0, tl) : R(0);
+  r.sl = at(A.slot + t0, tl);
 }
 
-// SPT_: local slots reduced per thread (nloc_max <= SPT_ * NTH)
-template <int MAT, unsigned WHAT, class R, class X, int SPT_>
+// 4-lane padded vector (one 16/32-byte LDS access)
+template <class T>
+struct alignas(4 * sizeof(T)) Vec4 {
+  T v[4];
+};
+
+// SPT: local slots reduced per thread (nloc_max <= SPT * NTH)
+template <int MAT, unsigned WHAT, class R, class X, int SPT>
 __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R, X> A) {
-  constexpr int SPT = SPT_;
   constexpr bool HAS_G = (WHAT & W_G) != 0;
   constexpr bool HAS_H = (WHAT & W_H) != 0;
   constexpr bool HAS_HP = (WHAT & W_HP) != 0;
@@ -132,10 +149,10 @@ __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R, X> A) {
 
   extern __shared__ double smem_d[];
   double* red = smem_d;
-  X* u_lds = reinterpret_cast<X*>(smem_d + 8);
-  R* v_lds = reinterpret_cast<R*>(u_lds + 3 * A.nloc_max);
-  R* contrib = v_lds + (NEEDV ? 3 * A.nloc_max : 0);
-  // the block's incidence CSR (sorted by local slot) staged next to contrib
+  X* u_lds = reinterpret_cast<X*>(smem_d + 8);  // 4 per slot (padded)
+  R* v_lds = reinterpret_cast<R*>(u_lds + 4 * A.nloc_max);
+  R* contrib = v_lds + (NEEDV ? 4 * A.nloc_max : 0);
+  // the block's incidence CSR (grouped by local slot) staged next to contrib
   uint16_t* inc_lds = reinterpret_cast<uint16_t*>(contrib + (SCATTER ? NCH * 4 * NTH : 0));
   uint16_t* off_lds = inc_lds + (SCATTER ? 4 * NTH * A.chunks : 0);
 
@@ -152,37 +169,41 @@ __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R, X> A) {
     const int nt = A.blk_tet_off[b + 1] - t0;
     const int sb = A.blk_loc_off[b];
     const int nloc = A.blk_loc_off[b + 1] - sb;
-    const int nown = A.blk_nown[b];
     const X alpha = A.alpha != nullptr ? (X)A.alpha[inst] : X(0);
 
     // this thread's first tet record, loaded before the staging barrier so the
     // two global-load latencies overlap
     TetRec<R> rec;
-    if (tid < nt) load_rec<MAT>(A, (long long)t0 + tid, rec);
+    if (tid < nt) load_rec<MAT>(A, t0, (unsigned)tid, rec);
 
     // stage the block's vertex window (u = x + alpha p for a trial point)
+    const int32_t* gids = A.loc_gid + sb;
     for (int s = tid; s < nloc; s += NTH) {
-      const long long gid = A.loc_gid[sb + s];
-      X ux = A.x[3 * gid], uy = A.x[3 * gid + 1], uz = A.x[3 * gid + 2];
-      if (A.alpha != nullptr) {
-        ux += alpha * (X)A.p[3 * gid];
-        uy += alpha * (X)A.p[3 * gid + 1];
-        uz += alpha * (X)A.p[3 * gid + 2];
+      const unsigned g3 = 3u * (unsigned)at(gids, (unsigned)s);
+      X ux = at(A.x, g3), uy = at(A.x, g3 + 1), uz = at(A.x, g3 + 2);
+      R px = R(0), py = R(0), pz = R(0);
+      if (NEEDV || A.alpha != nullptr) {
+        px = at(A.p, g3);
+        py = at(A.p, g3 + 1);
+        pz = at(A.p, g3 + 2);
       }
-      u_lds[3 * s] = ux;
-      u_lds[3 * s + 1] = uy;
-      u_lds[3 * s + 2] = uz;
+      if (A.alpha != nullptr) {
+        ux += alpha * (X)px;
+        uy += alpha * (X)py;
+        uz += alpha * (X)pz;
+      }
+      Vec4<X> u4 = {{ux, uy, uz, X(0)}};
+      *reinterpret_cast<Vec4<X>*>(u_lds + 4 * s) = u4;
       if constexpr (NEEDV) {
-        v_lds[3 * s] = A.p[3 * gid];
-        v_lds[3 * s + 1] = A.p[3 * gid + 1];
-        v_lds[3 * s + 2] = A.p[3 * gid + 2];
+        Vec4<R> v4 = {{px, py, pz, R(0)}};
+        *reinterpret_cast<Vec4<R>*>(v_lds + 4 * s) = v4;
       }
     }
     if constexpr (SCATTER) {
-      const uint16_t* inc_g = A.loc_inc + 4LL * t0;
-      for (int k = tid; k < 4 * nt; k += NTH) inc_lds[k] = inc_g[k];
-      const uint16_t* off_g = A.loc_inc_off + sb + b;
-      for (int k = tid; k <= nloc; k += NTH) off_lds[k] = off_g[k];
+      const uint16_t* inc_g = A.loc_inc + 4 * (size_t)t0;
+      for (int k = tid; k < 4 * nt; k += NTH) inc_lds[k] = at(inc_g, (unsigned)k);
+      const uint16_t* off_g = A.loc_inc_off + ((size_t)sb + b);
+      for (int k = tid; k <= nloc; k += NTH) off_lds[k] = at(off_g, (unsigned)k);
     }
     __syncthreads();
 
@@ -200,41 +221,44 @@ __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R, X> A) {
 #pragma unroll
       for (int ch = 0; ch < NCHA; ++ch) acc[k][ch] = R(0);
     }
-    double e_acc = 0.0, q_acc = 0.0;
+    R e_acc = R(0), q_acc = R(0);
     const int nchunks = (nt + NTH - 1) / NTH;
     for (int c = 0; c < nchunks; ++c) {
       const int tl = c * NTH + tid;
       if (tl < nt) {
-        const long long t = t0 + tl;
-        if (c > 0) load_rec<MAT>(A, t, rec);
+        if (c > 0) load_rec<MAT>(A, t0, (unsigned)tl, rec);
         const TetGeom<R> g = make_geom(rec.B, rec.dV);
         const int sidx[4] = {rec.sl.x, rec.sl.y, rec.sl.z, rec.sl.w};
         // edge differences u_a - u_0 in the storage precision X, then R: F - I
         // depends only on them, and forming them before rounding keeps F
         // accurate relative to the strain rather than to |u|.
+        Vec4<X> uc[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) uc[a] = *reinterpret_cast<const Vec4<X>*>(u_lds + 4 * sidx[a]);
         V3<R> u[4], v[4];
-        const X u0x = u_lds[3 * sidx[0]], u0y = u_lds[3 * sidx[0] + 1], u0z = u_lds[3 * sidx[0] + 2];
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
-          u[a] = v3((R)(u_lds[3 * sidx[a]] - u0x), (R)(u_lds[3 * sidx[a] + 1] - u0y), (R)(u_lds[3 * sidx[a] + 2] - u0z));
-          if constexpr (NEEDV)
-            v[a] = v3(v_lds[3 * sidx[a]], v_lds[3 * sidx[a] + 1], v_lds[3 * sidx[a] + 2]);
-          else
-            v[a] = v3(R(0), R(0), R(0));
-        }
-        const R mu = rec.mu;
-        const R lam = rec.lam;
-        TetOut<R> o;
-        tet_eval<MAT, WHAT, R>(g, mu, lam, u, v, o);
-        if constexpr ((WHAT & W_E) != 0) e_acc += (double)o.e;
-        if constexpr ((WHAT & W_HQ) != 0) q_acc += (double)o.hq;
-        if constexpr ((WHAT & W_MIX) != 0) {
-          if (A.accumulate) {
-            A.out_a[t] += o.dmu;
-            if (MAT == MAT_SNH) A.out_b[t] += o.dlam;
+          u[a] = v3((R)(uc[a].v[0] - uc[0].v[0]), (R)(uc[a].v[1] - uc[0].v[1]), (R)(uc[a].v[2] - uc[0].v[2]));
+          if constexpr (NEEDV) {
+            const Vec4<R> vv = *reinterpret_cast<const Vec4<R>*>(v_lds + 4 * sidx[a]);
+            v[a] = v3(vv.v[0], vv.v[1], vv.v[2]);
           } else {
-            A.out_a[t] = o.dmu;
-            if (MAT == MAT_SNH) A.out_b[t] = o.dlam;
+            v[a] = v3(R(0), R(0), R(0));
+          }
+        }
+        TetOut<R> o;
+        tet_eval<MAT, WHAT, R>(g, rec.mu, rec.lam, u, v, o);
+        if constexpr ((WHAT & W_E) != 0) e_acc += o.e;
+        if constexpr ((WHAT & W_HQ) != 0) q_acc += o.hq;
+        if constexpr ((WHAT & W_MIX) != 0) {
+          R* oa = A.out_a + t0;
+          R* ob = A.out_b + t0;
+          if (A.accumulate) {
+            at(oa, (unsigned)tl) += o.dmu;
+            if (MAT == MAT_SNH) at(ob, (unsigned)tl) += o.dlam;
+          } else {
+            at(oa, (unsigned)tl) = o.dmu;
+            if (MAT == MAT_SNH) at(ob, (unsigned)tl) = o.dlam;
           }
         }
         if constexpr (SCATTER) {
@@ -270,14 +294,14 @@ __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R, X> A) {
       if constexpr (SCATTER) {
         __syncthreads();
         if (nchunks == 1) {
-          // whole incidence list of every slot lies in this chunk
+          // the whole incidence list of every slot lies in this chunk
 #pragma unroll
           for (int k = 0; k < SPT; ++k) {
             for (int q = cur[k]; q < endi[k]; ++q) {
               const int id = inc_lds[q];
-              const int tlc = id >> 2, a = id & 3;
+              const R* cp = contrib + (id & 3) * NTH + (id >> 2);
 #pragma unroll
-              for (int ch = 0; ch < NCH; ++ch) acc[k][ch] += contrib[(ch * 4 + a) * NTH + tlc];
+              for (int ch = 0; ch < NCH; ++ch) acc[k][ch] += cp[ch * 4 * NTH];
             }
           }
         } else {
@@ -289,9 +313,9 @@ __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R, X> A) {
               const int id = inc_lds[cur[k]];
               if (id >= lim) break;
               const int j = id - base;
-              const int tlc = j >> 2, a = j & 3;
+              const R* cp = contrib + (j & 3) * NTH + (j >> 2);
 #pragma unroll
-              for (int ch = 0; ch < NCH; ++ch) acc[k][ch] += contrib[(ch * 4 + a) * NTH + tlc];
+              for (int ch = 0; ch < NCH; ++ch) acc[k][ch] += cp[ch * 4 * NTH];
               ++cur[k];
             }
           }
@@ -301,83 +325,62 @@ __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R, X> A) {
     }
 
     if constexpr (SCATTER) {
-      R* outs[2] = {A.out_a, A.out_b};
+      // one partial per local slot, consecutive slots consecutive in memory
+      R* dst = A.partial + (size_t)sb * NCH;
 #pragma unroll
       for (int k = 0; k < SPT; ++k) {
         const int s = tid + k * NTH;
-        if (s >= nloc) continue;
-        if (s < nown) {
-          const long long gid = A.loc_gid[sb + s];
+        if (s < nloc) {
 #pragma unroll
-          for (int grp = 0; grp < NCH / 3; ++grp) {
-            // group -> output: (G,H) -> (a,b); H alone -> b; HP -> a
-            R* out = (HAS_G || HAS_HP) ? outs[grp] : outs[1];
-            if (A.accumulate) {
-              out[3 * gid] += acc[k][3 * grp];
-              out[3 * gid + 1] += acc[k][3 * grp + 1];
-              out[3 * gid + 2] += acc[k][3 * grp + 2];
-            } else {
-              out[3 * gid] = acc[k][3 * grp];
-              out[3 * gid + 1] = acc[k][3 * grp + 1];
-              out[3 * gid + 2] = acc[k][3 * grp + 2];
-            }
-          }
-        } else {
-          const long long pidx = (long long)A.blk_sh_base[b] + (s - nown);
-          R* dst = A.partial + pidx * NCH;
-#pragma unroll
-          for (int ch = 0; ch < NCH; ++ch) dst[ch] = acc[k][ch];
+          for (int ch = 0; ch < NCH; ++ch) at(dst, (unsigned)s * NCH + ch) = acc[k][ch];
         }
       }
     }
     if constexpr (RED) {
-      block_sum2(e_acc, q_acc, red);
+      double e = (double)e_acc, q = (double)q_acc;
+      block_sum2(e, q, red);
       if (tid == 0) {
-        A.blk_red[2 * b] = e_acc;
-        A.blk_red[2 * b + 1] = q_acc;
+        A.blk_red[2 * b] = e;
+        A.blk_red[2 * b + 1] = q;
       }
     }
     __syncthreads();
   }
 }
 
-// Sum the partials of every shared vertex (ascending block order).
+// Vertex sums of the slot partials, in ascending block order.  One thread
+// per vertex over the instance-aligned vertex blocks.
 template <class R, int NCH>
-__global__ __launch_bounds__(NTH) void shared_reduce_kernel(
-    const int32_t* sh_vtx, const int32_t* sh_off, const int32_t* sh_src, const R* partial,
-    long long n_partials, long long n_sh, const int64_t* inst_vtx_off, long long n_inst,
-    const int32_t* active, R* out_a, R* out_b, int h_only, int accumulate) {
-  const long long i = (long long)blockIdx.x * NTH + threadIdx.x;
-  if (i >= n_sh) return;
-  const long long v = sh_vtx[i];
-  if (active != nullptr) {
-    // instance of v by binary search over the vertex offsets
-    long long lo = 0, hi = n_inst;
-    while (hi - lo > 1) {
-      long long mid = (lo + hi) >> 1;
-      if (inst_vtx_off[mid] <= v) lo = mid; else hi = mid;
-    }
-    if (active[lo] == 0) return;
-  }
+__global__ __launch_bounds__(NTH) void vertex_reduce_kernel(const int32_t* vblk_off, const int32_t* vblk_inst,
+                                                            const int32_t* vtx_slot_off,
+                                                            const int32_t* vtx_slot_src, const R* partial,
+                                                            const int32_t* active, R* out_a, R* out_b, int h_only,
+                                                            int accumulate) {
+  const int vb = blockIdx.x;
+  if (active != nullptr && active[vblk_inst[vb]] == 0) return;
+  const int v = vblk_off[vb] + threadIdx.x;
+  if (v >= vblk_off[vb + 1]) return;
   R acc[NCH];
 #pragma unroll
   for (int ch = 0; ch < NCH; ++ch) acc[ch] = R(0);
-  for (int k = sh_off[i]; k < sh_off[i + 1]; ++k) {
-    const R* src = partial + (long long)sh_src[k] * NCH;
+  const int k1 = at(vtx_slot_off, (unsigned)v + 1);
+  for (int k = at(vtx_slot_off, (unsigned)v); k < k1; ++k) {
+    const unsigned src = (unsigned)at(vtx_slot_src, (unsigned)k) * NCH;
 #pragma unroll
-    for (int ch = 0; ch < NCH; ++ch) acc[ch] += src[ch];
+    for (int ch = 0; ch < NCH; ++ch) acc[ch] += at(partial, src + ch);
   }
 #pragma unroll
   for (int grp = 0; grp < NCH / 3; ++grp) {
     R* out = h_only ? out_b : (grp == 0 ? out_a : out_b);
+    const unsigned o = 3u * (unsigned)v;
     if (accumulate) {
-      out[3 * v] += acc[3 * grp];
-      out[3 * v + 1] += acc[3 * grp + 1];
-      out[3 * v + 2] += acc[3 * grp + 2];
+      at(out, o) += acc[3 * grp];
+      at(out, o + 1) += acc[3 * grp + 1];
+      at(out, o + 2) += acc[3 * grp + 2];
     } else {
-      out[3 * v] = acc[3 * grp];
-      out[3 * v + 1] = acc[3 * grp + 1];
-      out[3 * v + 2] = acc[3 * grp + 2];
+      at(out, o) = acc[3 * grp];
+      at(out, o + 1) = acc[3 * grp + 1];
+      at(out, o + 2) = acc[3 * grp + 2];
     }
   }
 }
@@ -442,8 +445,6 @@ static ElemArgs<R, X> make_args(apl_mesh* m, const void* mu, const void* lam, co
   A.slot = (const ushort4*)m->d.slot;
   A.blk_tet_off = m->d.blk_tet_off;
   A.blk_loc_off = m->d.blk_loc_off;
-  A.blk_nown = m->d.blk_nown;
-  A.blk_sh_base = m->d.blk_sh_base;
   A.blk_inst = m->d.blk_inst;
   A.loc_gid = m->d.loc_gid;
   A.loc_inc_off = m->d.loc_inc_off;
@@ -458,8 +459,7 @@ static ElemArgs<R, X> make_args(apl_mesh* m, const void* mu, const void* lam, co
   A.out_b = (R*)out_b;
   A.partial = (R*)m->d.partial;
   A.blk_red = m->d.blk_red;
-  A.T = m->L.T;
-  A.n_partials = m->L.n_partials;
+  A.T = (int)m->L.T;
   A.nblocks = (int)m->L.nblocks;
   A.nloc_max = m->L.nloc_max;
   A.chunks = m->L.chunks;
@@ -471,7 +471,7 @@ template <int MAT, unsigned WHAT, class R, class X>
 static size_t smem_bytes(const apl_mesh* m) {
   constexpr bool NEEDV = (WHAT & (W_HP | W_HQ | W_MIX)) != 0;
   constexpr int NCH = ((WHAT & W_G) ? 3 : 0) + ((WHAT & W_H) ? 3 : 0) + ((WHAT & W_HP) ? 3 : 0);
-  size_t n = 8 * sizeof(double) + 3 * (size_t)m->L.nloc_max * (sizeof(X) + (NEEDV ? sizeof(R) : 0));
+  size_t n = 8 * sizeof(double) + 4 * (size_t)m->L.nloc_max * (sizeof(X) + (NEEDV ? sizeof(R) : 0));
   n += sizeof(R) * (size_t)NCH * 4 * NTH;
   if (NCH > 0) n += sizeof(uint16_t) * ((size_t)4 * NTH * m->L.chunks + (size_t)m->L.nloc_max + 1);
   return n;
@@ -600,29 +600,28 @@ int elem_grid(apl_mesh* m, int material, unsigned what, int x_f64) {
 }
 
 template <class R, int NCH>
-static int shared_one(apl_mesh* m, void* out_a, void* out_b, int h_only, int accumulate, const int32_t* active,
+static int vertex_one(apl_mesh* m, void* out_a, void* out_b, int h_only, int accumulate, const int32_t* active,
                       hipStream_t s) {
-  const long long n = m->L.n_shared_vtx;
-  if (n == 0) return 0;
-  const int grid = (int)((n + NTH - 1) / NTH);
-  hipLaunchKernelGGL((shared_reduce_kernel<R, NCH>), dim3(grid), dim3(NTH), 0, s, m->d.sh_vtx, m->d.sh_off,
-                     m->d.sh_src, (const R*)m->d.partial, (long long)m->L.n_partials, n, m->d.inst_vtx_off,
-                     (long long)m->L.n_inst, active, (R*)out_a, (R*)out_b, h_only, accumulate);
+  const unsigned nvb = (unsigned)m->L.vblk_inst.size();
+  if (nvb == 0) return 0;
+  hipLaunchKernelGGL((vertex_reduce_kernel<R, NCH>), dim3(nvb), dim3(NTH), 0, s, m->d.vblk_off, m->d.vblk_inst,
+                     m->d.vtx_slot_off, m->d.vtx_slot_src, (const R*)m->d.partial, active, (R*)out_a, (R*)out_b,
+                     h_only, accumulate);
   APL_CHECK_HIP(hipGetLastError());
   return 0;
 }
 
-int launch_shared_reduce(apl_mesh* m, unsigned what, void* out_a, void* out_b, int accumulate,
+int launch_vertex_reduce(apl_mesh* m, unsigned what, void* out_a, void* out_b, int accumulate,
                          const int32_t* active, hipStream_t s) {
   const bool g = what & W_G, h = what & W_H, hp = what & W_HP;
   const int nch = (g ? 3 : 0) + (h ? 3 : 0) + (hp ? 3 : 0);
   const int h_only = (h && !g && !hp) ? 1 : 0;
   if (nch == 0) return 0;
   if (m->dtype == DT_F32)
-    return nch == 6 ? shared_one<float, 6>(m, out_a, out_b, h_only, accumulate, active, s)
-                    : shared_one<float, 3>(m, out_a, out_b, h_only, accumulate, active, s);
-  return nch == 6 ? shared_one<double, 6>(m, out_a, out_b, h_only, accumulate, active, s)
-                  : shared_one<double, 3>(m, out_a, out_b, h_only, accumulate, active, s);
+    return nch == 6 ? vertex_one<float, 6>(m, out_a, out_b, h_only, accumulate, active, s)
+                    : vertex_one<float, 3>(m, out_a, out_b, h_only, accumulate, active, s);
+  return nch == 6 ? vertex_one<double, 6>(m, out_a, out_b, h_only, accumulate, active, s)
+                  : vertex_one<double, 3>(m, out_a, out_b, h_only, accumulate, active, s);
 }
 
 int launch_inst_reduce(apl_mesh* m, double* out_e, double* out_q, int accumulate, const int32_t* active,

@@ -34,23 +34,19 @@ struct DevLayout {
   uint16_t* slot = nullptr;  // 4*T
   int32_t* blk_tet_off = nullptr;
   int32_t* blk_loc_off = nullptr;
-  int32_t* blk_nown = nullptr;
-  int32_t* blk_sh_base = nullptr;
   int32_t* blk_inst = nullptr;
   int64_t* inst_blk_off = nullptr;
   int32_t* loc_gid = nullptr;
   uint16_t* loc_inc_off = nullptr;
   uint16_t* loc_inc = nullptr;
-  int32_t* sh_vtx = nullptr;
-  int32_t* sh_off = nullptr;
-  int32_t* sh_src = nullptr;
-  int32_t* vtx_sh_off = nullptr;  // V+1: partial range of every vertex (empty if owned)
+  int32_t* vtx_slot_off = nullptr;  // V+1: slot range of every vertex
+  int32_t* vtx_slot_src = nullptr;  // n_slots: slot ids, ascending per vertex
   int32_t* vblk_off = nullptr;
   int32_t* vblk_inst = nullptr;
   int64_t* inst_vblk_off = nullptr;
   int64_t* inst_vtx_off = nullptr;
   // workspace
-  void* partial = nullptr;   // MAX_CH * n_partials (SoA by channel)
+  void* partial = nullptr;   // n_slots x MAX_CH, slot-major (one partial sum per local slot)
   double* blk_red = nullptr; // 2 * nblocks
   double* vblk_dots = nullptr;  // N_DOTS * nvblk
 };
@@ -71,14 +67,14 @@ namespace apl {
 
 inline size_t real_size(const apl_mesh* m) { return m->dtype == DT_F64 ? 8 : 4; }
 
-// element pass: per-block partials (vertex sums of owned slots are written
-// directly into out_a/out_b; shared slots into the partial workspace)
+// element pass: per-slot partial sums into the partial workspace (scatter
+// passes), per-block energy/quadratic-form sums, or per-tet outputs (mixed);
 // x_f64: x holds FP64 displacements while the mesh is FP32 (solver passes)
 int launch_elem(apl_mesh* m, int material, unsigned what, const void* mu, const void* lam,
                 const void* x, const void* p, const double* alpha, const int32_t* active,
                 void* out_a, void* out_b, int accumulate, int grid_override, hipStream_t s, int x_f64 = 0);
-// vertex sums of shared vertices from the partials
-int launch_shared_reduce(apl_mesh* m, unsigned what, void* out_a, void* out_b, int accumulate,
+// vertex sums of the slot partials into the (V,3) outputs
+int launch_vertex_reduce(apl_mesh* m, unsigned what, void* out_a, void* out_b, int accumulate,
                          const int32_t* active, hipStream_t s);
 // per-instance sums of the block energies / quadratic forms
 int launch_inst_reduce(apl_mesh* m, double* out_e, double* out_q, int accumulate,

@@ -1,14 +1,14 @@
 // Host-side block partition of a (possibly batched) tet mesh.
 //
-// The element kernels process runs of consecutive tets as one block.  For a
-// block, the distinct vertices its tets touch are its "local slots"; a slot
-// is *owned* when every tet incident to that vertex lies in the block (its
-// vertex sum is complete inside the block and is written straight to the
-// output) and *shared* otherwise (the block writes a partial sum that a
-// second, vertex-wise pass adds up).  Both the in-block sums and the
-// cross-block sums run over CSR lists in ascending order, so every result is
-// bitwise reproducible -- this replaces the float atomics of
-// warp/fem/_base.py:261,288-289,321-322,348-349,381.
+// The element kernels process runs of consecutive tets as one block.  The
+// distinct vertices a block's tets touch are its "local slots" (first-touch
+// order).  A block sums the corner contributions of its tets per local slot
+// in LDS, over an ascending in-block CSR, and writes one partial sum per slot
+// into a slot-major buffer (a contiguous run per block).  A vertex-wise pass
+// then adds, for every vertex, the partials of the slots that refer to it, in
+// ascending block order (the vertex CSR below).  Both sums run in a fixed
+// order, so every result is bitwise reproducible -- this replaces the float
+// atomics of warp/fem/_base.py:261,288-289,321-322,348-349,381.
 #pragma once
 
 #include <cstdint>
@@ -23,23 +23,20 @@ struct Layout {
   int nloc_cap = 1024;   // max local slots per block
   int64_t T = 0, V = 0, n_inst = 0;
   int64_t nblocks = 0;
-  int nloc_max = 0;      // max local slots over all blocks
-  int64_t n_partials = 0;
-  int64_t n_shared_vtx = 0;
+  int nloc_max = 0;        // max local slots over all blocks
+  int64_t n_slots = 0;     // total local slots = number of partial sums per pass
+  int64_t n_shared_vtx = 0;  // vertices touched by more than one block (statistic)
 
   std::vector<int32_t> blk_tet_off;   // nblocks+1
-  std::vector<int32_t> blk_loc_off;   // nblocks+1 (into loc_gid)
-  std::vector<int32_t> blk_nown;      // nblocks
-  std::vector<int32_t> blk_sh_base;   // nblocks (first partial index of the block)
+  std::vector<int32_t> blk_loc_off;   // nblocks+1 (into loc_gid; slot id = blk_loc_off[b] + s)
   std::vector<int32_t> blk_inst;      // nblocks
   std::vector<int64_t> inst_blk_off;  // n_inst+1
-  std::vector<int32_t> loc_gid;       // total slots
-  std::vector<uint16_t> loc_inc_off;  // total slots + nblocks (nloc+1 per block)
-  std::vector<uint16_t> loc_inc;      // 4*T: per block, (tl*4 + corner) sorted by slot then id
+  std::vector<int32_t> loc_gid;       // n_slots: vertex of each slot
+  std::vector<uint16_t> loc_inc_off;  // n_slots + nblocks (nloc+1 per block)
+  std::vector<uint16_t> loc_inc;      // 4*T: per block, (tl*4 + corner) grouped by slot, ascending
   std::vector<uint16_t> slot;         // 4*T: local slot of each corner
-  std::vector<int32_t> sh_vtx;        // shared vertices (ascending)
-  std::vector<int32_t> sh_off;        // n_shared_vtx+1 into sh_src
-  std::vector<int32_t> sh_src;        // partial indices, ascending
+  std::vector<int32_t> vtx_slot_off;  // V+1 into vtx_slot_src
+  std::vector<int32_t> vtx_slot_src;  // n_slots: slot ids per vertex, ascending (= block order)
   // vertex blocks for the vector kernels: aligned to instance ranges
   int vb = 256;
   std::vector<int32_t> vblk_off;      // nvblk+1

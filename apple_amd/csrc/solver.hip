@@ -73,10 +73,9 @@ template <class R>
 struct VecArgs {
   const int32_t* vblk_off;
   const int32_t* vblk_inst;
-  const int32_t* vtx_sh_off;
-  const int32_t* sh_src;
+  const int32_t* vtx_slot_off;
+  const int32_t* vtx_slot_src;
   const R* partial;
-  long long n_partials;
   const uint8_t* mask;
   const R* fext;
   double* x;  // iterate, always FP64 (the line-search step is taken in FP64)
@@ -138,33 +137,21 @@ __global__ __launch_bounds__(NTH) void pncg_reduce_kernel(VecArgs<R> A) {
 #pragma unroll
   for (int k = 0; k < N_DOTS; ++k) d[k] = 0.0;
   if (v < v1) {
-    R ge[3], h[3];
-    const int o0 = A.vtx_sh_off[v], o1 = A.vtx_sh_off[v + 1];
-    if (o1 > o0) {
+    // vertex sums of the slot partials (ascending block order)
+    R ge[3] = {R(0), R(0), R(0)}, h[3] = {R(0), R(0), R(0)};
+    const int o1 = A.vtx_slot_off[v + 1];
+    for (int k = A.vtx_slot_off[v]; k < o1; ++k) {
+      const R* src = A.partial + (unsigned)A.vtx_slot_src[k] * 6;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        ge[c] = R(0);
-        h[c] = R(0);
+        ge[c] += src[c];
+        h[c] += src[3 + c];
       }
-      for (int k = o0; k < o1; ++k) {
-        const R* src = A.partial + (long long)A.sh_src[k] * 6;
+    }
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          ge[c] += src[c];
-          h[c] += src[3 + c];
-        }
-      }
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        A.gE[3 * v + c] = ge[c];
-        A.hd[3 * v + c] = h[c];
-      }
-    } else {
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        ge[c] = A.gE[3 * v + c];
-        h[c] = A.hd[3 * v + c];
-      }
+    for (int c = 0; c < 3; ++c) {
+      A.gE[3 * v + c] = ge[c];
+      A.hd[3 * v + c] = h[c];
     }
     const unsigned msk = A.mask != nullptr ? A.mask[v] : 0u;
 #pragma unroll
@@ -374,10 +361,9 @@ template <class R>
 struct CgArgs {
   const int32_t* vblk_off;
   const int32_t* vblk_inst;
-  const int32_t* vtx_sh_off;
-  const int32_t* sh_src;
+  const int32_t* vtx_slot_off;
+  const int32_t* vtx_slot_src;
   const R* partial;
-  long long n_partials;
   const uint8_t* mask;
   const R* rhs;
   R* y;
@@ -407,19 +393,12 @@ __global__ __launch_bounds__(NTH) void cg_vec_kernel(CgArgs<R> A) {
   if (in) {
     if constexpr (STAGE == CG_DIAG || STAGE == CG_Q) {
       R* out = STAGE == CG_DIAG ? A.hd : A.q;
-      R s3[3];
-      const int o0 = A.vtx_sh_off[v], o1 = A.vtx_sh_off[v + 1];
-      if (o1 > o0) {
+      R s3[3] = {R(0), R(0), R(0)};
+      const int o1 = A.vtx_slot_off[v + 1];
+      for (int k = A.vtx_slot_off[v]; k < o1; ++k) {
+        const R* src = A.partial + (unsigned)A.vtx_slot_src[k] * 3;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) s3[c] = R(0);
-        for (int k = o0; k < o1; ++k) {
-          const R* src = A.partial + (long long)A.sh_src[k] * 3;
-#pragma unroll
-          for (int c = 0; c < 3; ++c) s3[c] += src[c];
-        }
-      } else {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) s3[c] = out[3 * v + c];
+        for (int c = 0; c < 3; ++c) s3[c] += src[c];
       }
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
@@ -561,10 +540,9 @@ static VecArgs<R> vec_args(apl_solver* s, double* x, const void* fext, const uin
   VecArgs<R> A;
   A.vblk_off = m->d.vblk_off;
   A.vblk_inst = m->d.vblk_inst;
-  A.vtx_sh_off = m->d.vtx_sh_off;
-  A.sh_src = m->d.sh_src;
+  A.vtx_slot_off = m->d.vtx_slot_off;
+  A.vtx_slot_src = m->d.vtx_slot_src;
   A.partial = (const R*)m->d.partial;
-  A.n_partials = m->L.n_partials;
   A.mask = mask;
   A.fext = (const R*)fext;
   A.x = x;
@@ -740,10 +718,9 @@ static CgArgs<R> cg_args(apl_solver* s, const void* rhs, void* y, const uint8_t*
   CgArgs<R> A;
   A.vblk_off = m->d.vblk_off;
   A.vblk_inst = m->d.vblk_inst;
-  A.vtx_sh_off = m->d.vtx_sh_off;
-  A.sh_src = m->d.sh_src;
+  A.vtx_slot_off = m->d.vtx_slot_off;
+  A.vtx_slot_src = m->d.vtx_slot_src;
   A.partial = (const R*)m->d.partial;
-  A.n_partials = m->L.n_partials;
   A.mask = mask;
   A.rhs = (const R*)rhs;
   A.y = (R*)y;
