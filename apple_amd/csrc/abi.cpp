@@ -36,8 +36,8 @@ static int alloc_bytes(apl_mesh* m, void** dst, size_t n) {
 
 static void free_mesh(apl_mesh* m) {
   DevLayout& d = m->d;
-  void* ptrs[] = {d.dminv,        d.dV,           d.slot,        d.blk_tet_off,  d.blk_loc_off,
-                  d.blk_inst,     d.inst_blk_off, d.loc_gid,     d.loc_inc_off,  d.loc_inc,
+  void* ptrs[] = {d.dminv,        d.dV,           d.corner,      d.blk_tet_off,  d.blk_loc_off,
+                  d.blk_inst,     d.inst_blk_off, d.loc_gid,     d.blk_co_off,   d.chunk_off,
                   d.vtx_slot_off, d.vtx_slot_src, d.vblk_off,    d.vblk_inst,    d.inst_vblk_off,
                   d.inst_vtx_off, d.partial,      d.blk_red,     d.vblk_dots};
   for (void* p : ptrs)
@@ -142,14 +142,14 @@ int apl_mesh_create(int dtype, int64_t n_tets, int64_t n_verts, const int32_t* c
     rc |= alloc_bytes(m, &d.dV, bv.size());
     if (!rc && !bv.empty() && hipMemcpy(d.dV, bv.data(), bv.size(), hipMemcpyHostToDevice) != hipSuccess) rc = -1;
   }
-  rc |= upload(m, &d.slot, L.slot.data(), L.slot.size());
+  rc |= upload(m, &d.corner, L.corner.data(), L.corner.size());
   rc |= upload(m, &d.blk_tet_off, L.blk_tet_off.data(), L.blk_tet_off.size());
   rc |= upload(m, &d.blk_loc_off, L.blk_loc_off.data(), L.blk_loc_off.size());
   rc |= upload(m, &d.blk_inst, L.blk_inst.data(), L.blk_inst.size());
   rc |= upload(m, &d.inst_blk_off, L.inst_blk_off.data(), L.inst_blk_off.size());
   rc |= upload(m, &d.loc_gid, L.loc_gid.data(), L.loc_gid.size());
-  rc |= upload(m, &d.loc_inc_off, L.loc_inc_off.data(), L.loc_inc_off.size());
-  rc |= upload(m, &d.loc_inc, L.loc_inc.data(), L.loc_inc.size());
+  rc |= upload(m, &d.blk_co_off, L.blk_co_off.data(), L.blk_co_off.size());
+  rc |= upload(m, &d.chunk_off, L.chunk_off.data(), L.chunk_off.size());
   rc |= upload(m, &d.vtx_slot_off, L.vtx_slot_off.data(), L.vtx_slot_off.size());
   rc |= upload(m, &d.vtx_slot_src, L.vtx_slot_src.data(), L.vtx_slot_src.size());
   rc |= upload(m, &d.vblk_off, L.vblk_off.data(), L.vblk_off.size());
@@ -165,9 +165,8 @@ int apl_mesh_create(int dtype, int64_t n_tets, int64_t n_verts, const int32_t* c
     return -1;
   }
   // free the host-side tables the device now holds; keep the sizes
-  L.slot = std::vector<uint16_t>();
-  L.loc_inc = std::vector<uint16_t>();
-  L.loc_inc_off = std::vector<uint16_t>();
+  L.corner = std::vector<uint16_t>();
+  L.chunk_off = std::vector<uint16_t>();
   L.loc_gid = std::vector<int32_t>();
   L.vtx_slot_src = std::vector<int32_t>();
   *out = m;

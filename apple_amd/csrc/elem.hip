@@ -11,14 +11,15 @@
 //    stages the displacement of every vertex the block touches into LDS once
 //    (padded to 4 lanes, one wide LDS read per corner);
 //  * per-tet element data are SoA in HBM (9 x T inverse rest shapes, dV, mu,
-//    lambda, 4 x uint16 local corner slots), addressed by 32-bit offsets from
-//    per-block uniform base pointers: every load is coalesced and needs no
-//    per-lane 64-bit address arithmetic;
-//  * per-corner contributions go to LDS in [channel][corner][thread] order
-//    (bank-conflict free writes); each local vertex sums its incidences over
-//    an ascending CSR list (a fixed order: bitwise reproducible) and writes
-//    one partial per slot into a slot-major buffer -- a contiguous run per
-//    block; the vertex pass adds the partials of every vertex in block order;
+//    lambda, and one 16-byte record of 4 local corner slots and 4 corner
+//    positions), addressed by 32-bit offsets from per-block uniform base
+//    pointers: every load is coalesced;
+//  * each corner's contribution goes to LDS at its position (corners ranked
+//    by slot, then tet, then corner), so the corners of a local vertex form a
+//    contiguous run; one to four lanes sum each run in a fixed order
+//    (bitwise reproducible) and one partial per slot goes into a slot-major
+//    buffer -- a contiguous run per block; the vertex pass adds the partials
+//    of every vertex in block order;
 //  * energies and quadratic forms are summed per thread, then in FP64 by a
 //    fixed wave-shuffle tree and a per-instance pass over the block sums;
 //  * the grid is persistent and XCD-aware: workgroup i runs on XCD i % 8, and
@@ -33,13 +34,13 @@ template <class R, class X>
 struct ElemArgs {
   const R* dminv;
   const R* dV;
-  const ushort4* slot;
+  const uint4* corner;  // per tet: 4 x u16 local slots, 4 x u16 chunk positions
   const int32_t* blk_tet_off;
   const int32_t* blk_loc_off;
   const int32_t* blk_inst;
   const int32_t* loc_gid;
-  const uint16_t* loc_inc_off;
-  const uint16_t* loc_inc;
+  const int32_t* blk_co_off;
+  const uint16_t* chunk_off;
   const R* mu;
   const R* lam;
   const X* x;  // displacements (X = R, or double when the solver keeps them in FP64)
@@ -109,23 +110,66 @@ __device__ __forceinline__ T& at(T* base, unsigned i) {
   return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + i * (unsigned)sizeof(T));
 }
 
+// Raw buffer loads: a scalar resource (base address) plus a per-lane and a
+// uniform byte offset.  Used for the per-tet record so that no lane does
+// 64-bit address arithmetic.  The resource's fourth word is the gfx9 raw
+// buffer format; the record count is left unbounded (indices are exact).
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+__device__ __forceinline__ rsrc_t make_rsrc(const void* base) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, -1, 0x00020000);
+}
+template <class R>
+__device__ __forceinline__ R bload(rsrc_t r, unsigned voff, unsigned soff);
+template <>
+__device__ __forceinline__ float bload<float>(rsrc_t r, unsigned voff, unsigned soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+template <>
+__device__ __forceinline__ double bload<double>(rsrc_t r, unsigned voff, unsigned soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+__device__ __forceinline__ uint4 bload_u4(rsrc_t r, unsigned voff, unsigned soff) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+  return make_uint4(v[0], v[1], v[2], v[3]);
+}
+
 // Per-tet record, loaded into registers ahead of the block's LDS staging.
 template <class R>
 struct TetRec {
   R B[9];
   R dV, mu, lam;
-  ushort4 sl;
+  uint4 cr;  // local slots (x, y), positions (z, w): two u16 per word
 };
 
-// Loads from a per-block uniform base with a 32-bit lane offset.
+// Resources of one block's tet data.  dminv is SoA (row kk at kk * T), so its
+// base moves with the block and the row offset stays below 2^32 bytes.
+struct BlockRes {
+  rsrc_t dm, dv, mu, lam, cr;
+  unsigned row, t0b;  // row stride and block start, in bytes of R
+};
+
 template <int MAT, class R, class X>
-__device__ inline void load_rec(const ElemArgs<R, X>& A, int t0, unsigned tl, TetRec<R>& r) {
+__device__ inline BlockRes block_res(const ElemArgs<R, X>& A, int t0) {
+  BlockRes br;
+  br.dm = make_rsrc(A.dminv + t0);
+  br.dv = make_rsrc(A.dV);
+  br.mu = make_rsrc(A.mu);
+  br.lam = make_rsrc(MAT == MAT_SNH ? A.lam : A.mu);
+  br.cr = make_rsrc(A.corner);
+  br.row = (unsigned)A.T * (unsigned)sizeof(R);
+  br.t0b = (unsigned)t0 * (unsigned)sizeof(R);
+  return br;
+}
+
+template <int MAT, class R>
+__device__ inline void load_rec(const BlockRes& br, unsigned tl, TetRec<R>& r) {
+  const unsigned vo = tl * (unsigned)sizeof(R);
 #pragma unroll
-  for (int kk = 0; kk < 9; ++kk) r.B[kk] = at(A.dminv + ((size_t)kk * A.T + t0), tl);
-  r.dV = at(A.dV + t0, tl);
-  r.mu = at(A.mu + t0, tl);
-  r.lam = (MAT == MAT_SNH) ? at(A.lam + t0, tl) : R(0);
-  r.sl = at(A.slot + t0, tl);
+  for (int kk = 0; kk < 9; ++kk) r.B[kk] = bload<R>(br.dm, vo, (unsigned)kk * br.row);
+  r.dV = bload<R>(br.dv, vo, br.t0b);
+  r.mu = bload<R>(br.mu, vo, br.t0b);
+  r.lam = (MAT == MAT_SNH) ? bload<R>(br.lam, vo, br.t0b) : R(0);
+  r.cr = bload_u4(br.cr, tl * 16u, br.t0b / (unsigned)sizeof(R) * 16u);
 }
 
 // 4-lane padded vector (one 16/32-byte LDS access)
@@ -134,7 +178,10 @@ struct alignas(4 * sizeof(T)) Vec4 {
   T v[4];
 };
 
-// SPT: local slots reduced per thread (nloc_max <= SPT * NTH)
+// SPT: slot items per thread.  A block with nloc slots splits each slot's
+// position range over tps = 4, 2 or 1 adjacent lanes (the most that fit
+// SPT * NTH items), so all four waves share the sums; the lanes' interleaved
+// partial sums are combined by a fixed xor-shuffle tree.
 template <int MAT, unsigned WHAT, class R, class X, int SPT>
 __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R, X> A) {
   constexpr bool HAS_G = (WHAT & W_G) != 0;
@@ -145,16 +192,14 @@ __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R, X> A) {
   constexpr int NCHA = NCH > 0 ? NCH : 1;
   constexpr bool NEEDV = (WHAT & (W_HP | W_HQ | W_MIX)) != 0;
   constexpr bool RED = (WHAT & (W_E | W_HQ)) != 0;
+  constexpr int CS = 4 * NTH;  // channel stride of the contribution buffer
   static_assert(NCH <= MAX_CH, "too many channels");
 
   extern __shared__ double smem_d[];
   double* red = smem_d;
   X* u_lds = reinterpret_cast<X*>(smem_d + 8);  // 4 per slot (padded)
   R* v_lds = reinterpret_cast<R*>(u_lds + 4 * A.nloc_max);
-  R* contrib = v_lds + (NEEDV ? 4 * A.nloc_max : 0);
-  // the block's incidence CSR (grouped by local slot) staged next to contrib
-  uint16_t* inc_lds = reinterpret_cast<uint16_t*>(contrib + (SCATTER ? NCH * 4 * NTH : 0));
-  uint16_t* off_lds = inc_lds + (SCATTER ? 4 * NTH * A.chunks : 0);
+  R* contrib = v_lds + (NEEDV ? 4 * A.nloc_max : 0);  // [channel][position]
 
   const int tid = threadIdx.x;
   const int pid = blockIdx.x;
@@ -170,11 +215,14 @@ __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R, X> A) {
     const int sb = A.blk_loc_off[b];
     const int nloc = A.blk_loc_off[b + 1] - sb;
     const X alpha = A.alpha != nullptr ? (X)A.alpha[inst] : X(0);
+    const int lg = (4 * nloc <= SPT * NTH) ? 2 : ((2 * nloc <= SPT * NTH) ? 1 : 0);
+    const int tps = 1 << lg;
 
     // this thread's first tet record, loaded before the staging barrier so the
     // two global-load latencies overlap
+    const BlockRes br = block_res<MAT>(A, t0);
     TetRec<R> rec;
-    if (tid < nt) load_rec<MAT>(A, t0, (unsigned)tid, rec);
+    if (tid < nt) load_rec<MAT>(br, (unsigned)tid, rec);
 
     // stage the block's vertex window (u = x + alpha p for a trial point)
     const int32_t* gids = A.loc_gid + sb;
@@ -199,36 +247,39 @@ __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R, X> A) {
         *reinterpret_cast<Vec4<R>*>(v_lds + 4 * s) = v4;
       }
     }
-    if constexpr (SCATTER) {
-      const uint16_t* inc_g = A.loc_inc + 4 * (size_t)t0;
-      for (int k = tid; k < 4 * nt; k += NTH) inc_lds[k] = at(inc_g, (unsigned)k);
-      const uint16_t* off_g = A.loc_inc_off + ((size_t)sb + b);
-      for (int k = tid; k <= nloc; k += NTH) off_lds[k] = at(off_g, (unsigned)k);
-    }
     __syncthreads();
 
     R acc[SPT][NCHA];
-    int cur[SPT], endi[SPT];
 #pragma unroll
-    for (int k = 0; k < SPT; ++k) {
-      const int s = tid + k * NTH;
-      cur[k] = 0;
-      endi[k] = 0;
-      if (SCATTER && s < nloc) {
-        cur[k] = off_lds[s];
-        endi[k] = off_lds[s + 1];
-      }
+    for (int k = 0; k < SPT; ++k)
 #pragma unroll
       for (int ch = 0; ch < NCHA; ++ch) acc[k][ch] = R(0);
-    }
     R e_acc = R(0), q_acc = R(0);
+    const uint16_t* co = A.chunk_off + A.blk_co_off[b];
     const int nchunks = (nt + NTH - 1) / NTH;
     for (int c = 0; c < nchunks; ++c) {
       const int tl = c * NTH + tid;
+      // this chunk's position ranges of the thread's slot items
+      int q0[SPT], q1[SPT];
+      if constexpr (SCATTER) {
+        const uint16_t* cc = co + c * (nloc + 1);
+#pragma unroll
+        for (int k = 0; k < SPT; ++k) {
+          const int item = tid + k * NTH;
+          const int s = item >> lg;
+          q0[k] = 0;
+          q1[k] = 0;
+          if (s < nloc) {
+            q0[k] = at(cc, (unsigned)s) + (item & (tps - 1));
+            q1[k] = at(cc, (unsigned)s + 1);
+          }
+        }
+      }
       if (tl < nt) {
-        if (c > 0) load_rec<MAT>(A, t0, (unsigned)tl, rec);
+        if (c > 0) load_rec<MAT>(br, (unsigned)tl, rec);
         const TetGeom<R> g = make_geom(rec.B, rec.dV);
-        const int sidx[4] = {rec.sl.x, rec.sl.y, rec.sl.z, rec.sl.w};
+        const int sidx[4] = {(int)(rec.cr.x & 0xffffu), (int)(rec.cr.x >> 16), (int)(rec.cr.y & 0xffffu),
+                             (int)(rec.cr.y >> 16)};
         // edge differences u_a - u_0 in the storage precision X, then R: F - I
         // depends only on them, and forming them before rounding keeps F
         // accurate relative to the strain rather than to |u|.
@@ -262,62 +313,39 @@ __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R, X> A) {
           }
         }
         if constexpr (SCATTER) {
-          int ch = 0;
-          if constexpr (HAS_G) {
+          const int pos[4] = {(int)(rec.cr.z & 0xffffu), (int)(rec.cr.z >> 16), (int)(rec.cr.w & 0xffffu),
+                              (int)(rec.cr.w >> 16)};
 #pragma unroll
-            for (int a = 0; a < 4; ++a) {
-              contrib[((ch + 0) * 4 + a) * NTH + tid] = o.g[a].x;
-              contrib[((ch + 1) * 4 + a) * NTH + tid] = o.g[a].y;
-              contrib[((ch + 2) * 4 + a) * NTH + tid] = o.g[a].z;
+          for (int a = 0; a < 4; ++a) {
+            R* cp = contrib + pos[a];
+            int ch = 0;
+            if constexpr (HAS_G) {
+              cp[(ch + 0) * CS] = o.g[a].x;
+              cp[(ch + 1) * CS] = o.g[a].y;
+              cp[(ch + 2) * CS] = o.g[a].z;
+              ch += 3;
             }
-            ch += 3;
-          }
-          if constexpr (HAS_H) {
-#pragma unroll
-            for (int a = 0; a < 4; ++a) {
-              contrib[((ch + 0) * 4 + a) * NTH + tid] = o.h[a].x;
-              contrib[((ch + 1) * 4 + a) * NTH + tid] = o.h[a].y;
-              contrib[((ch + 2) * 4 + a) * NTH + tid] = o.h[a].z;
+            if constexpr (HAS_H) {
+              cp[(ch + 0) * CS] = o.h[a].x;
+              cp[(ch + 1) * CS] = o.h[a].y;
+              cp[(ch + 2) * CS] = o.h[a].z;
+              ch += 3;
             }
-            ch += 3;
-          }
-          if constexpr (HAS_HP) {
-#pragma unroll
-            for (int a = 0; a < 4; ++a) {
-              contrib[((ch + 0) * 4 + a) * NTH + tid] = o.hp[a].x;
-              contrib[((ch + 1) * 4 + a) * NTH + tid] = o.hp[a].y;
-              contrib[((ch + 2) * 4 + a) * NTH + tid] = o.hp[a].z;
+            if constexpr (HAS_HP) {
+              cp[(ch + 0) * CS] = o.hp[a].x;
+              cp[(ch + 1) * CS] = o.hp[a].y;
+              cp[(ch + 2) * CS] = o.hp[a].z;
             }
           }
         }
       }
       if constexpr (SCATTER) {
         __syncthreads();
-        if (nchunks == 1) {
-          // the whole incidence list of every slot lies in this chunk
 #pragma unroll
-          for (int k = 0; k < SPT; ++k) {
-            for (int q = cur[k]; q < endi[k]; ++q) {
-              const int id = inc_lds[q];
-              const R* cp = contrib + (id & 3) * NTH + (id >> 2);
+        for (int k = 0; k < SPT; ++k) {
+          for (int q = q0[k]; q < q1[k]; q += tps) {
 #pragma unroll
-              for (int ch = 0; ch < NCH; ++ch) acc[k][ch] += cp[ch * 4 * NTH];
-            }
-          }
-        } else {
-          const int lim = (c + 1) * 4 * NTH;
-          const int base = c * 4 * NTH;
-#pragma unroll
-          for (int k = 0; k < SPT; ++k) {
-            while (cur[k] < endi[k]) {
-              const int id = inc_lds[cur[k]];
-              if (id >= lim) break;
-              const int j = id - base;
-              const R* cp = contrib + (j & 3) * NTH + (j >> 2);
-#pragma unroll
-              for (int ch = 0; ch < NCH; ++ch) acc[k][ch] += cp[ch * 4 * NTH];
-              ++cur[k];
-            }
+            for (int ch = 0; ch < NCH; ++ch) acc[k][ch] += contrib[ch * CS + q];
           }
         }
         __syncthreads();
@@ -325,12 +353,26 @@ __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R, X> A) {
     }
 
     if constexpr (SCATTER) {
-      // one partial per local slot, consecutive slots consecutive in memory
+      // combine the lanes of a slot, then one partial per slot, consecutive
+      // slots consecutive in memory
+      if (lg > 0) {
+#pragma unroll
+        for (int k = 0; k < SPT; ++k)
+#pragma unroll
+          for (int ch = 0; ch < NCH; ++ch) acc[k][ch] += __shfl_xor(acc[k][ch], 1, 64);
+      }
+      if (lg > 1) {
+#pragma unroll
+        for (int k = 0; k < SPT; ++k)
+#pragma unroll
+          for (int ch = 0; ch < NCH; ++ch) acc[k][ch] += __shfl_xor(acc[k][ch], 2, 64);
+      }
       R* dst = A.partial + (size_t)sb * NCH;
 #pragma unroll
       for (int k = 0; k < SPT; ++k) {
-        const int s = tid + k * NTH;
-        if (s < nloc) {
+        const int item = tid + k * NTH;
+        const int s = item >> lg;
+        if (s < nloc && (item & (tps - 1)) == 0) {
 #pragma unroll
           for (int ch = 0; ch < NCH; ++ch) at(dst, (unsigned)s * NCH + ch) = acc[k][ch];
         }
@@ -442,13 +484,13 @@ static ElemArgs<R, X> make_args(apl_mesh* m, const void* mu, const void* lam, co
   ElemArgs<R, X> A;
   A.dminv = (const R*)m->d.dminv;
   A.dV = (const R*)m->d.dV;
-  A.slot = (const ushort4*)m->d.slot;
+  A.corner = (const uint4*)m->d.corner;
   A.blk_tet_off = m->d.blk_tet_off;
   A.blk_loc_off = m->d.blk_loc_off;
   A.blk_inst = m->d.blk_inst;
   A.loc_gid = m->d.loc_gid;
-  A.loc_inc_off = m->d.loc_inc_off;
-  A.loc_inc = m->d.loc_inc;
+  A.blk_co_off = m->d.blk_co_off;
+  A.chunk_off = m->d.chunk_off;
   A.mu = (const R*)mu;
   A.lam = (const R*)lam;
   A.x = (const X*)x;
@@ -473,7 +515,6 @@ static size_t smem_bytes(const apl_mesh* m) {
   constexpr int NCH = ((WHAT & W_G) ? 3 : 0) + ((WHAT & W_H) ? 3 : 0) + ((WHAT & W_HP) ? 3 : 0);
   size_t n = 8 * sizeof(double) + 4 * (size_t)m->L.nloc_max * (sizeof(X) + (NEEDV ? sizeof(R) : 0));
   n += sizeof(R) * (size_t)NCH * 4 * NTH;
-  if (NCH > 0) n += sizeof(uint16_t) * ((size_t)4 * NTH * m->L.chunks + (size_t)m->L.nloc_max + 1);
   return n;
 }
 

@@ -31,14 +31,14 @@ void set_error(const std::string& s);
 struct DevLayout {
   void* dminv = nullptr;  // 9*T, SoA (k*T + t)
   void* dV = nullptr;     // T
-  uint16_t* slot = nullptr;  // 4*T
+  uint16_t* corner = nullptr;  // 8*T: 4 local slots, then 4 chunk positions per tet
   int32_t* blk_tet_off = nullptr;
   int32_t* blk_loc_off = nullptr;
   int32_t* blk_inst = nullptr;
   int64_t* inst_blk_off = nullptr;
   int32_t* loc_gid = nullptr;
-  uint16_t* loc_inc_off = nullptr;
-  uint16_t* loc_inc = nullptr;
+  int32_t* blk_co_off = nullptr;   // nblocks+1 into chunk_off
+  uint16_t* chunk_off = nullptr;   // per block and chunk: nloc+1 slot offsets into positions
   int32_t* vtx_slot_off = nullptr;  // V+1: slot range of every vertex
   int32_t* vtx_slot_src = nullptr;  // n_slots: slot ids, ascending per vertex
   int32_t* vblk_off = nullptr;

@@ -70,10 +70,11 @@ std::string build_layout(const int32_t* cells, int64_t T, int64_t V, int64_t n_i
   }
   L.nblocks = (int64_t)L.blk_inst.size();
 
-  // 2. local slots in first-touch order, corner slots, in-block CSR by slot
-  L.slot.resize(4 * T);
-  L.loc_inc.resize(4 * T);
+  // 2. local slots in first-touch order; per chunk, corner positions ranked
+  //    by (slot, tet, corner) and the slot offsets into them
+  L.corner.resize(8 * T);
   L.blk_loc_off.push_back(0);
+  L.blk_co_off.push_back(0);
   std::vector<int32_t> local_of(V, -1);
   std::fill(stamp.begin(), stamp.end(), -1);
   std::vector<int32_t> cnt;
@@ -94,20 +95,22 @@ std::string build_layout(const int32_t* cells, int64_t T, int64_t V, int64_t n_i
     const int nloc = (int)(L.loc_gid.size() - base);
     if ((int64_t)L.loc_gid.size() >= (int64_t(1) << 31)) return "too many local slots";
     L.nloc_max = std::max(L.nloc_max, nloc);
-    cnt.assign(nloc + 1, 0);
-    for (int64_t t = t0; t < t1; ++t)
-      for (int c = 0; c < 4; ++c) {
-        const int s = local_of[cells[t * 4 + c]];
-        L.slot[t * 4 + c] = (uint16_t)s;
-        ++cnt[s + 1];
-      }
-    for (int s = 0; s < nloc; ++s) cnt[s + 1] += cnt[s];
-    for (int s = 0; s <= nloc; ++s) L.loc_inc_off.push_back((uint16_t)cnt[s]);
-    for (int64_t t = t0; t < t1; ++t)
-      for (int c = 0; c < 4; ++c) {
-        const int s = local_of[cells[t * 4 + c]];
-        L.loc_inc[4 * t0 + cnt[s]++] = (uint16_t)((t - t0) * 4 + c);
-      }
+    for (int64_t c0 = t0; c0 < t1; c0 += nth) {
+      const int64_t c1 = std::min<int64_t>(t1, c0 + nth);
+      cnt.assign(nloc + 1, 0);
+      for (int64_t t = c0; t < c1; ++t)
+        for (int c = 0; c < 4; ++c) {
+          const int s = local_of[cells[t * 4 + c]];
+          L.corner[8 * t + c] = (uint16_t)s;
+          ++cnt[s + 1];
+        }
+      for (int s = 0; s < nloc; ++s) cnt[s + 1] += cnt[s];
+      for (int s = 0; s <= nloc; ++s) L.chunk_off.push_back((uint16_t)cnt[s]);
+      for (int64_t t = c0; t < c1; ++t)
+        for (int c = 0; c < 4; ++c) L.corner[8 * t + 4 + c] = (uint16_t)cnt[L.corner[8 * t + c]]++;
+    }
+    if (L.chunk_off.size() >= (size_t(1) << 31)) return "too many slot offsets";
+    L.blk_co_off.push_back((int32_t)L.chunk_off.size());
     L.blk_loc_off.push_back((int32_t)L.loc_gid.size());
   }
   L.n_slots = (int64_t)L.loc_gid.size();

@@ -1,14 +1,18 @@
 // Host-side block partition of a (possibly batched) tet mesh.
 //
-// The element kernels process runs of consecutive tets as one block.  The
-// distinct vertices a block's tets touch are its "local slots" (first-touch
-// order).  A block sums the corner contributions of its tets per local slot
-// in LDS, over an ascending in-block CSR, and writes one partial sum per slot
-// into a slot-major buffer (a contiguous run per block).  A vertex-wise pass
-// then adds, for every vertex, the partials of the slots that refer to it, in
-// ascending block order (the vertex CSR below).  Both sums run in a fixed
-// order, so every result is bitwise reproducible -- this replaces the float
-// atomics of warp/fem/_base.py:261,288-289,321-322,348-349,381.
+// The element kernels process runs of consecutive tets as one block, in
+// chunks of nth tets (one tet per thread).  The distinct vertices a block's
+// tets touch are its "local slots" (first-touch order).  Within a chunk, the
+// 4 nth corners are ranked by (slot, tet, corner): a corner's rank is its
+// "position", and the corners of one slot occupy a contiguous position range
+// (the chunk's slot offsets).  A thread writes each corner's contribution at
+// its position in LDS; the slot sums then read contiguous runs.  Each block
+// writes one partial sum per slot into a slot-major buffer (a contiguous run
+// per block), and a vertex-wise pass adds, for every vertex, the partials of
+// the slots that refer to it in ascending block order (the vertex CSR below).
+// Every sum runs in a fixed order, so every result is bitwise reproducible --
+// this replaces the float atomics of
+// warp/fem/_base.py:261,288-289,321-322,348-349,381.
 #pragma once
 
 #include <cstdint>
@@ -32,9 +36,9 @@ struct Layout {
   std::vector<int32_t> blk_inst;      // nblocks
   std::vector<int64_t> inst_blk_off;  // n_inst+1
   std::vector<int32_t> loc_gid;       // n_slots: vertex of each slot
-  std::vector<uint16_t> loc_inc_off;  // n_slots + nblocks (nloc+1 per block)
-  std::vector<uint16_t> loc_inc;      // 4*T: per block, (tl*4 + corner) grouped by slot, ascending
-  std::vector<uint16_t> slot;         // 4*T: local slot of each corner
+  std::vector<int32_t> blk_co_off;    // nblocks+1 into chunk_off
+  std::vector<uint16_t> chunk_off;    // per block and chunk: nloc+1 slot offsets into positions
+  std::vector<uint16_t> corner;       // 8*T: per tet, the local slots of its 4 corners, then their positions
   std::vector<int32_t> vtx_slot_off;  // V+1 into vtx_slot_src
   std::vector<int32_t> vtx_slot_src;  // n_slots: slot ids per vertex, ascending (= block order)
   // vertex blocks for the vector kernels: aligned to instance ranges

@@ -451,42 +451,46 @@ __host__ __device__ inline void tet_eval(const TetGeom<R>& g, R mu, R lam, const
   constexpr bool needV = (WHAT & (W_HP | W_HQ | W_MIX)) != 0;
   M3<R> dF;
   if constexpr (needV) dF = outer_sum(g, v);
+  // dV folded into the moduli once: every output below is per unit volume
+  // times dV (warp/fem/_base.py multiplies each result by dV separately)
+  const R muV = g.dV * mu;
   if constexpr (MAT == MAT_SNH) {
+    const R lamV = g.dV * lam;
     if constexpr ((WHAT & W_E) != 0)
-      o.e = g.dV * (R(0.5) * mu * I.G2 - mu * (I.c2 + I.detG) + R(0.5) * lam * I.Jm1 * I.Jm1);
+      o.e = R(0.5) * muV * I.G2 - muV * (I.c2 + I.detG) + R(0.5) * lamV * I.Jm1 * I.Jm1;
     constexpr bool needC = (WHAT & (W_G | W_H | W_HP | W_HQ | W_MIX)) != 0;
     if constexpr (needC) {
       const M3<R> C = cofF_from_G(G, I.trG);  // g3 = cof F
       if constexpr ((WHAT & W_G) != 0) {
         const M3<R> D = F_minus_cofF(G, I.trG);
         M3<R> P;
-        const R lj = lam * I.Jm1;
+        const R lj = lamV * I.Jm1;
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
-          for (int j = 0; j < 3; ++j) P.m[i][j] = mu * D.m[i][j] + lj * C.m[i][j];
+          for (int j = 0; j < 3; ++j) P.m[i][j] = muV * D.m[i][j] + lj * C.m[i][j];
 #pragma unroll
-        for (int a = 0; a < 4; ++a) o.g[a] = g.dV * mul(P, g.d[a]);
+        for (int a = 0; a < 4; ++a) o.g[a] = mul(P, g.d[a]);
       }
       if constexpr ((WHAT & W_H) != 0) {
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
           V3<R> w = mul(C, g.d[a]);
-          R t = mu * dot(g.d[a], g.d[a]);
-          R hx = g.dV * (lam * w.x * w.x + t);
-          R hy = g.dV * (lam * w.y * w.y + t);
-          R hz = g.dV * (lam * w.z * w.z + t);
+          R t = muV * dot(g.d[a], g.d[a]);
+          R hx = lamV * w.x * w.x + t;
+          R hy = lamV * w.y * w.y + t;
+          R hz = lamV * w.z * w.z + t;
           o.h[a] = v3(hx > R(0) ? hx : R(0), hy > R(0) ? hy : R(0), hz > R(0) ? hz : R(0));
         }
       }
       if constexpr ((WHAT & (W_HP | W_HQ | W_MIX)) != 0) {
-        const R s = -mu + lam * I.Jm1;
+        const R sV = lamV * I.Jm1 - muV;
         const R cd = ddot(C, dF);
         if constexpr ((WHAT & W_HQ) != 0) {
           // 2 F : cof(dF) = d^2/dt^2 det(F + t dF)  (h6_quad, func/_hess_quad.py:59-74)
           M3<R> cdF = cof3(dF);
           R h6 = R(2) * (cdF.m[0][0] + cdF.m[1][1] + cdF.m[2][2] + ddot(G, cdF));
-          R q = g.dV * (lam * cd * cd + mu * ddot(dF, dF) + s * h6);
+          R q = lamV * cd * cd + muV * ddot(dF, dF) + sV * h6;
           o.hq = q > R(0) ? q : R(0);
         }
         if constexpr ((WHAT & W_HP) != 0) {
@@ -500,13 +504,13 @@ __host__ __device__ inline void tet_eval(const TetGeom<R>& g, R mu, R lam, const
           set_col(Hs, 1, cross(f2, p0) - cross(f0, p2));
           set_col(Hs, 2, cross(f0, p1) - cross(f1, p0));
           M3<R> T;
-          const R lc = lam * cd;
+          const R lc = lamV * cd;
 #pragma unroll
           for (int i = 0; i < 3; ++i)
 #pragma unroll
-            for (int j = 0; j < 3; ++j) T.m[i][j] = lc * C.m[i][j] + mu * dF.m[i][j] + s * Hs.m[i][j];
+            for (int j = 0; j < 3; ++j) T.m[i][j] = lc * C.m[i][j] + muV * dF.m[i][j] + sV * Hs.m[i][j];
 #pragma unroll
-          for (int a = 0; a < 4; ++a) o.hp[a] = g.dV * mul(T, g.d[a]);
+          for (int a = 0; a < 4; ++a) o.hp[a] = mul(T, g.d[a]);
         }
         if constexpr ((WHAT & W_MIX) != 0) {
           const M3<R> D = F_minus_cofF(G, I.trG);
@@ -518,12 +522,12 @@ __host__ __device__ inline void tet_eval(const TetGeom<R>& g, R mu, R lam, const
   } else {  // ARAP
     const Svd<R> S = svd_rv_in<R, double>(G);
     if constexpr ((WHAT & W_E) != 0)
-      o.e = g.dV * R(0.5) * mu * (S.sm1[0] * S.sm1[0] + S.sm1[1] * S.sm1[1] + S.sm1[2] * S.sm1[2]);
+      o.e = R(0.5) * muV * (S.sm1[0] * S.sm1[0] + S.sm1[1] * S.sm1[1] + S.sm1[2] * S.sm1[2]);
     if constexpr ((WHAT & W_G) != 0) {
-      R d[3] = {mu * S.sm1[0], mu * S.sm1[1], mu * S.sm1[2]};
+      R d[3] = {muV * S.sm1[0], muV * S.sm1[1], muV * S.sm1[2]};
       const M3<R> P = udv(S, d);
 #pragma unroll
-      for (int a = 0; a < 4; ++a) o.g[a] = g.dV * mul(P, g.d[a]);
+      for (int a = 0; a < 4; ++a) o.g[a] = mul(P, g.d[a]);
     }
     R lk[3];
     if constexpr ((WHAT & (W_H | W_HP | W_HQ)) != 0) arap_lambdas(S.s, lk);
@@ -533,9 +537,9 @@ __host__ __device__ inline void tet_eval(const TetGeom<R>& g, R mu, R lam, const
         V3<R> qd[3];
         twist_apply(S, g.d[a], qd);
         R t = dot(g.d[a], g.d[a]);
-        R hx = g.dV * mu * (t - (lk[0] * qd[0].x * qd[0].x + lk[1] * qd[1].x * qd[1].x + lk[2] * qd[2].x * qd[2].x));
-        R hy = g.dV * mu * (t - (lk[0] * qd[0].y * qd[0].y + lk[1] * qd[1].y * qd[1].y + lk[2] * qd[2].y * qd[2].y));
-        R hz = g.dV * mu * (t - (lk[0] * qd[0].z * qd[0].z + lk[1] * qd[1].z * qd[1].z + lk[2] * qd[2].z * qd[2].z));
+        R hx = muV * (t - (lk[0] * qd[0].x * qd[0].x + lk[1] * qd[1].x * qd[1].x + lk[2] * qd[2].x * qd[2].x));
+        R hy = muV * (t - (lk[0] * qd[0].y * qd[0].y + lk[1] * qd[1].y * qd[1].y + lk[2] * qd[2].y * qd[2].y));
+        R hz = muV * (t - (lk[0] * qd[0].z * qd[0].z + lk[1] * qd[1].z * qd[1].z + lk[2] * qd[2].z * qd[2].z));
         o.h[a] = v3(hx > R(0) ? hx : R(0), hy > R(0) ? hy : R(0), hz > R(0) ? hz : R(0));
       }
     }
@@ -543,7 +547,7 @@ __host__ __device__ inline void tet_eval(const TetGeom<R>& g, R mu, R lam, const
       R q[3], dM[3];
       twist_coords(S, dF, q, dM);
       if constexpr ((WHAT & W_HQ) != 0) {
-        R val = g.dV * mu * (ddot(dF, dF) - (lk[0] * q[0] * q[0] + lk[1] * q[1] * q[1] + lk[2] * q[2] * q[2]));
+        R val = muV * (ddot(dF, dF) - (lk[0] * q[0] * q[0] + lk[1] * q[1] * q[1] + lk[2] * q[2] * q[2]));
         o.hq = val > R(0) ? val : R(0);
       }
       if constexpr ((WHAT & W_HP) != 0) {
@@ -553,9 +557,9 @@ __host__ __device__ inline void tet_eval(const TetGeom<R>& g, R mu, R lam, const
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
-          for (int j = 0; j < 3; ++j) T.m[i][j] = mu * (dF.m[i][j] - Qc.m[i][j]);
+          for (int j = 0; j < 3; ++j) T.m[i][j] = muV * (dF.m[i][j] - Qc.m[i][j]);
 #pragma unroll
-        for (int a = 0; a < 4; ++a) o.hp[a] = g.dV * mul(T, g.d[a]);
+        for (int a = 0; a < 4; ++a) o.hp[a] = mul(T, g.d[a]);
       }
       if constexpr ((WHAT & W_MIX) != 0) {
         o.dmu = g.dV * (S.sm1[0] * dM[0] + S.sm1[1] * dM[1] + S.sm1[2] * dM[2]);
