@@ -38,7 +38,7 @@ static void free_mesh(apl_mesh* m) {
   DevLayout& d = m->d;
   void* ptrs[] = {d.dminv,        d.dV,           d.corner,      d.blk_tet_off,  d.blk_loc_off,
                   d.blk_inst,     d.inst_blk_off, d.loc_gid,     d.blk_co_off,   d.chunk_off,
-                  d.vtx_slot_off, d.vtx_slot_src, d.vblk_off,    d.vblk_inst,    d.inst_vblk_off,
+                  d.vtx_slot_off, d.loc_dst,      d.vblk_off,    d.vblk_inst,    d.inst_vblk_off,
                   d.inst_vtx_off, d.partial,      d.blk_red,     d.vblk_dots};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -151,7 +151,7 @@ int apl_mesh_create(int dtype, int64_t n_tets, int64_t n_verts, const int32_t* c
   rc |= upload(m, &d.blk_co_off, L.blk_co_off.data(), L.blk_co_off.size());
   rc |= upload(m, &d.chunk_off, L.chunk_off.data(), L.chunk_off.size());
   rc |= upload(m, &d.vtx_slot_off, L.vtx_slot_off.data(), L.vtx_slot_off.size());
-  rc |= upload(m, &d.vtx_slot_src, L.vtx_slot_src.data(), L.vtx_slot_src.size());
+  rc |= upload(m, &d.loc_dst, L.loc_dst.data(), L.loc_dst.size());
   rc |= upload(m, &d.vblk_off, L.vblk_off.data(), L.vblk_off.size());
   rc |= upload(m, &d.vblk_inst, L.vblk_inst.data(), L.vblk_inst.size());
   rc |= upload(m, &d.inst_vblk_off, L.inst_vblk_off.data(), L.inst_vblk_off.size());
@@ -168,7 +168,7 @@ int apl_mesh_create(int dtype, int64_t n_tets, int64_t n_verts, const int32_t* c
   L.corner = std::vector<uint16_t>();
   L.chunk_off = std::vector<uint16_t>();
   L.loc_gid = std::vector<int32_t>();
-  L.vtx_slot_src = std::vector<int32_t>();
+  L.loc_dst = std::vector<int32_t>();
   *out = m;
   return 0;
 }
@@ -247,17 +247,16 @@ int apl_elem_eval(apl_mesh* m, int material, const void* mu, const void* lam, co
         if (egh & b) passes.push_back(b);
     for (unsigned pw : passes) {
       if (launch_elem(m, material, pw, mu, lam, u, nullptr, nullptr, nullptr, grad, hess_diag, acc, 0, s)) return -1;
-      if ((pw & (W_G | W_H)) && launch_vertex_reduce(m, pw, grad, hess_diag, acc, nullptr, s)) return -1;
-      if ((pw & W_E) && launch_inst_reduce(m, energy, nullptr, acc, nullptr, s)) return -1;
+      if (launch_finish(m, pw, grad, hess_diag, (pw & W_E) ? energy : nullptr, nullptr, acc, nullptr, s)) return -1;
     }
   }
   if (w & W_HP) {
     if (launch_elem(m, material, W_HP, mu, lam, u, p, nullptr, nullptr, hess_prod, nullptr, acc, 0, s)) return -1;
-    if (launch_vertex_reduce(m, W_HP, hess_prod, nullptr, acc, nullptr, s)) return -1;
+    if (launch_finish(m, W_HP, hess_prod, nullptr, nullptr, nullptr, acc, nullptr, s)) return -1;
   }
   if (w & W_HQ) {
     if (launch_elem(m, material, W_HQ, mu, lam, u, p, nullptr, nullptr, nullptr, nullptr, acc, 0, s)) return -1;
-    if (launch_inst_reduce(m, nullptr, hess_quad, acc, nullptr, s)) return -1;
+    if (launch_finish(m, W_HQ, nullptr, nullptr, nullptr, hess_quad, acc, nullptr, s)) return -1;
   }
   if (w & W_MIX) {
     if (launch_elem(m, material, W_MIX, mu, lam, u, p, nullptr, nullptr, dmu, dlam, acc, 0, s)) return -1;
@@ -287,9 +286,9 @@ int apl_bench_elem_pass(apl_mesh* m, int material, unsigned what, const void* mu
     APL_CHECK_HIP(hipEventRecord(ev[3 * it], s));
     rc = launch_elem(m, material, w, mu, lam, u, p, nullptr, nullptr, out_a, out_b, 0, 0, s);
     APL_CHECK_HIP(hipEventRecord(ev[3 * it + 1], s));
-    if (rc == 0 && (w & (W_G | W_H | W_HP))) rc = launch_vertex_reduce(m, w, out_a, out_b, 0, nullptr, s);
-    if (rc == 0 && (w & (W_E | W_HQ)))
-      rc = launch_inst_reduce(m, (w & W_E) ? energy : nullptr, (w & W_HQ) ? energy : nullptr, 0, nullptr, s);
+    if (rc == 0)
+      rc = launch_finish(m, w, out_a, out_b, (w & W_E) ? energy : nullptr, (w & W_HQ) ? energy : nullptr, 0,
+                         nullptr, s);
     APL_CHECK_HIP(hipEventRecord(ev[3 * it + 2], s));
   }
   if (rc == 0) {

@@ -17,9 +17,9 @@
 //  * each corner's contribution goes to LDS at its position (corners ranked
 //    by slot, then tet, then corner), so the corners of a local vertex form a
 //    contiguous run; one to four lanes sum each run in a fixed order
-//    (bitwise reproducible) and one partial per slot goes into a slot-major
-//    buffer -- a contiguous run per block; the vertex pass adds the partials
-//    of every vertex in block order;
+//    (bitwise reproducible) and one partial per slot goes into a
+//    vertex-major buffer (a vertex's partials contiguous, in block order);
+//    the vertex pass adds each vertex's contiguous run;
 //  * energies and quadratic forms are summed per thread, then in FP64 by a
 //    fixed wave-shuffle tree and a per-instance pass over the block sums;
 //  * the grid is persistent and XCD-aware: workgroup i runs on XCD i % 8, and
@@ -27,6 +27,9 @@
 //    blocks (which share vertices) hit the same L2.
 #include "internal.h"
 #include "tetmath.h"
+#include "vsum.h"
+
+#include <type_traits>
 
 namespace apl {
 
@@ -39,6 +42,7 @@ struct ElemArgs {
   const int32_t* blk_loc_off;
   const int32_t* blk_inst;
   const int32_t* loc_gid;
+  const int32_t* loc_dst;
   const int32_t* blk_co_off;
   const uint16_t* chunk_off;
   const R* mu;
@@ -49,7 +53,7 @@ struct ElemArgs {
   const int32_t* active;
   R* out_a;  // per-tet outputs of the mixed pass
   R* out_b;
-  R* partial;  // slot-major partial sums (scatter passes)
+  R* partial;  // vertex-major partial sums (scatter passes)
   double* blk_red;
   int T;
   int nblocks;
@@ -353,8 +357,8 @@ __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R, X> A) {
     }
 
     if constexpr (SCATTER) {
-      // combine the lanes of a slot, then one partial per slot, consecutive
-      // slots consecutive in memory
+      // combine the lanes of a slot, then one partial per slot, at the
+      // slot's place in the vertex-major order
       if (lg > 0) {
 #pragma unroll
         for (int k = 0; k < SPT; ++k)
@@ -367,14 +371,15 @@ __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R, X> A) {
 #pragma unroll
           for (int ch = 0; ch < NCH; ++ch) acc[k][ch] += __shfl_xor(acc[k][ch], 2, 64);
       }
-      R* dst = A.partial + (size_t)sb * NCH;
+      const int32_t* dsts = A.loc_dst + sb;
 #pragma unroll
       for (int k = 0; k < SPT; ++k) {
         const int item = tid + k * NTH;
         const int s = item >> lg;
         if (s < nloc && (item & (tps - 1)) == 0) {
+          const unsigned j = (unsigned)at(dsts, (unsigned)s) * NCH;
 #pragma unroll
-          for (int ch = 0; ch < NCH; ++ch) at(dst, (unsigned)s * NCH + ch) = acc[k][ch];
+          for (int ch = 0; ch < NCH; ++ch) at(A.partial, j + ch) = acc[k][ch];
         }
       }
     }
@@ -390,88 +395,97 @@ __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R, X> A) {
   }
 }
 
-// Vertex sums of the slot partials, in ascending block order.  One thread
-// per vertex over the instance-aligned vertex blocks.
-template <class R, int NCH>
-__global__ __launch_bounds__(NTH) void vertex_reduce_kernel(const int32_t* vblk_off, const int32_t* vblk_inst,
-                                                            const int32_t* vtx_slot_off,
-                                                            const int32_t* vtx_slot_src, const R* partial,
-                                                            const int32_t* active, R* out_a, R* out_b, int h_only,
-                                                            int accumulate) {
-  const int vb = blockIdx.x;
-  if (active != nullptr && active[vblk_inst[vb]] == 0) return;
-  const int v = vblk_off[vb] + threadIdx.x;
-  if (v >= vblk_off[vb + 1]) return;
-  R acc[NCH];
-#pragma unroll
-  for (int ch = 0; ch < NCH; ++ch) acc[ch] = R(0);
-  const int k1 = at(vtx_slot_off, (unsigned)v + 1);
-  for (int k = at(vtx_slot_off, (unsigned)v); k < k1; ++k) {
-    const unsigned src = (unsigned)at(vtx_slot_src, (unsigned)k) * NCH;
-#pragma unroll
-    for (int ch = 0; ch < NCH; ++ch) acc[ch] += at(partial, src + ch);
-  }
-#pragma unroll
-  for (int grp = 0; grp < NCH / 3; ++grp) {
-    R* out = h_only ? out_b : (grp == 0 ? out_a : out_b);
-    const unsigned o = 3u * (unsigned)v;
-    if (accumulate) {
-      at(out, o) += acc[3 * grp];
-      at(out, o + 1) += acc[3 * grp + 1];
-      at(out, o + 2) += acc[3 * grp + 2];
-    } else {
-      at(out, o) = acc[3 * grp];
-      at(out, o + 1) = acc[3 * grp + 1];
-      at(out, o + 2) = acc[3 * grp + 2];
-    }
-  }
-}
+// Finish pass of an element pass, one launch: workgroups [0, ninst) sum one
+// instance's block energies / quadratic forms in a fixed order (first, so
+// their longer loops start early); workgroups [ninst, ninst + nvblk) form
+// the vertex sums of the slot partials (one thread per vertex over the
+// instance-aligned vertex blocks, ascending block order).
+template <class R>
+struct FinishArgs {
+  const int32_t* vblk_off;
+  const int32_t* vblk_inst;
+  const int32_t* vtx_slot_off;
+  const R* partial;
+  const int32_t* active;
+  R* out_a;
+  R* out_b;
+  const double* blk_red;
+  const int64_t* inst_blk_off;
+  double* out_e;
+  double* out_q;
+  int ninst;  // instance workgroups (0: no energies / quadratic forms)
+  int h_only;
+  int accumulate;
+};
 
-// Per-instance sums of the block energies and quadratic forms, fixed order.
-constexpr int RED_TH = 1024;
-__global__ __launch_bounds__(RED_TH) void inst_reduce_kernel(const double* blk_red, const int64_t* inst_blk_off,
-                                                             const int32_t* active, double* out_e,
-                                                             double* out_q, int accumulate) {
-  __shared__ double red[2 * (RED_TH / 64)];
-  const int i = blockIdx.x;
-  if (active != nullptr && active[i] == 0) return;
-  const long long b0 = inst_blk_off[i], b1 = inst_blk_off[i + 1];
-  const double2* br = reinterpret_cast<const double2*>(blk_red);
-  double e[4] = {0, 0, 0, 0}, q[4] = {0, 0, 0, 0};
-  long long b = b0 + threadIdx.x;
-  for (; b + 3 * RED_TH < b1; b += 4 * RED_TH) {
+template <class R, int NCH>
+__global__ __launch_bounds__(NTH) void finish_kernel(FinishArgs<R> A) {
+  __shared__ double red[2 * (NTH / 64)];
+  if ((int)blockIdx.x < A.ninst) {
+    const int i = (int)blockIdx.x;
+    if (A.active != nullptr && A.active[i] == 0) return;
+    const long long b0 = A.inst_blk_off[i], b1 = A.inst_blk_off[i + 1];
+    const double2* br = reinterpret_cast<const double2*>(A.blk_red);
+    double e[4] = {0, 0, 0, 0}, q[4] = {0, 0, 0, 0};
+    long long b = b0 + threadIdx.x;
+    for (; b + 3 * NTH < b1; b += 4 * NTH) {
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const double2 v = br[b + u * RED_TH];
-      e[u] += v.x;
-      q[u] += v.y;
+      for (int u = 0; u < 4; ++u) {
+        const double2 v = br[b + u * NTH];
+        e[u] += v.x;
+        q[u] += v.y;
+      }
     }
-  }
-  for (int u = 0; b < b1; b += RED_TH, ++u) {
-    const double2 v = br[b];
-    e[u & 3] += v.x;
-    q[u & 3] += v.y;
-  }
-  double ee = (e[0] + e[1]) + (e[2] + e[3]);
-  double qq = (q[0] + q[1]) + (q[2] + q[3]);
+    for (int u = 0; b < b1; b += NTH, ++u) {
+      const double2 v = br[b];
+      e[u & 3] += v.x;
+      q[u & 3] += v.y;
+    }
+    double ee = (e[0] + e[1]) + (e[2] + e[3]);
+    double qq = (q[0] + q[1]) + (q[2] + q[3]);
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    ee += __shfl_down(ee, off, 64);
-    qq += __shfl_down(qq, off, 64);
-  }
-  if ((threadIdx.x & 63) == 0) {
-    red[2 * (threadIdx.x >> 6)] = ee;
-    red[2 * (threadIdx.x >> 6) + 1] = qq;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double se = 0.0, sq = 0.0;
-    for (int w = 0; w < RED_TH / 64; ++w) {
-      se += red[2 * w];
-      sq += red[2 * w + 1];
+    for (int off = 32; off > 0; off >>= 1) {
+      ee += __shfl_down(ee, off, 64);
+      qq += __shfl_down(qq, off, 64);
     }
-    if (out_e != nullptr) out_e[i] = accumulate ? out_e[i] + se : se;
-    if (out_q != nullptr) out_q[i] = accumulate ? out_q[i] + sq : sq;
+    if ((threadIdx.x & 63) == 0) {
+      red[2 * (threadIdx.x >> 6)] = ee;
+      red[2 * (threadIdx.x >> 6) + 1] = qq;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double se = 0.0, sq = 0.0;
+#pragma unroll
+      for (int w = 0; w < NTH / 64; ++w) {
+        se += red[2 * w];
+        sq += red[2 * w + 1];
+      }
+      if (A.out_e != nullptr) A.out_e[i] = A.accumulate ? A.out_e[i] + se : se;
+      if (A.out_q != nullptr) A.out_q[i] = A.accumulate ? A.out_q[i] + sq : sq;
+    }
+    return;
+  }
+  if constexpr (NCH > 0) {
+    const int vb = (int)blockIdx.x - A.ninst;
+    if (A.active != nullptr && A.active[A.vblk_inst[vb]] == 0) return;
+    const int v = A.vblk_off[vb] + threadIdx.x;
+    if (v >= A.vblk_off[vb + 1]) return;
+    R acc[NCH];
+    vertex_partial_sum<R, NCH>(A.vtx_slot_off, A.partial, v, acc);
+#pragma unroll
+    for (int grp = 0; grp < NCH / 3; ++grp) {
+      R* out = A.h_only ? A.out_b : (grp == 0 ? A.out_a : A.out_b);
+      const unsigned o = 3u * (unsigned)v;
+      if (A.accumulate) {
+        at(out, o) += acc[3 * grp];
+        at(out, o + 1) += acc[3 * grp + 1];
+        at(out, o + 2) += acc[3 * grp + 2];
+      } else {
+        at(out, o) = acc[3 * grp];
+        at(out, o + 1) = acc[3 * grp + 1];
+        at(out, o + 2) = acc[3 * grp + 2];
+      }
+    }
   }
 }
 
@@ -489,6 +503,7 @@ static ElemArgs<R, X> make_args(apl_mesh* m, const void* mu, const void* lam, co
   A.blk_loc_off = m->d.blk_loc_off;
   A.blk_inst = m->d.blk_inst;
   A.loc_gid = m->d.loc_gid;
+  A.loc_dst = m->d.loc_dst;
   A.blk_co_off = m->d.blk_co_off;
   A.chunk_off = m->d.chunk_off;
   A.mu = (const R*)mu;
@@ -640,35 +655,44 @@ int elem_grid(apl_mesh* m, int material, unsigned what, int x_f64) {
   return material == MAT_SNH ? grid_mat<MAT_SNH>(m, what, x_f64) : grid_mat<MAT_ARAP>(m, what, x_f64);
 }
 
-template <class R, int NCH>
-static int vertex_one(apl_mesh* m, void* out_a, void* out_b, int h_only, int accumulate, const int32_t* active,
-                      hipStream_t s) {
-  const unsigned nvb = (unsigned)m->L.vblk_inst.size();
-  if (nvb == 0) return 0;
-  hipLaunchKernelGGL((vertex_reduce_kernel<R, NCH>), dim3(nvb), dim3(NTH), 0, s, m->d.vblk_off, m->d.vblk_inst,
-                     m->d.vtx_slot_off, m->d.vtx_slot_src, (const R*)m->d.partial, active, (R*)out_a, (R*)out_b,
-                     h_only, accumulate);
-  APL_CHECK_HIP(hipGetLastError());
-  return 0;
-}
-
-int launch_vertex_reduce(apl_mesh* m, unsigned what, void* out_a, void* out_b, int accumulate,
-                         const int32_t* active, hipStream_t s) {
+int launch_finish(apl_mesh* m, unsigned what, void* out_a, void* out_b, double* out_e, double* out_q,
+                  int accumulate, const int32_t* active, hipStream_t s) {
   const bool g = what & W_G, h = what & W_H, hp = what & W_HP;
   const int nch = (g ? 3 : 0) + (h ? 3 : 0) + (hp ? 3 : 0);
-  const int h_only = (h && !g && !hp) ? 1 : 0;
-  if (nch == 0) return 0;
-  if (m->dtype == DT_F32)
-    return nch == 6 ? vertex_one<float, 6>(m, out_a, out_b, h_only, accumulate, active, s)
-                    : vertex_one<float, 3>(m, out_a, out_b, h_only, accumulate, active, s);
-  return nch == 6 ? vertex_one<double, 6>(m, out_a, out_b, h_only, accumulate, active, s)
-                  : vertex_one<double, 3>(m, out_a, out_b, h_only, accumulate, active, s);
-}
-
-int launch_inst_reduce(apl_mesh* m, double* out_e, double* out_q, int accumulate, const int32_t* active,
-                       hipStream_t s) {
-  hipLaunchKernelGGL(inst_reduce_kernel, dim3((unsigned)m->L.n_inst), dim3(RED_TH), 0, s, m->d.blk_red,
-                     m->d.inst_blk_off, active, out_e, out_q, accumulate);
+  const bool inst = out_e != nullptr || out_q != nullptr;
+  const int nvblk = nch > 0 ? (int)m->L.vblk_inst.size() : 0;
+  const int ninst = inst ? (int)m->L.n_inst : 0;
+  const int grid = nvblk + ninst;
+  if (grid == 0) return 0;
+  auto go = [&](auto* tag, auto nch_c) {
+    using R = std::remove_pointer_t<decltype(tag)>;
+    FinishArgs<R> A;
+    A.vblk_off = m->d.vblk_off;
+    A.vblk_inst = m->d.vblk_inst;
+    A.vtx_slot_off = m->d.vtx_slot_off;
+    A.partial = (const R*)m->d.partial;
+    A.active = active;
+    A.out_a = (R*)out_a;
+    A.out_b = (R*)out_b;
+    A.blk_red = m->d.blk_red;
+    A.inst_blk_off = m->d.inst_blk_off;
+    A.out_e = out_e;
+    A.out_q = out_q;
+    A.ninst = ninst;
+    A.h_only = (h && !g && !hp) ? 1 : 0;
+    A.accumulate = accumulate;
+    hipLaunchKernelGGL((finish_kernel<R, decltype(nch_c)::value>), dim3((unsigned)grid), dim3(NTH), 0, s, A);
+  };
+  using I0 = std::integral_constant<int, 0>;
+  using I3 = std::integral_constant<int, 3>;
+  using I6 = std::integral_constant<int, 6>;
+  if (m->dtype == DT_F32) {
+    float* t = nullptr;
+    if (nch == 6) go(t, I6{}); else if (nch == 3) go(t, I3{}); else go(t, I0{});
+  } else {
+    double* t = nullptr;
+    if (nch == 6) go(t, I6{}); else if (nch == 3) go(t, I3{}); else go(t, I0{});
+  }
   APL_CHECK_HIP(hipGetLastError());
   return 0;
 }

@@ -40,13 +40,13 @@ struct DevLayout {
   int32_t* blk_co_off = nullptr;   // nblocks+1 into chunk_off
   uint16_t* chunk_off = nullptr;   // per block and chunk: nloc+1 slot offsets into positions
   int32_t* vtx_slot_off = nullptr;  // V+1: slot range of every vertex
-  int32_t* vtx_slot_src = nullptr;  // n_slots: slot ids, ascending per vertex
+  int32_t* loc_dst = nullptr;       // n_slots: vertex-major partial index of every slot
   int32_t* vblk_off = nullptr;
   int32_t* vblk_inst = nullptr;
   int64_t* inst_vblk_off = nullptr;
   int64_t* inst_vtx_off = nullptr;
   // workspace
-  void* partial = nullptr;   // n_slots x MAX_CH, slot-major (one partial sum per local slot)
+  void* partial = nullptr;   // n_slots x MAX_CH, vertex-major (one partial sum per local slot)
   double* blk_red = nullptr; // 2 * nblocks
   double* vblk_dots = nullptr;  // N_DOTS * nvblk
 };
@@ -74,11 +74,12 @@ int launch_elem(apl_mesh* m, int material, unsigned what, const void* mu, const 
                 const void* x, const void* p, const double* alpha, const int32_t* active,
                 void* out_a, void* out_b, int accumulate, int grid_override, hipStream_t s, int x_f64 = 0);
 // vertex sums of the slot partials into the (V,3) outputs
-int launch_vertex_reduce(apl_mesh* m, unsigned what, void* out_a, void* out_b, int accumulate,
-                         const int32_t* active, hipStream_t s);
-// per-instance sums of the block energies / quadratic forms
-int launch_inst_reduce(apl_mesh* m, double* out_e, double* out_q, int accumulate,
-                       const int32_t* active, hipStream_t s);
+// Finish pass of an element pass in one launch: vertex sums of the slot
+// partials into out_a / out_b (what: the scatter channels of the pass) and,
+// when out_e / out_q are given, per-instance sums of the block energies /
+// quadratic forms.
+int launch_finish(apl_mesh* m, unsigned what, void* out_a, void* out_b, double* out_e, double* out_q,
+                  int accumulate, const int32_t* active, hipStream_t s);
 int elem_grid(apl_mesh* m, int material, unsigned what, int x_f64 = 0);
 
 }  // namespace apl

@@ -116,13 +116,14 @@ std::string build_layout(const int32_t* cells, int64_t T, int64_t V, int64_t n_i
   L.n_slots = (int64_t)L.loc_gid.size();
 
   // 3. vertex CSR over slots: a vertex's slots in ascending slot id, i.e. in
-  //    block order; vertices no tet touches get empty lists (their sums are 0)
+  //    block order, each slot's partial at its place in that order; vertices
+  //    no tet touches get empty ranges (their sums are 0)
   L.vtx_slot_off.assign(V + 1, 0);
   for (int64_t v = 0; v < V; ++v) L.vtx_slot_off[v + 1] = L.vtx_slot_off[v] + nslot_of_v[v + 1];
-  L.vtx_slot_src.assign(L.n_slots, 0);
+  L.loc_dst.assign(L.n_slots, 0);
   {
     std::vector<int32_t> fill(L.vtx_slot_off.begin(), L.vtx_slot_off.end() - 1);
-    for (int64_t k = 0; k < L.n_slots; ++k) L.vtx_slot_src[fill[L.loc_gid[k]]++] = (int32_t)k;
+    for (int64_t k = 0; k < L.n_slots; ++k) L.loc_dst[k] = fill[L.loc_gid[k]]++;
   }
   L.n_shared_vtx = 0;
   for (int64_t v = 0; v < V; ++v) L.n_shared_vtx += nslot_of_v[v + 1] > 1;

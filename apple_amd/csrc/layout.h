@@ -7,9 +7,9 @@
 // "position", and the corners of one slot occupy a contiguous position range
 // (the chunk's slot offsets).  A thread writes each corner's contribution at
 // its position in LDS; the slot sums then read contiguous runs.  Each block
-// writes one partial sum per slot into a slot-major buffer (a contiguous run
-// per block), and a vertex-wise pass adds, for every vertex, the partials of
-// the slots that refer to it in ascending block order (the vertex CSR below).
+// writes one partial sum per slot into a vertex-major buffer: the partials
+// of one vertex are contiguous, in ascending block order (the vertex CSR
+// below), so the vertex-wise pass reads one contiguous run per vertex.
 // Every sum runs in a fixed order, so every result is bitwise reproducible --
 // this replaces the float atomics of
 // warp/fem/_base.py:261,288-289,321-322,348-349,381.
@@ -39,8 +39,8 @@ struct Layout {
   std::vector<int32_t> blk_co_off;    // nblocks+1 into chunk_off
   std::vector<uint16_t> chunk_off;    // per block and chunk: nloc+1 slot offsets into positions
   std::vector<uint16_t> corner;       // 8*T: per tet, the local slots of its 4 corners, then their positions
-  std::vector<int32_t> vtx_slot_off;  // V+1 into vtx_slot_src
-  std::vector<int32_t> vtx_slot_src;  // n_slots: slot ids per vertex, ascending (= block order)
+  std::vector<int32_t> vtx_slot_off;  // V+1: partial range of every vertex
+  std::vector<int32_t> loc_dst;       // n_slots: vertex-major partial index of every slot
   // vertex blocks for the vector kernels: aligned to instance ranges
   int vb = 256;
   std::vector<int32_t> vblk_off;      // nvblk+1

@@ -31,6 +31,7 @@
 
 #include "../../include/apple_amd.h"
 #include "internal.h"
+#include "vsum.h"
 #include "tetmath.h"
 
 namespace apl {
@@ -74,7 +75,6 @@ struct VecArgs {
   const int32_t* vblk_off;
   const int32_t* vblk_inst;
   const int32_t* vtx_slot_off;
-  const int32_t* vtx_slot_src;
   const R* partial;
   const uint8_t* mask;
   const R* fext;
@@ -138,16 +138,9 @@ __global__ __launch_bounds__(NTH) void pncg_reduce_kernel(VecArgs<R> A) {
   for (int k = 0; k < N_DOTS; ++k) d[k] = 0.0;
   if (v < v1) {
     // vertex sums of the slot partials (ascending block order)
-    R ge[3] = {R(0), R(0), R(0)}, h[3] = {R(0), R(0), R(0)};
-    const int o1 = A.vtx_slot_off[v + 1];
-    for (int k = A.vtx_slot_off[v]; k < o1; ++k) {
-      const R* src = A.partial + (unsigned)A.vtx_slot_src[k] * 6;
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        ge[c] += src[c];
-        h[c] += src[3 + c];
-      }
-    }
+    R gh[6];
+    vertex_partial_sum<R, 6>(A.vtx_slot_off, A.partial, v, gh);
+    const R ge[3] = {gh[0], gh[1], gh[2]}, h[3] = {gh[3], gh[4], gh[5]};
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       A.gE[3 * v + c] = ge[c];
@@ -362,7 +355,6 @@ struct CgArgs {
   const int32_t* vblk_off;
   const int32_t* vblk_inst;
   const int32_t* vtx_slot_off;
-  const int32_t* vtx_slot_src;
   const R* partial;
   const uint8_t* mask;
   const R* rhs;
@@ -393,13 +385,8 @@ __global__ __launch_bounds__(NTH) void cg_vec_kernel(CgArgs<R> A) {
   if (in) {
     if constexpr (STAGE == CG_DIAG || STAGE == CG_Q) {
       R* out = STAGE == CG_DIAG ? A.hd : A.q;
-      R s3[3] = {R(0), R(0), R(0)};
-      const int o1 = A.vtx_slot_off[v + 1];
-      for (int k = A.vtx_slot_off[v]; k < o1; ++k) {
-        const R* src = A.partial + (unsigned)A.vtx_slot_src[k] * 3;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) s3[c] += src[c];
-      }
+      R s3[3];
+      vertex_partial_sum<R, 3>(A.vtx_slot_off, A.partial, v, s3);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         const bool fixed = (msk & (1u << c)) != 0;
@@ -541,7 +528,6 @@ static VecArgs<R> vec_args(apl_solver* s, double* x, const void* fext, const uin
   A.vblk_off = m->d.vblk_off;
   A.vblk_inst = m->d.vblk_inst;
   A.vtx_slot_off = m->d.vtx_slot_off;
-  A.vtx_slot_src = m->d.vtx_slot_src;
   A.partial = (const R*)m->d.partial;
   A.mask = mask;
   A.fext = (const R*)fext;
@@ -719,7 +705,6 @@ static CgArgs<R> cg_args(apl_solver* s, const void* rhs, void* y, const uint8_t*
   A.vblk_off = m->d.vblk_off;
   A.vblk_inst = m->d.vblk_inst;
   A.vtx_slot_off = m->d.vtx_slot_off;
-  A.vtx_slot_src = m->d.vtx_slot_src;
   A.partial = (const R*)m->d.partial;
   A.mask = mask;
   A.rhs = (const R*)rhs;

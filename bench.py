@@ -37,7 +37,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_cube_problem(n, material, dtype, device, seed_mu=1, seed_noise=0, noise_frac=0.1):
+def make_cube_problem(n, material, dtype, device, seed_mu=1, seed_noise=0, noise_frac=0.1, chunks=1):
     """Config-2 style cube: lognormal mu, lambda = 10 mu, gravity, z = 0 pinned, noisy start."""
     import torch
 
@@ -51,7 +51,7 @@ def make_cube_problem(n, material, dtype, device, seed_mu=1, seed_noise=0, noise
     m.cell_data["lambda"] = 10.0 * mu
     region = Region.from_mesh(m)
     cls = fem.StableNeoHookean if material == "snh" else fem.Arap
-    pot = cls.from_region(region, dtype=dtype, device=device)
+    pot = cls.from_region(region, dtype=dtype, device=device, chunks=chunks)
     f = np.zeros((m.n_points, 3))
     np.add.at(f[:, 2], m.cells.reshape(-1), np.repeat(-9.8 * 1e3 * region.dV / 4.0, 4))
     fixed = np.zeros((m.n_points, 3), bool)
@@ -136,6 +136,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--chunks", type=int, default=1, help="tet chunks of 256 per element block")
     ap.add_argument("--profile-pass", action="store_true",
                     help="only run pass-A launches and a reference copy (for rocprofv3 counter runs)")
     args = ap.parse_args()
@@ -153,7 +154,7 @@ def main():
     dtype = torch.float32
     n = args.n
     log(f"[bench] building {n}^3 x 5 cube (SNH, FP32)")
-    prob = make_cube_problem(n, "snh", dtype, dev)
+    prob = make_cube_problem(n, "snh", dtype, dev, chunks=args.chunks)
     pot = prob["pot"]
     T, V = pot.mesh.n_tets, pot.mesh.n_verts
     info = pot.mesh.info()
