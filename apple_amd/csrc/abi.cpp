@@ -35,14 +35,19 @@ static int alloc_bytes(apl_mesh* m, void** dst, size_t n) {
 }
 
 static void free_mesh(apl_mesh* m) {
+  drop_hip_error(hipSetDevice(m->device));
   DevLayout& d = m->d;
   void* ptrs[] = {d.dminv,        d.dV,           d.corner,      d.blk_tet_off,  d.blk_loc_off,
                   d.blk_inst,     d.inst_blk_off, d.loc_gid,     d.blk_co_off,   d.chunk_off,
                   d.vtx_slot_off, d.loc_dst,      d.vblk_off,    d.vblk_inst,    d.inst_vblk_off,
                   d.inst_vtx_off, d.partial,      d.blk_red,     d.vblk_dots};
   for (void* p : ptrs)
-    if (p) (void)hipFree(p);
+    if (p) drop_hip_error(hipFree(p));
   delete m;
+}
+
+void mesh_release(apl_mesh* m) {
+  if (--m->refs == 0) free_mesh(m);
 }
 
 static unsigned to_internal(unsigned what) {
@@ -175,8 +180,7 @@ int apl_mesh_create(int dtype, int64_t n_tets, int64_t n_verts, const int32_t* c
 
 int apl_mesh_destroy(apl_mesh* mesh) {
   if (mesh == nullptr) return 0;
-  (void)hipSetDevice(mesh->device);
-  free_mesh(mesh);
+  mesh_release(mesh);
   return 0;
 }
 
@@ -186,7 +190,7 @@ int apl_mesh_info(const apl_mesh* mesh, apl_mesh_info_t* info) {
     return -1;
   }
   apl_mesh* m = const_cast<apl_mesh*>(mesh);
-  (void)hipSetDevice(m->device);
+  drop_hip_error(hipSetDevice(m->device));
   const Layout& L = m->L;
   info->n_tets = L.T;
   info->n_verts = L.V;
@@ -304,7 +308,8 @@ int apl_bench_elem_pass(apl_mesh* m, int material, unsigned what, const void* mu
     *ms_kernel = k / iters;
     *ms_pass = t / iters;
   }
-  for (auto& e : ev) (void)hipEventDestroy(e);
+  for (auto& e : ev)
+    if (e) drop_hip_error(hipEventDestroy(e));
   return rc;
 }
 

@@ -27,6 +27,7 @@
 //    blocks (which share vertices) hit the same L2.
 #include "internal.h"
 #include "tetmath.h"
+#include "flatsum.h"
 #include "vsum.h"
 
 #include <type_traits>
@@ -398,8 +399,9 @@ __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R, X> A) {
 // Finish pass of an element pass, one launch: workgroups [0, ninst) sum one
 // instance's block energies / quadratic forms in a fixed order (first, so
 // their longer loops start early); workgroups [ninst, ninst + nvblk) form
-// the vertex sums of the slot partials (one thread per vertex over the
-// instance-aligned vertex blocks, ascending block order).
+// the vertex sums of the slot partials over an instance-aligned vertex
+// block, one degree of freedom per thread and stride (ascending block order
+// per vertex).
 template <class R>
 struct FinishArgs {
   const int32_t* vblk_off;
@@ -420,70 +422,50 @@ struct FinishArgs {
 
 template <class R, int NCH>
 __global__ __launch_bounds__(NTH) void finish_kernel(FinishArgs<R> A) {
-  __shared__ double red[2 * (NTH / 64)];
+  __shared__ double red[2 * (NTH / 64 + 1)];
   if ((int)blockIdx.x < A.ninst) {
     const int i = (int)blockIdx.x;
     if (A.active != nullptr && A.active[i] == 0) return;
-    const long long b0 = A.inst_blk_off[i], b1 = A.inst_blk_off[i + 1];
-    const double2* br = reinterpret_cast<const double2*>(A.blk_red);
-    double e[4] = {0, 0, 0, 0}, q[4] = {0, 0, 0, 0};
-    long long b = b0 + threadIdx.x;
-    for (; b + 3 * NTH < b1; b += 4 * NTH) {
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const double2 v = br[b + u * NTH];
-        e[u] += v.x;
-        q[u] += v.y;
-      }
-    }
-    for (int u = 0; b < b1; b += NTH, ++u) {
-      const double2 v = br[b];
-      e[u & 3] += v.x;
-      q[u & 3] += v.y;
-    }
-    double ee = (e[0] + e[1]) + (e[2] + e[3]);
-    double qq = (q[0] + q[1]) + (q[2] + q[3]);
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      ee += __shfl_down(ee, off, 64);
-      qq += __shfl_down(qq, off, 64);
-    }
-    if ((threadIdx.x & 63) == 0) {
-      red[2 * (threadIdx.x >> 6)] = ee;
-      red[2 * (threadIdx.x >> 6) + 1] = qq;
-    }
-    __syncthreads();
+    const long long b0 = A.inst_blk_off[i];
+    double eq[2];
+    flat_column_sums<2, NTH>(A.blk_red + 2 * b0, 2 * (A.inst_blk_off[i + 1] - b0), red, eq);
     if (threadIdx.x == 0) {
-      double se = 0.0, sq = 0.0;
-#pragma unroll
-      for (int w = 0; w < NTH / 64; ++w) {
-        se += red[2 * w];
-        sq += red[2 * w + 1];
-      }
-      if (A.out_e != nullptr) A.out_e[i] = A.accumulate ? A.out_e[i] + se : se;
-      if (A.out_q != nullptr) A.out_q[i] = A.accumulate ? A.out_q[i] + sq : sq;
+      if (A.out_e != nullptr) A.out_e[i] = A.accumulate ? A.out_e[i] + eq[0] : eq[0];
+      if (A.out_q != nullptr) A.out_q[i] = A.accumulate ? A.out_q[i] + eq[1] : eq[1];
     }
     return;
   }
   if constexpr (NCH > 0) {
     const int vb = (int)blockIdx.x - A.ninst;
     if (A.active != nullptr && A.active[A.vblk_inst[vb]] == 0) return;
-    const int v = A.vblk_off[vb] + threadIdx.x;
-    if (v >= A.vblk_off[vb + 1]) return;
-    R acc[NCH];
-    vertex_partial_sum<R, NCH>(A.vtx_slot_off, A.partial, v, acc);
+    __shared__ R win[VSUM_LDS_BYTES / sizeof(R)];
+    const int v0 = A.vblk_off[vb], v1 = A.vblk_off[vb + 1];
+    const int i0 = 3 * v0, i1 = 3 * v1;
+    int k0[DOF_PER_THREAD], k1[DOF_PER_THREAD];
 #pragma unroll
-    for (int grp = 0; grp < NCH / 3; ++grp) {
-      R* out = A.h_only ? A.out_b : (grp == 0 ? A.out_a : A.out_b);
-      const unsigned o = 3u * (unsigned)v;
-      if (A.accumulate) {
-        at(out, o) += acc[3 * grp];
-        at(out, o + 1) += acc[3 * grp + 1];
-        at(out, o + 2) += acc[3 * grp + 2];
-      } else {
-        at(out, o) = acc[3 * grp];
-        at(out, o + 1) = acc[3 * grp + 1];
-        at(out, o + 2) = acc[3 * grp + 2];
+    for (int j = 0; j < DOF_PER_THREAD; ++j) {
+      const int i = i0 + (int)threadIdx.x + j * NTH;
+      k0[j] = k1[j] = 0;
+      if (i < i1) {
+        k0[j] = at(A.vtx_slot_off, (unsigned)(i / 3));
+        k1[j] = at(A.vtx_slot_off, (unsigned)(i / 3) + 1);
+      }
+    }
+    VertexSums<R, NCH, NTH> vs{A.vtx_slot_off, A.partial, win};
+    vs.stage(v0, v1, VSUM_LDS_BYTES / (int)sizeof(R));
+#pragma unroll
+    for (int j = 0; j < DOF_PER_THREAD; ++j) {
+      const int i = i0 + (int)threadIdx.x + j * NTH;
+      if (i >= i1) continue;
+      R acc[NCH / 3];
+      vs.sum_dof(k0[j], k1[j], i - 3 * (i / 3), acc);
+#pragma unroll
+      for (int grp = 0; grp < NCH / 3; ++grp) {
+        R* out = A.h_only ? A.out_b : (grp == 0 ? A.out_a : A.out_b);
+        if (A.accumulate)
+          at(out, (unsigned)i) += acc[grp];
+        else
+          at(out, (unsigned)i) = acc[grp];
       }
     }
   }

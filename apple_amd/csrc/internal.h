@@ -24,6 +24,7 @@ void set_error(const std::string& s);
     hipError_t e_ = (expr);                                                    \
     if (e_ != hipSuccess) {                                                    \
       ::apl::set_error(std::string(#expr) + ": " + hipGetErrorString(e_));      \
+      (void)hipGetLastError(); /* reported through apl_last_error instead */   \
       return -1;                                                               \
     }                                                                          \
   } while (0)
@@ -61,9 +62,20 @@ struct apl_mesh {
   apl::DevLayout d;
   size_t device_bytes = 0;
   int grid_cache[2][128] = {};  // persistent grid per (material, pass variant, FP64 displacements)
+  int refs = 1;                 // the caller's handle plus one per solver built on the mesh
 };
 
 namespace apl {
+
+// Drop one reference; the last one frees the mesh.  A solver keeps its mesh
+// alive, so handles may be destroyed in any order (garbage collectors
+// finalise cycles in no particular order).
+void mesh_release(apl_mesh* m);
+
+// Clear the calling thread's HIP error state after a call whose failure is
+// deliberately ignored, so that the next unrelated caller of
+// hipGetLastError() (e.g. PyTorch after its own launches) does not see it.
+inline void drop_hip_error(hipError_t) { (void)hipGetLastError(); }
 
 inline size_t real_size(const apl_mesh* m) { return m->dtype == DT_F64 ? 8 : 4; }
 

@@ -12,6 +12,7 @@ std::string build_layout(const int32_t* cells, int64_t T, int64_t V, int64_t n_i
   char msg[256];
   if (T < 0 || V < 0 || n_inst < 1) return "invalid mesh sizes";
   if (nth != 256) return "threads per block must be 256";
+  if (L.vb != nth) return "vertex blocks must hold one vertex per thread";
   if (chunks < 1 || (int64_t)nth * chunks > 8192) return "chunks must be in [1, 32]";
   if (nloc_cap < 4 || nloc_cap > 1024) return "nloc_cap must be in [4, 1024]";
   // 32-bit element offsets: 9 T and 3 V (8-byte values: 24 V bytes) must fit

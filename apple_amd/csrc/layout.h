@@ -41,7 +41,8 @@ struct Layout {
   std::vector<uint16_t> corner;       // 8*T: per tet, the local slots of its 4 corners, then their positions
   std::vector<int32_t> vtx_slot_off;  // V+1: partial range of every vertex
   std::vector<int32_t> loc_dst;       // n_slots: vertex-major partial index of every slot
-  // vertex blocks for the vector kernels: aligned to instance ranges
+  // vertex blocks for the vector kernels: aligned to instance ranges, one
+  // vertex per thread of a 256-thread workgroup
   int vb = 256;
   std::vector<int32_t> vblk_off;      // nvblk+1
   std::vector<int32_t> vblk_inst;     // nvblk

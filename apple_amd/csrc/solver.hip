@@ -31,12 +31,14 @@
 
 #include "../../include/apple_amd.h"
 #include "internal.h"
+#include "flatsum.h"
 #include "vsum.h"
 #include "tetmath.h"
 
 namespace apl {
 
-enum : int { D_G2 = 0, D_GPG, D_GPG0, D_GPGC, D_GPGC0, D_GCPGC, D_GCPGC0, D_GTP, D_GCP, D_HSUM, D_HCNT, D_EXT, D_PP };
+enum : int { D_G2 = 0, D_GPG, D_GPG0, D_GPGC, D_GPGC0, D_GCPGC, D_GCPGC0, D_GTP, D_GCP, D_HSUM, D_HCNT, D_EXT, D_PP, D_E };
+static_assert(D_E < N_DOTS, "dot table too small");
 static_assert(D_PP < N_DOTS, "dot slots");
 
 struct StatePtrs {
@@ -50,24 +52,48 @@ struct Params {
   int max_steps, max_backtracking;
 };
 
-template <int N>
-__device__ inline void block_reduce_n(double (&v)[N], double* sm, double* out) {
-#pragma unroll
-  for (int k = 0; k < N; ++k)
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
+// Workgroup sums of the 16 per-thread dots, in a fixed order, to out[0..16).
+// Recursive halving across the wave (offsets 32, 16, 8, 4: each lane keeps
+// half of its entries and adds its partner's copy of them) leaves dot
+// lane / 4 in lanes 4j..4j+3; two more exchanges complete those sums, and
+// the waves are added in order.  17 shuffles instead of 16 x 6.
+__device__ inline void block_dots(double (&v)[N_DOTS], double* sm, double* out) {
+  static_assert(N_DOTS == 16, "halving tree is written for 16 dots");
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (lane == 0)
 #pragma unroll
-    for (int k = 0; k < N; ++k) sm[w * N + k] = v[k];
+  for (int half = 8, off = 32; half >= 1; half >>= 1, off >>= 1) {
+    const bool up = (lane & off) != 0;
+#pragma unroll
+    for (int k = 0; k < half; ++k) {
+      const double send = up ? v[k] : v[k + half];
+      const double keep = up ? v[k + half] : v[k];
+      v[k] = keep + __shfl_xor(send, off, 64);
+    }
+  }
+  v[0] += __shfl_xor(v[0], 2, 64);
+  v[0] += __shfl_xor(v[0], 1, 64);
+  if ((lane & 3) == 0) sm[w * N_DOTS + (lane >> 2)] = v[0];
   __syncthreads();
-  if ((int)threadIdx.x < N) {
+  if ((int)threadIdx.x < N_DOTS) {
     double s = 0.0;
 #pragma unroll
-    for (int ww = 0; ww < NTH / 64; ++ww) s += sm[ww * N + threadIdx.x];
+    for (int ww = 0; ww < NTH / 64; ++ww) s += sm[ww * N_DOTS + threadIdx.x];
     out[threadIdx.x] = s;
   }
-  __syncthreads();
+}
+
+// This vertex block's share of its instance's element-block energies: the
+// instance's blocks split into equal consecutive runs, one per vertex block
+// (both counted from the instance's start, so a batch sums the same way as
+// the instance alone).
+__device__ inline void energy_share(const double* blk_red, const int64_t* inst_blk_off,
+                                    const int64_t* inst_vblk_off, int inst, int vb, double& acc) {
+  const long long b0 = inst_blk_off[inst], nb = inst_blk_off[inst + 1] - b0;
+  const long long vb0 = inst_vblk_off[inst], nvb = inst_vblk_off[inst + 1] - vb0;
+  const long long per = (nb + nvb - 1) / nvb;
+  const long long lo = b0 + (vb - vb0) * per;
+  const long long hi = lo + per < b0 + nb ? lo + per : b0 + nb;
+  for (long long b = lo + threadIdx.x; b < hi; b += NTH) acc += blk_red[2 * b];
 }
 
 template <class R>
@@ -84,6 +110,9 @@ struct VecArgs {
   R* gE;
   R* hd;
   double* vblk_dots;
+  const double* blk_red;          // per element block: energy, quadratic form
+  const int64_t* inst_blk_off;
+  const int64_t* inst_vblk_off;
   StatePtrs st;
 };
 
@@ -123,78 +152,93 @@ __global__ void pncg_init_kernel(StatePtrs st, int n) {
   st.restarts[i] = 0;
 }
 
-// R: shared-vertex sums + FP64 partial dots of one vertex block
+// R: vertex sums of the slot partials + FP64 partial dots of one vertex
+// block (one DOF per thread and stride, coalesced), plus the block's share of
+// the instance's element-block energies
 template <class R>
 __global__ __launch_bounds__(NTH) void pncg_reduce_kernel(VecArgs<R> A) {
   __shared__ double sm[(NTH / 64) * N_DOTS];
+  __shared__ R win[VSUM_LDS_BYTES / sizeof(R)];
   const int vb = blockIdx.x;
   const int inst = A.vblk_inst[vb];
   if (A.st.act_eval[inst] == 0) return;
   const int v0 = A.vblk_off[vb], v1 = A.vblk_off[vb + 1];
-  const long long v = (long long)v0 + threadIdx.x;
+  const int i0 = 3 * v0, i1 = 3 * v1;
   const double alpha = A.st.alpha[inst];
+  // per-DOF inputs, loaded before the partial window so the loads overlap
+  int k0[DOF_PER_THREAD], k1[DOF_PER_THREAD];
+  double xv[DOF_PER_THREAD];
+  R pv[DOF_PER_THREAD], gcv[DOF_PER_THREAD], fv[DOF_PER_THREAD];
+  bool fixed[DOF_PER_THREAD];
+#pragma unroll
+  for (int j = 0; j < DOF_PER_THREAD; ++j) {
+    const int i = i0 + (int)threadIdx.x + j * NTH;
+    k0[j] = k1[j] = 0;
+    xv[j] = 0.0;
+    pv[j] = gcv[j] = fv[j] = R(0);
+    fixed[j] = true;
+    if (i < i1) {
+      const int v = i / 3, c = i - 3 * v;
+      k0[j] = A.vtx_slot_off[v];
+      k1[j] = A.vtx_slot_off[v + 1];
+      xv[j] = A.x[i];
+      pv[j] = A.p[i];
+      gcv[j] = A.gc[i];
+      fv[j] = A.fext != nullptr ? A.fext[i] : R(0);
+      fixed[j] = A.mask != nullptr && ((A.mask[v] >> c) & 1u) != 0;
+    }
+  }
+  VertexSums<R, 6, NTH> vs{A.vtx_slot_off, A.partial, win};
+  vs.stage(v0, v1, VSUM_LDS_BYTES / (int)sizeof(R));
   double d[N_DOTS];
 #pragma unroll
   for (int k = 0; k < N_DOTS; ++k) d[k] = 0.0;
-  if (v < v1) {
-    // vertex sums of the slot partials (ascending block order)
-    R gh[6];
-    vertex_partial_sum<R, 6>(A.vtx_slot_off, A.partial, v, gh);
-    const R ge[3] = {gh[0], gh[1], gh[2]}, h[3] = {gh[3], gh[4], gh[5]};
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      A.gE[3 * v + c] = ge[c];
-      A.hd[3 * v + c] = h[c];
+  for (int j = 0; j < DOF_PER_THREAD; ++j) {
+    const int i = i0 + (int)threadIdx.x + j * NTH;
+    if (i >= i1) continue;
+    R gh[2];
+    vs.sum_dof(k0[j], k1[j], i - 3 * (i / 3), gh);
+    A.gE[i] = gh[0];
+    A.hd[i] = gh[1];
+    const double pc = (double)pv[j];
+    d[D_EXT] -= (double)fv[j] * (xv[j] + alpha * pc);
+    if (fixed[j]) continue;
+    const double gt = (double)(gh[0] - fv[j]);
+    const double gc = (double)gcv[j];
+    const R h = gh[1];
+    d[D_G2] += gt * gt;
+    if (h > R(0)) {
+      const double P = (double)(R(1) / h);  // the preconditioner the direction update uses
+      d[D_GPG] += gt * gt * P;
+      d[D_GPGC] += gt * gc * P;
+      d[D_GCPGC] += gc * gc * P;
+      d[D_HSUM] += (double)h;
+      d[D_HCNT] += 1.0;
+    } else {
+      d[D_GPG0] += gt * gt;
+      d[D_GPGC0] += gt * gc;
+      d[D_GCPGC0] += gc * gc;
     }
-    const unsigned msk = A.mask != nullptr ? A.mask[v] : 0u;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const double f = A.fext != nullptr ? (double)A.fext[3 * v + c] : 0.0;
-      const double pc = (double)A.p[3 * v + c];
-      d[D_EXT] -= f * (A.x[3 * v + c] + alpha * pc);
-      if (msk & (1u << c)) continue;
-      const double gt = (double)(ge[c] - (A.fext != nullptr ? A.fext[3 * v + c] : R(0)));
-      const double gcv = (double)A.gc[3 * v + c];
-      const double hh = (double)h[c];
-      d[D_G2] += gt * gt;
-      if (hh > 0.0) {
-        const double P = 1.0 / hh;
-        d[D_GPG] += gt * gt * P;
-        d[D_GPGC] += gt * gcv * P;
-        d[D_GCPGC] += gcv * gcv * P;
-        d[D_HSUM] += hh;
-        d[D_HCNT] += 1.0;
-      } else {
-        d[D_GPG0] += gt * gt;
-        d[D_GPGC0] += gt * gcv;
-        d[D_GCPGC0] += gcv * gcv;
-      }
-      d[D_GTP] += gt * pc;
-      d[D_GCP] += gcv * pc;
-      d[D_PP] += pc * pc;
-    }
+    d[D_GTP] += gt * pc;
+    d[D_GCP] += gc * pc;
+    d[D_PP] += pc * pc;
   }
-  block_reduce_n<N_DOTS>(d, sm, A.vblk_dots + (long long)vb * N_DOTS);
+  energy_share(A.blk_red, A.inst_blk_off, A.inst_vblk_off, inst, vb, d[D_E]);
+  block_dots(d, sm, A.vblk_dots + (long long)vb * N_DOTS);
 }
 
 // S: per-instance decision
 __global__ __launch_bounds__(NTH) void pncg_scalar_kernel(StatePtrs st, Params P, const double* vblk_dots,
-                                                          const int64_t* inst_vblk_off, const double* blk_red,
-                                                          const int64_t* inst_blk_off) {
-  __shared__ double sm[(NTH / 64) * (N_DOTS + 1)];
-  __shared__ double tot[N_DOTS + 1];
+                                                          const int64_t* inst_vblk_off) {
+  __shared__ double sm[(NTH / 64 + 1) * N_DOTS];
   const int i = blockIdx.x;
   if (st.act_eval[i] == 0) return;
-  double acc[N_DOTS + 1];
-#pragma unroll
-  for (int k = 0; k <= N_DOTS; ++k) acc[k] = 0.0;
-  for (long long vb = inst_vblk_off[i] + threadIdx.x; vb < inst_vblk_off[i + 1]; vb += NTH)
-#pragma unroll
-    for (int k = 0; k < N_DOTS; ++k) acc[k] += vblk_dots[vb * N_DOTS + k];
-  for (long long b = inst_blk_off[i] + threadIdx.x; b < inst_blk_off[i + 1]; b += NTH) acc[N_DOTS] += blk_red[2 * b];
-  block_reduce_n<N_DOTS + 1>(acc, sm, tot);
+  double tot[N_DOTS];
+  const long long vb0 = inst_vblk_off[i];
+  flat_column_sums<N_DOTS, NTH>(vblk_dots + vb0 * N_DOTS, (inst_vblk_off[i + 1] - vb0) * N_DOTS, sm, tot);
   if (threadIdx.x != 0) return;
-  const double fE = tot[N_DOTS];
+  const double fE = tot[D_E];
   const double ft = fE + tot[D_EXT];
   const double gn = sqrt(tot[D_G2]);
   const int phase = st.phase[i];
@@ -292,24 +336,25 @@ __global__ __launch_bounds__(NTH) void pncg_commit_kernel(VecArgs<R> A) {
   const int vb = blockIdx.x;
   const int inst = A.vblk_inst[vb];
   if (A.st.commit[inst] == 0) return;
-  const long long v = (long long)A.vblk_off[vb] + threadIdx.x;
-  if (v >= A.vblk_off[vb + 1]) return;
   const double a = A.st.alpha_commit[inst];
   const R beta = (R)A.st.beta[inst];
   const bool dir = A.st.act_dir[inst] != 0;
   const R pm = (R)A.st.pmean[inst];
-  const unsigned msk = A.mask != nullptr ? A.mask[v] : 0u;
+  const int i0 = 3 * A.vblk_off[vb], i1 = 3 * A.vblk_off[vb + 1];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    const R pc = A.p[3 * v + c];
-    A.x[3 * v + c] += a * (double)pc;
-    const bool fixed = (msk & (1u << c)) != 0;
-    const R gt = fixed ? R(0) : A.gE[3 * v + c] - (A.fext != nullptr ? A.fext[3 * v + c] : R(0));
-    A.gc[3 * v + c] = gt;
+  for (int j = 0; j < DOF_PER_THREAD; ++j) {
+    const int i = i0 + (int)threadIdx.x + j * NTH;
+    if (i >= i1) continue;
+    const int v = i / 3, c = i - 3 * v;
+    const bool fixed = A.mask != nullptr && ((A.mask[v] >> c) & 1u) != 0;
+    const R pc = A.p[i];
+    A.x[i] += a * (double)pc;
+    const R gt = fixed ? R(0) : A.gE[i] - (A.fext != nullptr ? A.fext[i] : R(0));
+    A.gc[i] = gt;
     if (dir) {
-      const R h = A.hd[3 * v + c];
+      const R h = A.hd[i];
       const R Pc = fixed ? R(0) : (h > R(0) ? R(1) / h : R(1) / pm);
-      A.p[3 * v + c] = -Pc * gt + beta * pc;
+      A.p[i] = -Pc * gt + beta * pc;
     }
   }
 }
@@ -317,19 +362,15 @@ __global__ __launch_bounds__(NTH) void pncg_commit_kernel(VecArgs<R> A) {
 // S2: alpha from the quadratic form; clears the per-iteration flags
 __global__ __launch_bounds__(NTH) void pncg_alpha_kernel(StatePtrs st, Params P, const double* blk_red,
                                                          const int64_t* inst_blk_off) {
-  __shared__ double red[NTH / 64];
+  __shared__ double red[2 * (NTH / 64 + 1)];
   const int i = blockIdx.x;
   const bool dir = st.act_dir[i] != 0;
   if (dir) {
-    double q = 0.0;
-    for (long long b = inst_blk_off[i] + threadIdx.x; b < inst_blk_off[i + 1]; b += NTH) q += blk_red[2 * b + 1];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) q += __shfl_down(q, off, 64);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = q;
-    __syncthreads();
+    const long long b0 = inst_blk_off[i];
+    double eq[2];
+    flat_column_sums<2, NTH>(blk_red + 2 * b0, 2 * (inst_blk_off[i + 1] - b0), red, eq);
     if (threadIdx.x == 0) {
-      double pHp = 0.0;
-      for (int w = 0; w < NTH / 64; ++w) pHp += red[w];
+      const double pHp = eq[1];
       st.pHp[i] = pHp;
       double a = -st.gd[i] / pHp;
       if (isnan(a)) a = 0.0;
@@ -373,85 +414,89 @@ enum CgStage : int { CG_DIAG = 0, CG_PRECOND = 1, CG_Q = 2, CG_UPDATE = 3, CG_DI
 template <class R, int STAGE>
 __global__ __launch_bounds__(NTH) void cg_vec_kernel(CgArgs<R> A) {
   __shared__ double sm[(NTH / 64) * N_DOTS];
+  constexpr bool SUMS = STAGE == CG_DIAG || STAGE == CG_Q;
+  __shared__ R win[SUMS ? VSUM_LDS_BYTES / sizeof(R) : 1];
   const int vb = blockIdx.x;
   const int inst = A.vblk_inst[vb];
   if (STAGE != CG_DIAG && STAGE != CG_PRECOND && A.st.act_eval[inst] == 0) return;
-  const long long v = (long long)A.vblk_off[vb] + threadIdx.x;
-  const bool in = v < A.vblk_off[vb + 1];
+  const int v0 = A.vblk_off[vb], v1 = A.vblk_off[vb + 1];
+  const int i0 = 3 * v0, i1 = 3 * v1;
   double dd[N_DOTS];
 #pragma unroll
   for (int k = 0; k < N_DOTS; ++k) dd[k] = 0.0;
-  const unsigned msk = (in && A.mask != nullptr) ? A.mask[v] : 0u;
-  if (in) {
-    if constexpr (STAGE == CG_DIAG || STAGE == CG_Q) {
-      R* out = STAGE == CG_DIAG ? A.hd : A.q;
-      R s3[3];
-      vertex_partial_sum<R, 3>(A.vtx_slot_off, A.partial, v, s3);
+  int k0[DOF_PER_THREAD], k1[DOF_PER_THREAD];
+  if constexpr (SUMS) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const bool fixed = (msk & (1u << c)) != 0;
-        if constexpr (STAGE == CG_DIAG) {
-          out[3 * v + c] = s3[c];
-          const R b = fixed ? R(0) : A.rhs[3 * v + c];
-          A.r[3 * v + c] = b;
-          A.y[3 * v + c] = R(0);
-          A.d[3 * v + c] = R(0);
-          A.q[3 * v + c] = R(0);
-          if (!fixed) {
-            dd[D_G2] += (double)b * (double)b;
-            if (s3[c] > R(0)) {
-              dd[D_HSUM] += (double)s3[c];
-              dd[D_HCNT] += 1.0;
-            }
+    for (int j = 0; j < DOF_PER_THREAD; ++j) {
+      const int i = i0 + (int)threadIdx.x + j * NTH;
+      k0[j] = k1[j] = 0;
+      if (i < i1) {
+        k0[j] = A.vtx_slot_off[i / 3];
+        k1[j] = A.vtx_slot_off[i / 3 + 1];
+      }
+    }
+  }
+  VertexSums<R, 3, NTH> vs{A.vtx_slot_off, A.partial, win};
+  if constexpr (SUMS) vs.stage(v0, v1, VSUM_LDS_BYTES / (int)sizeof(R));
+#pragma unroll
+  for (int j = 0; j < DOF_PER_THREAD; ++j) {
+    const int i = i0 + (int)threadIdx.x + j * NTH;
+    if (i >= i1) continue;
+    const int v = i / 3, c = i - 3 * v;
+    const bool fixed = A.mask != nullptr && ((A.mask[v] >> c) & 1u) != 0;
+    if constexpr (SUMS) {
+      R s1[1];
+      vs.sum_dof(k0[j], k1[j], c, s1);
+      if constexpr (STAGE == CG_DIAG) {
+        A.hd[i] = s1[0];
+        const R b = fixed ? R(0) : A.rhs[i];
+        A.r[i] = b;
+        A.y[i] = R(0);
+        A.d[i] = R(0);
+        A.q[i] = R(0);
+        if (!fixed) {
+          dd[D_G2] += (double)b * (double)b;
+          if (s1[0] > R(0)) {
+            dd[D_HSUM] += (double)s1[0];
+            dd[D_HCNT] += 1.0;
           }
-        } else {
-          const R qv = fixed ? R(0) : s3[c];
-          out[3 * v + c] = qv;
-          dd[D_GTP] += (double)A.d[3 * v + c] * (double)qv;
         }
+      } else {
+        const R qv = fixed ? R(0) : s1[0];
+        A.q[i] = qv;
+        dd[D_GTP] += (double)A.d[i] * (double)qv;
       }
     } else if constexpr (STAGE == CG_PRECOND || STAGE == CG_UPDATE) {
       const R a = (R)A.st.alpha[inst];
       const R pm = (R)A.st.pmean[inst];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const bool fixed = (msk & (1u << c)) != 0;
-        R rv = A.r[3 * v + c];
-        if constexpr (STAGE == CG_UPDATE) {
-          A.y[3 * v + c] += a * A.d[3 * v + c];
-          rv -= a * A.q[3 * v + c];
-          A.r[3 * v + c] = rv;
-        }
-        const R h = A.hd[3 * v + c];
-        const R zv = fixed ? R(0) : (h > R(0) ? rv / h : rv / pm);
-        A.z[3 * v + c] = zv;
-        if constexpr (STAGE == CG_PRECOND) A.d[3 * v + c] = zv;
-        dd[D_GPG] += (double)rv * (double)zv;
-        dd[D_G2] += (double)rv * (double)rv;
+      R rv = A.r[i];
+      if constexpr (STAGE == CG_UPDATE) {
+        A.y[i] += a * A.d[i];
+        rv -= a * A.q[i];
+        A.r[i] = rv;
       }
+      const R h = A.hd[i];
+      const R zv = fixed ? R(0) : (h > R(0) ? rv / h : rv / pm);
+      A.z[i] = zv;
+      if constexpr (STAGE == CG_PRECOND) A.d[i] = zv;
+      dd[D_GPG] += (double)rv * (double)zv;
+      dd[D_G2] += (double)rv * (double)rv;
     } else {  // CG_DIR
-      const R beta = (R)A.st.beta[inst];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) A.d[3 * v + c] = A.z[3 * v + c] + beta * A.d[3 * v + c];
+      A.d[i] = A.z[i] + (R)A.st.beta[inst] * A.d[i];
     }
   }
-  if constexpr (STAGE != CG_DIR) block_reduce_n<N_DOTS>(dd, sm, A.vblk_dots + (long long)vb * N_DOTS);
+  if constexpr (STAGE != CG_DIR) block_dots(dd, sm, A.vblk_dots + (long long)vb * N_DOTS);
 }
 
 // per-instance CG scalars; stage 0: after DIAG, 1: after PRECOND, 2: after Q, 3: after UPDATE
 __global__ __launch_bounds__(NTH) void cg_scalar_kernel(StatePtrs st, int stage, double rtol, int max_iters,
                                                         const double* vblk_dots, const int64_t* inst_vblk_off) {
-  __shared__ double sm[(NTH / 64) * N_DOTS];
-  __shared__ double tot[N_DOTS];
+  __shared__ double sm[(NTH / 64 + 1) * N_DOTS];
   const int i = blockIdx.x;
   if (stage >= 2 && st.act_eval[i] == 0) return;
-  double acc[N_DOTS];
-#pragma unroll
-  for (int k = 0; k < N_DOTS; ++k) acc[k] = 0.0;
-  for (long long vb = inst_vblk_off[i] + threadIdx.x; vb < inst_vblk_off[i + 1]; vb += NTH)
-#pragma unroll
-    for (int k = 0; k < N_DOTS; ++k) acc[k] += vblk_dots[vb * N_DOTS + k];
-  block_reduce_n<N_DOTS>(acc, sm, tot);
+  double tot[N_DOTS];
+  const long long vb0 = inst_vblk_off[i];
+  flat_column_sums<N_DOTS, NTH>(vblk_dots + vb0 * N_DOTS, (inst_vblk_off[i + 1] - vb0) * N_DOTS, sm, tot);
   if (threadIdx.x != 0) return;
   if (stage == 0) {
     st.pmean[i] = tot[D_HCNT] > 0.0 ? tot[D_HSUM] / tot[D_HCNT] : 1.0;
@@ -503,7 +548,8 @@ __global__ __launch_bounds__(NTH) void cg_scalar_kernel(StatePtrs st, int stage,
 using namespace apl;
 
 struct apl_solver {
-  apl_mesh* mesh = nullptr;
+  apl_mesh* mesh = nullptr;  // holds one reference of the mesh
+  int device = 0;
   int material = 0;
   void* vecs = nullptr;  // p, gc, gE, hd, r, z, d, q, hd (9 x V x 3)
   void* x64 = nullptr;   // FP64 iterate of FP32 meshes (V x 3)
@@ -537,6 +583,9 @@ static VecArgs<R> vec_args(apl_solver* s, double* x, const void* fext, const uin
   A.gE = base + 2 * nv;
   A.hd = base + 3 * nv;
   A.vblk_dots = m->d.vblk_dots;
+  A.blk_red = m->d.blk_red;
+  A.inst_blk_off = m->d.inst_blk_off;
+  A.inst_vblk_off = m->d.inst_vblk_off;
   A.st = s->st;
   return A;
 }
@@ -567,8 +616,7 @@ static int pncg_iteration(apl_solver* s, const void* mu, const void* lam, double
                   grid, st, x_f64))
     return -1;
   if (nvb) hipLaunchKernelGGL((pncg_reduce_kernel<R>), dim3(nvb), dim3(NTH), 0, st, A);
-  hipLaunchKernelGGL(pncg_scalar_kernel, dim3(ni), dim3(NTH), 0, st, s->st, P, m->d.vblk_dots, m->d.inst_vblk_off,
-                     m->d.blk_red, m->d.inst_blk_off);
+  hipLaunchKernelGGL(pncg_scalar_kernel, dim3(ni), dim3(NTH), 0, st, s->st, P, m->d.vblk_dots, m->d.inst_vblk_off);
   if (nvb) hipLaunchKernelGGL((pncg_commit_kernel<R>), dim3(nvb), dim3(NTH), 0, st, A);
   if (launch_elem(m, s->material, W_HQ, mu, lam, x, A.p, nullptr, s->st.act_dir, nullptr, nullptr, 0, grid, st,
                   x_f64))
@@ -627,11 +675,11 @@ static int pncg_solve_t(apl_solver* s, const void* mu, const void* lam, void* x,
     std::memcpy(k, &grid, sizeof grid);
     if (s->gexec == nullptr || key != s->gkey) {
       if (s->gexec) {
-        (void)hipGraphExecDestroy(s->gexec);
+        drop_hip_error(hipGraphExecDestroy(s->gexec));
         s->gexec = nullptr;
       }
       if (s->graph) {
-        (void)hipGraphDestroy(s->graph);
+        drop_hip_error(hipGraphDestroy(s->graph));
         s->graph = nullptr;
       }
       if (s->cap == nullptr) APL_CHECK_HIP(hipStreamCreateWithFlags(&s->cap, hipStreamNonBlocking));
@@ -641,7 +689,7 @@ static int pncg_solve_t(apl_solver* s, const void* mu, const void* lam, void* x,
       hipGraph_t g = nullptr;
       hipError_t e = hipStreamEndCapture(s->cap, &g);
       if (rc != 0 || e != hipSuccess) {
-        if (g) (void)hipGraphDestroy(g);
+        if (g) drop_hip_error(hipGraphDestroy(g));
         if (rc == 0) set_error(std::string("graph capture failed: ") + hipGetErrorString(e));
         return -1;
       }
@@ -809,6 +857,8 @@ int apl_solver_create(apl_mesh* m, int material, apl_solver** out) {
   (void)hipGetLastError();  // drop errors left by unrelated earlier HIP calls
   apl_solver* s = new apl_solver();
   s->mesh = m;
+  ++m->refs;
+  s->device = m->device;
   s->material = material;
   const int n = (int)m->L.n_inst;
   const size_t vb = 9 * 3 * (size_t)std::max<int64_t>(m->L.V, 1) * real_size(m);
@@ -821,7 +871,11 @@ int apl_solver_create(apl_mesh* m, int material, apl_solver** out) {
     apl_solver_destroy(s);
     return -1;
   }
-  APL_CHECK_HIP(hipMemset(s->vecs, 0, vb));
+  if (hipMemset(s->vecs, 0, vb) != hipSuccess) {
+    set_error("hipMemset failed creating the solver");
+    apl_solver_destroy(s);
+    return -1;
+  }
   double* d = s->sd;
   s->st.f = d + 0 * n;
   s->st.gd = d + 1 * n;
@@ -850,15 +904,16 @@ int apl_solver_create(apl_mesh* m, int material, apl_solver** out) {
 
 int apl_solver_destroy(apl_solver* s) {
   if (s == nullptr) return 0;
-  if (s->mesh) (void)hipSetDevice(s->mesh->device);
-  if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
-  if (s->graph) (void)hipGraphDestroy(s->graph);
-  if (s->cap) (void)hipStreamDestroy(s->cap);
-  if (s->vecs) (void)hipFree(s->vecs);
-  if (s->x64) (void)hipFree(s->x64);
-  if (s->sd) (void)hipFree(s->sd);
-  if (s->si) (void)hipFree(s->si);
-  if (s->host_status) (void)hipHostFree(s->host_status);
+  drop_hip_error(hipSetDevice(s->device));
+  if (s->gexec) drop_hip_error(hipGraphExecDestroy(s->gexec));
+  if (s->graph) drop_hip_error(hipGraphDestroy(s->graph));
+  if (s->cap) drop_hip_error(hipStreamDestroy(s->cap));
+  if (s->vecs) drop_hip_error(hipFree(s->vecs));
+  if (s->x64) drop_hip_error(hipFree(s->x64));
+  if (s->sd) drop_hip_error(hipFree(s->sd));
+  if (s->si) drop_hip_error(hipFree(s->si));
+  if (s->host_status) drop_hip_error(hipHostFree(s->host_status));
+  if (s->mesh) mesh_release(s->mesh);
   delete s;
   return 0;
 }

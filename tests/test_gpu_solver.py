@@ -214,3 +214,27 @@ def test_host_driven_pncg_for_general_models():
     assert sol.success
     x_ref, gn = _oracle_equilibrium(region, "snh", mu, lam, f, fm, fwd.state.u.cpu().numpy())
     assert np.abs(fwd.state.u.cpu().numpy() - x_ref).max() <= 1e-7
+
+
+def test_handles_destroyed_in_any_order_leave_no_hip_error():
+    """A solver keeps its mesh alive; destroying the mesh handle first (as a
+    garbage collector may, finalising a reference cycle) must neither free
+    memory the solver still uses nor leave a HIP error for the next caller."""
+    from apple_amd import _lib
+    from apple_amd.solver import DeviceSolver
+
+    m = cube(3)
+    m.cell_data["mu"] = np.full(m.n_cells, 1e4)
+    m.cell_data["lambda"] = np.full(m.n_cells, 1e5)
+    pot = fem.StableNeoHookean.from_region(Region.from_mesh(m), dtype=torch.float64, device="cuda")
+    solver = DeviceSolver(pot)
+    pot.mesh.close()  # the solver still holds the mesh
+    x = torch.zeros(m.n_points, 3, dtype=torch.float64, device="cuda")
+    st = solver.solve(x, None, None, PncgConfig(rtol=1e-6, max_steps=5))
+    assert st[0].result in (Result.PRIMARY_SUCCESS, Result.MAX_STEPS_REACHED)
+    solver.close()
+    # torch checks hipGetLastError after its own launches: a stale error would raise here
+    y = torch.arange(10, device="cuda", dtype=torch.float64) * 2.0
+    torch.cuda.synchronize()
+    assert float(y.sum()) == 90.0
+    assert _lib.lib() is not None
