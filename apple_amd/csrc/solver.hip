@@ -57,19 +57,24 @@ struct Params {
 // half of its entries and adds its partner's copy of them) leaves dot
 // lane / 4 in lanes 4j..4j+3; two more exchanges complete those sums, and
 // the waves are added in order.  17 shuffles instead of 16 x 6.
+template <int HALF, int OFF>
+__device__ __forceinline__ void halving_step(double (&v)[N_DOTS], int lane) {
+  const bool up = (lane & OFF) != 0;
+#pragma unroll
+  for (int k = 0; k < HALF; ++k) {
+    const double send = up ? v[k] : v[k + HALF];
+    const double keep = up ? v[k + HALF] : v[k];
+    v[k] = keep + __shfl_xor(send, OFF, 64);
+  }
+}
+
 __device__ inline void block_dots(double (&v)[N_DOTS], double* sm, double* out) {
   static_assert(N_DOTS == 16, "halving tree is written for 16 dots");
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-  for (int half = 8, off = 32; half >= 1; half >>= 1, off >>= 1) {
-    const bool up = (lane & off) != 0;
-#pragma unroll
-    for (int k = 0; k < half; ++k) {
-      const double send = up ? v[k] : v[k + half];
-      const double keep = up ? v[k + half] : v[k];
-      v[k] = keep + __shfl_xor(send, off, 64);
-    }
-  }
+  halving_step<8, 32>(v, lane);
+  halving_step<4, 16>(v, lane);
+  halving_step<2, 8>(v, lane);
+  halving_step<1, 4>(v, lane);
   v[0] += __shfl_xor(v[0], 2, 64);
   v[0] += __shfl_xor(v[0], 1, 64);
   if ((lane & 3) == 0) sm[w * N_DOTS + (lane >> 2)] = v[0];
