@@ -37,6 +37,10 @@
 
 namespace apl {
 
+// threads of the per-instance scalar kernels: each sums whole tables of
+// per-block values, so more loads in flight shorten them
+constexpr int SCAL_TH = 1024;
+
 enum : int { D_G2 = 0, D_GPG, D_GPG0, D_GPGC, D_GPGC0, D_GCPGC, D_GCPGC0, D_GTP, D_GCP, D_HSUM, D_HCNT, D_EXT, D_PP, D_E };
 static_assert(D_E < N_DOTS, "dot table too small");
 static_assert(D_PP < N_DOTS, "dot slots");
@@ -234,14 +238,14 @@ __global__ __launch_bounds__(NTH) void pncg_reduce_kernel(VecArgs<R> A) {
 }
 
 // S: per-instance decision
-__global__ __launch_bounds__(NTH) void pncg_scalar_kernel(StatePtrs st, Params P, const double* vblk_dots,
+__global__ __launch_bounds__(SCAL_TH) void pncg_scalar_kernel(StatePtrs st, Params P, const double* vblk_dots,
                                                           const int64_t* inst_vblk_off) {
-  __shared__ double sm[(NTH / 64 + 1) * N_DOTS];
+  __shared__ double sm[(SCAL_TH / 64 + 1) * N_DOTS];
   const int i = blockIdx.x;
   if (st.act_eval[i] == 0) return;
   double tot[N_DOTS];
   const long long vb0 = inst_vblk_off[i];
-  flat_column_sums<N_DOTS, NTH>(vblk_dots + vb0 * N_DOTS, (inst_vblk_off[i + 1] - vb0) * N_DOTS, sm, tot);
+  flat_column_sums<N_DOTS, SCAL_TH>(vblk_dots + vb0 * N_DOTS, (inst_vblk_off[i + 1] - vb0) * N_DOTS, sm, tot);
   if (threadIdx.x != 0) return;
   const double fE = tot[D_E];
   const double ft = fE + tot[D_EXT];
@@ -365,15 +369,15 @@ __global__ __launch_bounds__(NTH) void pncg_commit_kernel(VecArgs<R> A) {
 }
 
 // S2: alpha from the quadratic form; clears the per-iteration flags
-__global__ __launch_bounds__(NTH) void pncg_alpha_kernel(StatePtrs st, Params P, const double* blk_red,
+__global__ __launch_bounds__(SCAL_TH) void pncg_alpha_kernel(StatePtrs st, Params P, const double* blk_red,
                                                          const int64_t* inst_blk_off) {
-  __shared__ double red[2 * (NTH / 64 + 1)];
+  __shared__ double red[2 * (SCAL_TH / 64 + 1)];
   const int i = blockIdx.x;
   const bool dir = st.act_dir[i] != 0;
   if (dir) {
     const long long b0 = inst_blk_off[i];
     double eq[2];
-    flat_column_sums<2, NTH>(blk_red + 2 * b0, 2 * (inst_blk_off[i + 1] - b0), red, eq);
+    flat_column_sums<2, SCAL_TH>(blk_red + 2 * b0, 2 * (inst_blk_off[i + 1] - b0), red, eq);
     if (threadIdx.x == 0) {
       const double pHp = eq[1];
       st.pHp[i] = pHp;
@@ -494,14 +498,14 @@ __global__ __launch_bounds__(NTH) void cg_vec_kernel(CgArgs<R> A) {
 }
 
 // per-instance CG scalars; stage 0: after DIAG, 1: after PRECOND, 2: after Q, 3: after UPDATE
-__global__ __launch_bounds__(NTH) void cg_scalar_kernel(StatePtrs st, int stage, double rtol, int max_iters,
+__global__ __launch_bounds__(SCAL_TH) void cg_scalar_kernel(StatePtrs st, int stage, double rtol, int max_iters,
                                                         const double* vblk_dots, const int64_t* inst_vblk_off) {
-  __shared__ double sm[(NTH / 64 + 1) * N_DOTS];
+  __shared__ double sm[(SCAL_TH / 64 + 1) * N_DOTS];
   const int i = blockIdx.x;
   if (stage >= 2 && st.act_eval[i] == 0) return;
   double tot[N_DOTS];
   const long long vb0 = inst_vblk_off[i];
-  flat_column_sums<N_DOTS, NTH>(vblk_dots + vb0 * N_DOTS, (inst_vblk_off[i + 1] - vb0) * N_DOTS, sm, tot);
+  flat_column_sums<N_DOTS, SCAL_TH>(vblk_dots + vb0 * N_DOTS, (inst_vblk_off[i + 1] - vb0) * N_DOTS, sm, tot);
   if (threadIdx.x != 0) return;
   if (stage == 0) {
     st.pmean[i] = tot[D_HCNT] > 0.0 ? tot[D_HSUM] / tot[D_HCNT] : 1.0;
@@ -621,12 +625,12 @@ static int pncg_iteration(apl_solver* s, const void* mu, const void* lam, double
                   grid, st, x_f64))
     return -1;
   if (nvb) hipLaunchKernelGGL((pncg_reduce_kernel<R>), dim3(nvb), dim3(NTH), 0, st, A);
-  hipLaunchKernelGGL(pncg_scalar_kernel, dim3(ni), dim3(NTH), 0, st, s->st, P, m->d.vblk_dots, m->d.inst_vblk_off);
+  hipLaunchKernelGGL(pncg_scalar_kernel, dim3(ni), dim3(SCAL_TH), 0, st, s->st, P, m->d.vblk_dots, m->d.inst_vblk_off);
   if (nvb) hipLaunchKernelGGL((pncg_commit_kernel<R>), dim3(nvb), dim3(NTH), 0, st, A);
   if (launch_elem(m, s->material, W_HQ, mu, lam, x, A.p, nullptr, s->st.act_dir, nullptr, nullptr, 0, grid, st,
                   x_f64))
     return -1;
-  hipLaunchKernelGGL(pncg_alpha_kernel, dim3(ni), dim3(NTH), 0, st, s->st, P, m->d.blk_red, m->d.inst_blk_off);
+  hipLaunchKernelGGL(pncg_alpha_kernel, dim3(ni), dim3(SCAL_TH), 0, st, s->st, P, m->d.blk_red, m->d.inst_blk_off);
   APL_CHECK_HIP(hipGetLastError());
   return 0;
 }
@@ -780,10 +784,10 @@ static int cg_iteration(apl_solver* s, const void* mu, const void* lam, const vo
   const unsigned ni = (unsigned)m->L.n_inst;
   if (launch_elem(m, s->material, W_HP, mu, lam, x, A.d, nullptr, s->st.act_eval, A.q, nullptr, 0, 0, st)) return -1;
   if (nvb) hipLaunchKernelGGL((cg_vec_kernel<R, CG_Q>), dim3(nvb), dim3(NTH), 0, st, A);
-  hipLaunchKernelGGL(cg_scalar_kernel, dim3(ni), dim3(NTH), 0, st, s->st, 2, rtol, max_iters, m->d.vblk_dots,
+  hipLaunchKernelGGL(cg_scalar_kernel, dim3(ni), dim3(SCAL_TH), 0, st, s->st, 2, rtol, max_iters, m->d.vblk_dots,
                      m->d.inst_vblk_off);
   if (nvb) hipLaunchKernelGGL((cg_vec_kernel<R, CG_UPDATE>), dim3(nvb), dim3(NTH), 0, st, A);
-  hipLaunchKernelGGL(cg_scalar_kernel, dim3(ni), dim3(NTH), 0, st, s->st, 3, rtol, max_iters, m->d.vblk_dots,
+  hipLaunchKernelGGL(cg_scalar_kernel, dim3(ni), dim3(SCAL_TH), 0, st, s->st, 3, rtol, max_iters, m->d.vblk_dots,
                      m->d.inst_vblk_off);
   if (nvb) hipLaunchKernelGGL((cg_vec_kernel<R, CG_DIR>), dim3(nvb), dim3(NTH), 0, st, A);
   APL_CHECK_HIP(hipGetLastError());
@@ -802,10 +806,10 @@ static int cg_solve_t(apl_solver* s, const void* mu, const void* lam, const void
   if (fill<R>(A.hd, 3 * (long long)m->L.V, R(0), st)) return -1;
   if (launch_elem(m, s->material, W_H, mu, lam, x, nullptr, nullptr, nullptr, nullptr, A.hd, 0, 0, st)) return -1;
   if (nvb) hipLaunchKernelGGL((cg_vec_kernel<R, CG_DIAG>), dim3(nvb), dim3(NTH), 0, st, A);
-  hipLaunchKernelGGL(cg_scalar_kernel, dim3(n), dim3(NTH), 0, st, s->st, 0, rtol, max_iters, m->d.vblk_dots,
+  hipLaunchKernelGGL(cg_scalar_kernel, dim3(n), dim3(SCAL_TH), 0, st, s->st, 0, rtol, max_iters, m->d.vblk_dots,
                      m->d.inst_vblk_off);
   if (nvb) hipLaunchKernelGGL((cg_vec_kernel<R, CG_PRECOND>), dim3(nvb), dim3(NTH), 0, st, A);
-  hipLaunchKernelGGL(cg_scalar_kernel, dim3(n), dim3(NTH), 0, st, s->st, 1, rtol, max_iters, m->d.vblk_dots,
+  hipLaunchKernelGGL(cg_scalar_kernel, dim3(n), dim3(SCAL_TH), 0, st, s->st, 1, rtol, max_iters, m->d.vblk_dots,
                      m->d.inst_vblk_off);
   APL_CHECK_HIP(hipGetLastError());
   (void)elem_grid(m, s->material, W_HP);

@@ -137,6 +137,8 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--chunks", type=int, default=1, help="tet chunks of 256 per element block")
+    ap.add_argument("--batched-only", action="store_true", help="only the batched forward+adjoint comparison")
+    ap.add_argument("--batched-meshes", type=int, default=256)
     ap.add_argument("--profile-pass", action="store_true",
                     help="only run pass-A launches and a reference copy (for rocprofv3 counter runs)")
     args = ap.parse_args()
@@ -153,6 +155,9 @@ def main():
     torch.cuda.set_device(dev)
     dtype = torch.float32
     n = args.n
+    if args.batched_only:
+        print(json.dumps(bench_batched(dev, dtype, n_meshes=args.batched_meshes)), flush=True)
+        return
     log(f"[bench] building {n}^3 x 5 cube (SNH, FP32)")
     prob = make_cube_problem(n, "snh", dtype, dev, chunks=args.chunks)
     pot = prob["pot"]
@@ -275,7 +280,7 @@ def main():
 
     # ---------------------------------------------------------------- batched vs plain
     if not (args.quick or args.no_batch):
-        out["batched"] = bench_batched(dev, dtype)
+        out["batched"] = bench_batched(dev, dtype, n_meshes=args.batched_meshes)
 
     # ---------------------------------------------------------------- CPU baseline
     if not (args.quick or args.no_cpu):
@@ -284,59 +289,113 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def bench_batched(dev, dtype, n_meshes=32, lo=6, hi=16, seed=2):
-    """Batched PNCG over uneven meshes vs the same solves one instance at a time."""
-    import torch
-
+def make_cube_instances(n_meshes, lo, hi, seed_sizes, seed_mat, noise_frac=0.1):
+    """Cubes with n ~ U{lo..hi} cells per side (seed_sizes); per-instance mu ~
+    U[1e3, 1e5], nu ~ U[0.3, 0.45], lambda = 2 mu nu / (1 - 2 nu) (seed_mat);
+    gravity (rho = 1e3, lumped), z = 0 face pinned, start = noise_frac * h *
+    N(0, 1) on the free DOFs (seed 100 + k)."""
     from apple_amd import batch
     from apple_amd.mesh import Region, cube
-    from apple_amd.solver import PncgConfig
 
-    rng = np.random.default_rng(seed)
-    mrng = np.random.default_rng(seed + 1)
+    sizes = np.random.default_rng(seed_sizes).integers(lo, hi + 1, n_meshes)
+    mrng = np.random.default_rng(seed_mat)
+    cache = {}
     insts = []
-    for k in range(n_meshes):
-        n = int(rng.integers(lo, hi + 1))
-        m = cube(n)
-        region = Region.from_mesh(m)
+    for k, n in enumerate(sizes):
+        n = int(n)
+        if n not in cache:
+            m = cube(n)
+            region = Region.from_mesh(m)
+            f = np.zeros((m.n_points, 3))
+            np.add.at(f[:, 2], m.cells.reshape(-1), np.repeat(-9.8e3 * region.dV / 4, 4))
+            fixed = np.zeros((m.n_points, 3), bool)
+            fixed[m.points[:, 2] == 0] = True
+            cache[n] = (m, region, f, fixed)
+        m, region, f, fixed = cache[n]
         mu_i = mrng.uniform(1e3, 1e5)
         nu = mrng.uniform(0.3, 0.45)
         mu = np.full(m.n_cells, mu_i)
         lam = 2 * mu * nu / (1 - 2 * nu)
-        f = np.zeros((m.n_points, 3))
-        np.add.at(f[:, 2], m.cells.reshape(-1), np.repeat(-9.8e3 * region.dV / 4, 4))
-        fixed = np.zeros((m.n_points, 3), bool)
-        fixed[m.points[:, 2] == 0] = True
-        x0 = 0.1 / n * np.random.default_rng(100 + k).standard_normal((m.n_points, 3))
+        x0 = noise_frac / n * np.random.default_rng(100 + k).standard_normal((m.n_points, 3))
         x0[fixed] = 0
         insts.append(batch.Instance(region, mu, lam, f, fixed, x0))
+    return insts, [int(n) for n in sizes]
+
+
+def forward_adjoint(b, cfg, adj_rtol, adjoint=True):
+    """Forward PNCG of every instance of batch b, then (adjoint) the PCG solve
+    H p = -(x - x0) and the per-tet d L / d(mu, lambda) for
+    L = 1/2 ||x - x0||^2.  Returns (x, forward stats, adjoint stats, dmu, dlam)."""
+    import torch
+
+    solver = b.solver()
+    x = b.x0.clone()
+    st = solver.solve(x, b.f_ext, b.mask, cfg)
+    if not adjoint:
+        return x, st, None, None, None
+    rhs = (b.x0 - x).contiguous()
+    p = torch.zeros_like(x)
+    cs = solver.cg(x, rhs, p, b.mask, rtol=adj_rtol, max_iters=20000)
+    dmu = torch.empty(b.n_tets, dtype=b.dtype, device=b.device)
+    dlam = torch.empty_like(dmu)
+    b.mesh.eval(b.material, b.mu, b.lam, x, p, dmu=dmu, dlam=dlam)
+    return x, st, cs, dmu, dlam
+
+
+def bench_batched(dev, dtype, n_meshes=256, lo=6, hi=32, seed_sizes=2, seed_mat=3, adj_rtol=1e-6):
+    """Forward + adjoint of uneven meshes as one batch vs the same work as a
+    sequential loop of per-instance GPU solves (setup excluded from both)."""
+    import torch
+
+    from apple_amd import _lib, batch
+    from apple_amd.solver import PncgConfig
+
+    t_setup = time.perf_counter()
+    insts, sizes = make_cube_instances(n_meshes, lo, hi, seed_sizes, seed_mat)
     cfg = PncgConfig(rtol=1e-6, max_steps=20000)
-    b = batch.Batch(insts, 0, dtype=dtype, device=dev)
-    b.solve(config=PncgConfig(rtol=1e-6, max_steps=16))
+    b = batch.Batch(insts, _lib.APL_SNH, dtype=dtype, device=dev)
+    singles = [batch.Batch([inst], _lib.APL_SNH, dtype=dtype, device=dev) for inst in insts]
+    warm = PncgConfig(rtol=1e-6, max_steps=16)
+    for bb in [b] + singles:
+        forward_adjoint(bb, warm, adj_rtol, adjoint=False)
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    xb, stb = b.solve(config=cfg)
-    torch.cuda.synchronize()
-    t_batch = time.perf_counter() - t0
-    singles = [batch.Batch([inst], 0, dtype=dtype, device=dev) for inst in insts]
-    for s in singles:
-        s.solve(config=PncgConfig(rtol=1e-6, max_steps=16))
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ok = 0
-    for s in singles:
-        _, st = s.solve(config=cfg)
-        ok += st[0].success
-    torch.cuda.synchronize()
-    t_plain = time.perf_counter() - t0
-    return {
-        "workload": f"{n_meshes} cubes, n ~ U{{{lo}..{hi}}}, {b.n_tets} tets total, SNH FP32, PNCG to 1e-6",
-        "batched_solves_per_s": round(n_meshes / t_batch, 3),
-        "plain_solves_per_s": round(n_meshes / t_plain, 3),
-        "speedup": round(t_plain / t_batch, 3),
-        "batched_all_converged": all(s.success for s in stb),
-        "plain_all_converged": ok == n_meshes,
-    }
+    t_setup = time.perf_counter() - t_setup
+
+    res = {}
+    for adjoint in (False, True):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, stb, csb, _, _ = forward_adjoint(b, cfg, adj_rtol, adjoint)
+        torch.cuda.synchronize()
+        t_batch = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ok_fwd, ok_adj = 0, 0
+        for bb in singles:
+            _, st, cs, _, _ = forward_adjoint(bb, cfg, adj_rtol, adjoint)
+            ok_fwd += st[0].success
+            ok_adj += cs[0].success if cs is not None else 0
+        torch.cuda.synchronize()
+        t_plain = time.perf_counter() - t0
+        key = "forward_adjoint" if adjoint else "forward"
+        res[key] = {
+            "batched_instances_per_s": round(n_meshes / t_batch, 3),
+            "plain_instances_per_s": round(n_meshes / t_plain, 3),
+            "speedup": round(t_plain / t_batch, 3),
+            "batched_s": round(t_batch, 4),
+            "plain_s": round(t_plain, 4),
+            "batched_forward_converged": int(sum(s.success for s in stb)),
+            "plain_forward_converged": int(ok_fwd),
+        }
+        if adjoint:
+            res[key]["batched_adjoint_converged"] = int(sum(s.success for s in csb))
+            res[key]["plain_adjoint_converged"] = int(ok_adj)
+        log(f"[bench] batched {key}: {t_batch:.3f} s vs plain {t_plain:.3f} s ({t_plain / t_batch:.2f}x)")
+    res["workload"] = (f"{n_meshes} cubes, n ~ U{{{lo}..{hi}}} (seed {seed_sizes}), {b.n_tets} tets total, SNH FP32, "
+                       f"mu ~ U[1e3,1e5], nu ~ U[0.3,0.45] (seed {seed_mat}), PNCG to 1e-6; adjoint: PCG to "
+                       f"{adj_rtol:g} on H p = -(x - x0), then the mixed pass for dL/d(mu, lambda)")
+    res["max_n"] = max(sizes)
+    res["setup_s"] = round(t_setup, 2)
+    return res
 
 
 if __name__ == "__main__":

@@ -66,6 +66,9 @@ int apl_version(void);
 int apl_mesh_create(int dtype, int64_t n_tets, int64_t n_verts, const int32_t* cells, const double* dminv,
                     const double* dv, int64_t n_inst, const int64_t* inst_tet_off, const int64_t* inst_vtx_off,
                     int chunks, int device, apl_mesh** out);
+/* Releases the caller's handle.  Solvers built on the mesh hold their own
+ * reference: the device memory is freed when the last of them is destroyed,
+ * so the handles may be destroyed in any order. */
 int apl_mesh_destroy(apl_mesh* mesh);
 
 typedef struct {
