@@ -63,7 +63,7 @@ class PncgParams(ctypes.Structure):
         ("iters_per_launch", c_i32),
         ("use_graph", c_i32),
         ("grid_override", c_i32),
-        ("pad", c_i32),
+        ("stall_steps", c_i32),
     ]
 
 

@@ -46,14 +46,15 @@ static_assert(D_E < N_DOTS, "dot table too small");
 static_assert(D_PP < N_DOTS, "dot slots");
 
 struct StatePtrs {
-  double *f, *gd, *alpha, *alpha_commit, *beta, *gnorm0, *gnorm, *pmean, *f_first, *pHp;
-  int32_t *status, *act_eval, *act_dir, *commit, *n_steps, *n_evals, *ls_step, *n_backtracks, *phase, *restarts;
+  double *f, *gd, *alpha, *alpha_commit, *beta, *gnorm0, *gnorm, *pmean, *f_first, *pHp, *gbest;
+  int32_t *status, *act_eval, *act_dir, *commit, *n_steps, *n_evals, *ls_step, *n_backtracks, *phase, *restarts,
+      *best_step;
 };
-constexpr int N_SD = 10, N_SI = 10;
+constexpr int N_SD = 11, N_SI = 11;
 
 struct Params {
   double rtol, atol, overstep, c1, wolfe_delta, wolfe_eps, beta_max;
-  int max_steps, max_backtracking;
+  int max_steps, max_backtracking, stall_steps;
 };
 
 // Workgroup sums of the 16 per-thread dots, in a fixed order, to out[0..16).
@@ -149,6 +150,8 @@ __global__ void pncg_init_kernel(StatePtrs st, int n) {
   st.pmean[i] = 1.0;
   st.f_first[i] = 0.0;
   st.pHp[i] = 0.0;
+  st.gbest[i] = INFINITY;
+  st.best_step[i] = 0;
   st.status[i] = APL_RUNNING;
   st.act_eval[i] = 1;
   st.act_dir[i] = 0;
@@ -276,6 +279,11 @@ __global__ __launch_bounds__(SCAL_TH) void pncg_scalar_kernel(StatePtrs st, Para
       st.n_steps[i] += 1;
       st.restarts[i] = 0;
     }
+    // progress marker: the last step that lowered the best gradient norm by 10%
+    if (gn < 0.9 * st.gbest[i]) {
+      st.gbest[i] = gn;
+      st.best_step[i] = st.n_steps[i];
+    }
     int status = APL_RUNNING;
     if (!isfinite(gn) || !isfinite(ft))
       status = APL_NON_FINITE;
@@ -283,6 +291,9 @@ __global__ __launch_bounds__(SCAL_TH) void pncg_scalar_kernel(StatePtrs st, Para
       status = APL_PRIMARY_SUCCESS;
     else if (st.n_steps[i] >= P.max_steps)
       status = APL_MAX_STEPS_REACHED;
+    else if (P.stall_steps > 0 && st.n_steps[i] - st.best_step[i] >= P.stall_steps &&
+             st.gbest[i] <= 1e3 * P.rtol * st.gnorm0[i])
+      status = APL_STAGNATION;  // within 3 decades of the target: the rounding floor of the gradient
     st.status[i] = status;
     if (status == APL_RUNNING) {
       const double pm = tot[D_HCNT] > 0.0 ? tot[D_HSUM] / tot[D_HCNT] : 1.0;
@@ -610,6 +621,7 @@ static Params to_params(const apl_pncg_params* p) {
   P.beta_max = p->beta_max;
   P.max_steps = p->max_steps;
   P.max_backtracking = p->max_backtracking;
+  P.stall_steps = p->stall_steps;
   return P;
 }
 
@@ -896,6 +908,7 @@ int apl_solver_create(apl_mesh* m, int material, apl_solver** out) {
   s->st.pmean = d + 7 * n;
   s->st.f_first = d + 8 * n;
   s->st.pHp = d + 9 * n;
+  s->st.gbest = d + 10 * n;
   int32_t* ii = s->si;
   s->st.status = ii + 0 * n;
   s->st.act_eval = ii + 1 * n;
@@ -907,6 +920,7 @@ int apl_solver_create(apl_mesh* m, int material, apl_solver** out) {
   s->st.n_backtracks = ii + 7 * n;
   s->st.phase = ii + 8 * n;
   s->st.restarts = ii + 9 * n;
+  s->st.best_step = ii + 10 * n;
   *out = s;
   return 0;
 }

@@ -69,6 +69,7 @@ class PncgConfig:
     iters_per_launch: int = 16
     use_graph: bool = True
     grid: int = 0
+    stall_steps: int = 2000  # STAGNATION: this many steps without a 10% better gradient norm near rtol
 
     def to_c(self, dtype: torch.dtype) -> _lib.PncgParams:
         eps = self.wolfe_eps
@@ -77,7 +78,7 @@ class PncgConfig:
         return _lib.PncgParams(
             self.rtol, self.atol, int(self.max_steps), int(self.max_backtracking), self.overstep, self.c1,
             self.wolfe_delta, eps, self.beta_max, int(self.iters_per_launch), 1 if self.use_graph else 0,
-            int(self.grid), 0,
+            int(self.grid), int(self.stall_steps),
         )
 
 

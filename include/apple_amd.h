@@ -106,12 +106,16 @@ int apl_bench_elem_pass(apl_mesh* mesh, int material, unsigned what, const void*
  * max_steps 1500 (forward/_forward.py:26-27), max_backtracking 8, overstep 1,
  * c1 1e-4, wolfe_delta 0.1, wolfe_eps 1e-6 (FP32) / 1e-12 (FP64), beta_max 10
  * (benches/bench_pncg_branching_backends.py:288-289), iters_per_launch 16,
- * use_graph 1, grid_override 0 (occupancy-derived persistent grid). */
+ * use_graph 1, grid_override 0 (occupancy-derived persistent grid),
+ * stall_steps 2000: an instance whose best relative gradient norm is within
+ * three decades of rtol and has not dropped by 10% within that many accepted
+ * steps ends with APL_STAGNATION (0 = never); this is where an FP32 solve
+ * stops when its gradient reaches the rounding floor of the element forces. */
 typedef struct {
   double rtol, atol;
   int32_t max_steps, max_backtracking;
   double overstep, c1, wolfe_delta, wolfe_eps, beta_max;
-  int32_t iters_per_launch, use_graph, grid_override, pad;
+  int32_t iters_per_launch, use_graph, grid_override, stall_steps;
 } apl_pncg_params;
 
 /* Per-instance result: status (APL_* code), accepted steps, energy

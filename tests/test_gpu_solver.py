@@ -184,6 +184,12 @@ def test_failure_codes():
     x = torch.as_tensor(x0, dtype=torch.float32, device="cuda").contiguous()
     st = solver.solve(x, ft, mask, PncgConfig(max_steps=3, rtol=1e-12))[0]
     assert st.result == Result.MAX_STEPS_REACHED and st.n_steps == 3
+    # FP32 cannot reach 1e-9: the gradient stops improving at its rounding
+    # floor and the stall test ends the solve well before max_steps
+    x = torch.as_tensor(x0, dtype=torch.float32, device="cuda").contiguous()
+    st = solver.solve(x, ft, mask, PncgConfig(max_steps=100000, rtol=1e-9, stall_steps=200))[0]
+    assert st.result == Result.STAGNATION and st.n_steps < 100000
+    assert st.relative_grad_norm < 1e-4
     bad = pot.materials["mu"].clone()
     bad[5] = float("nan")
     pot.set_materials(mu=bad)
