@@ -23,16 +23,20 @@
 //  * energies and quadratic forms are summed per thread, then in FP64 by a
 //    fixed wave-shuffle tree and a per-instance pass over the block sums;
 //  * the grid is persistent and XCD-aware: workgroup i runs on XCD i % 8, and
-//    XCD x walks the contiguous x-th eighth of the blocks, so neighbouring
-//    blocks (which share vertices) hit the same L2.
+//    XCD x walks every eighth chunk of up to 16 consecutive blocks, so
+//    neighbouring blocks (which share vertices) mostly hit the same L2 while
+//    any instance of more than 8 chunks spreads over all XCDs.
 #include "internal.h"
 #include "tetmath.h"
 #include "flatsum.h"
+#include "persist.h"
 #include "vsum.h"
 
 #include <type_traits>
 
 namespace apl {
+
+constexpr int XCD_CHUNK = 16;  // consecutive blocks per XCD chunk
 
 template <class R, class X>
 struct ElemArgs {
@@ -209,191 +213,208 @@ __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R, X> A) {
   const int tid = threadIdx.x;
   const int pid = blockIdx.x;
   const int xcd = pid & 7, k0 = pid >> 3, K = gridDim.x >> 3;
-  const long long nb = A.nblocks;
-  const int lo = (int)((xcd * nb) / 8), hi = (int)(((xcd + 1) * nb) / 8);
+  // XCD x owns the chunks x, x + 8, x + 16, ... of C consecutive blocks
+  // (C up to XCD_CHUNK, smaller for small meshes so that every XCD gets
+  // work): neighbouring blocks (which share vertices) mostly meet in one L2,
+  // and every instance larger than 8 chunks spreads over all XCDs (a batch
+  // whose few unconverged instances sat in one XCD's range would idle the
+  // others).  The XCD's j-th block is xcd_block(j).
+  const int nb = A.nblocks;
+  const int C = nb / 64 < 1 ? 1 : (nb / 64 > XCD_CHUNK ? XCD_CHUNK : nb / 64);  // >= 8 chunks per XCD if possible
+  const int nchunk = (nb + C - 1) / C;
+  const int my_chunks = xcd < nchunk ? (nchunk - 1 - xcd) / 8 + 1 : 0;
+  const bool owns_last = my_chunks > 0 && (nchunk - 1) % 8 == xcd;
+  const int nb_x = my_chunks * C - (owns_last ? nchunk * C - nb : 0);
+  auto xcd_block = [&](int j) { return ((j / C) * 8 + xcd) * C + j % C; };
 
-  for (int b = lo + k0; b < hi; b += K) {
-    const int inst = A.blk_inst[b];
-    if (A.active != nullptr && A.active[inst] == 0) continue;
-    const int t0 = A.blk_tet_off[b];
-    const int nt = A.blk_tet_off[b + 1] - t0;
-    const int sb = A.blk_loc_off[b];
-    const int nloc = A.blk_loc_off[b + 1] - sb;
-    const X alpha = A.alpha != nullptr ? (X)A.alpha[inst] : X(0);
-    const int lg = (4 * nloc <= SPT * NTH) ? 2 : ((2 * nloc <= SPT * NTH) ? 1 : 0);
-    const int tps = 1 << lg;
+  // This workgroup's blocks are xcd_block(k0 + j K), j < n_cand; in a solve
+  // only those of still-active instances.
+  const int n_cand = k0 < nb_x ? (nb_x - k0 + K - 1) / K : 0;
+  // the active-block list (solves only) sits after the contribution buffer
+  int* act_list = reinterpret_cast<int*>(contrib + (SCATTER ? NCH * CS : 0));
+  for_active<NTH>(
+      n_cand, A.active != nullptr, act_list, [&](int j) { return xcd_block(k0 + j * K); },
+      [&](int bb) { return A.active[A.blk_inst[bb]] != 0; }, [&](const int b) {
+      const int inst = A.blk_inst[b];
+      const int t0 = A.blk_tet_off[b];
+      const int nt = A.blk_tet_off[b + 1] - t0;
+      const int sb = A.blk_loc_off[b];
+      const int nloc = A.blk_loc_off[b + 1] - sb;
+      const X alpha = A.alpha != nullptr ? (X)A.alpha[inst] : X(0);
+      const int lg = (4 * nloc <= SPT * NTH) ? 2 : ((2 * nloc <= SPT * NTH) ? 1 : 0);
+      const int tps = 1 << lg;
 
-    // this thread's first tet record, loaded before the staging barrier so the
-    // two global-load latencies overlap
-    const BlockRes br = block_res<MAT>(A, t0);
-    TetRec<R> rec;
-    if (tid < nt) load_rec<MAT>(br, (unsigned)tid, rec);
+      // this thread's first tet record, loaded before the staging barrier so the
+      // two global-load latencies overlap
+      const BlockRes br = block_res<MAT>(A, t0);
+      TetRec<R> rec;
+      if (tid < nt) load_rec<MAT>(br, (unsigned)tid, rec);
 
-    // stage the block's vertex window (u = x + alpha p for a trial point)
-    const int32_t* gids = A.loc_gid + sb;
-    for (int s = tid; s < nloc; s += NTH) {
-      const unsigned g3 = 3u * (unsigned)at(gids, (unsigned)s);
-      X ux = at(A.x, g3), uy = at(A.x, g3 + 1), uz = at(A.x, g3 + 2);
-      R px = R(0), py = R(0), pz = R(0);
-      if (NEEDV || A.alpha != nullptr) {
-        px = at(A.p, g3);
-        py = at(A.p, g3 + 1);
-        pz = at(A.p, g3 + 2);
+      // stage the block's vertex window (u = x + alpha p for a trial point)
+      const int32_t* gids = A.loc_gid + sb;
+      for (int s = tid; s < nloc; s += NTH) {
+        const unsigned g3 = 3u * (unsigned)at(gids, (unsigned)s);
+        X ux = at(A.x, g3), uy = at(A.x, g3 + 1), uz = at(A.x, g3 + 2);
+        R px = R(0), py = R(0), pz = R(0);
+        if (NEEDV || A.alpha != nullptr) {
+          px = at(A.p, g3);
+          py = at(A.p, g3 + 1);
+          pz = at(A.p, g3 + 2);
+        }
+        if (A.alpha != nullptr) {
+          ux += alpha * (X)px;
+          uy += alpha * (X)py;
+          uz += alpha * (X)pz;
+        }
+        Vec4<X> u4 = {{ux, uy, uz, X(0)}};
+        *reinterpret_cast<Vec4<X>*>(u_lds + 4 * s) = u4;
+        if constexpr (NEEDV) {
+          Vec4<R> v4 = {{px, py, pz, R(0)}};
+          *reinterpret_cast<Vec4<R>*>(v_lds + 4 * s) = v4;
+        }
       }
-      if (A.alpha != nullptr) {
-        ux += alpha * (X)px;
-        uy += alpha * (X)py;
-        uz += alpha * (X)pz;
-      }
-      Vec4<X> u4 = {{ux, uy, uz, X(0)}};
-      *reinterpret_cast<Vec4<X>*>(u_lds + 4 * s) = u4;
-      if constexpr (NEEDV) {
-        Vec4<R> v4 = {{px, py, pz, R(0)}};
-        *reinterpret_cast<Vec4<R>*>(v_lds + 4 * s) = v4;
-      }
-    }
-    __syncthreads();
+      __syncthreads();
 
-    R acc[SPT][NCHA];
+      R acc[SPT][NCHA];
 #pragma unroll
-    for (int k = 0; k < SPT; ++k)
+      for (int k = 0; k < SPT; ++k)
 #pragma unroll
-      for (int ch = 0; ch < NCHA; ++ch) acc[k][ch] = R(0);
-    R e_acc = R(0), q_acc = R(0);
-    const uint16_t* co = A.chunk_off + A.blk_co_off[b];
-    const int nchunks = (nt + NTH - 1) / NTH;
-    for (int c = 0; c < nchunks; ++c) {
-      const int tl = c * NTH + tid;
-      // this chunk's position ranges of the thread's slot items
-      int q0[SPT], q1[SPT];
+        for (int ch = 0; ch < NCHA; ++ch) acc[k][ch] = R(0);
+      R e_acc = R(0), q_acc = R(0);
+      const uint16_t* co = A.chunk_off + A.blk_co_off[b];
+      const int nchunks = (nt + NTH - 1) / NTH;
+      for (int c = 0; c < nchunks; ++c) {
+        const int tl = c * NTH + tid;
+        // this chunk's position ranges of the thread's slot items
+        int q0[SPT], q1[SPT];
+        if constexpr (SCATTER) {
+          const uint16_t* cc = co + c * (nloc + 1);
+#pragma unroll
+          for (int k = 0; k < SPT; ++k) {
+            const int item = tid + k * NTH;
+            const int s = item >> lg;
+            q0[k] = 0;
+            q1[k] = 0;
+            if (s < nloc) {
+              q0[k] = at(cc, (unsigned)s) + (item & (tps - 1));
+              q1[k] = at(cc, (unsigned)s + 1);
+            }
+          }
+        }
+        if (tl < nt) {
+          if (c > 0) load_rec<MAT>(br, (unsigned)tl, rec);
+          const TetGeom<R> g = make_geom(rec.B, rec.dV);
+          const int sidx[4] = {(int)(rec.cr.x & 0xffffu), (int)(rec.cr.x >> 16), (int)(rec.cr.y & 0xffffu),
+                               (int)(rec.cr.y >> 16)};
+          // edge differences u_a - u_0 in the storage precision X, then R: F - I
+          // depends only on them, and forming them before rounding keeps F
+          // accurate relative to the strain rather than to |u|.
+          Vec4<X> uc[4];
+#pragma unroll
+          for (int a = 0; a < 4; ++a) uc[a] = *reinterpret_cast<const Vec4<X>*>(u_lds + 4 * sidx[a]);
+          V3<R> u[4], v[4];
+#pragma unroll
+          for (int a = 0; a < 4; ++a) {
+            u[a] = v3((R)(uc[a].v[0] - uc[0].v[0]), (R)(uc[a].v[1] - uc[0].v[1]), (R)(uc[a].v[2] - uc[0].v[2]));
+            if constexpr (NEEDV) {
+              const Vec4<R> vv = *reinterpret_cast<const Vec4<R>*>(v_lds + 4 * sidx[a]);
+              v[a] = v3(vv.v[0], vv.v[1], vv.v[2]);
+            } else {
+              v[a] = v3(R(0), R(0), R(0));
+            }
+          }
+          TetOut<R> o;
+          tet_eval<MAT, WHAT, R>(g, rec.mu, rec.lam, u, v, o);
+          if constexpr ((WHAT & W_E) != 0) e_acc += o.e;
+          if constexpr ((WHAT & W_HQ) != 0) q_acc += o.hq;
+          if constexpr ((WHAT & W_MIX) != 0) {
+            R* oa = A.out_a + t0;
+            R* ob = A.out_b + t0;
+            if (A.accumulate) {
+              at(oa, (unsigned)tl) += o.dmu;
+              if (MAT == MAT_SNH) at(ob, (unsigned)tl) += o.dlam;
+            } else {
+              at(oa, (unsigned)tl) = o.dmu;
+              if (MAT == MAT_SNH) at(ob, (unsigned)tl) = o.dlam;
+            }
+          }
+          if constexpr (SCATTER) {
+            const int pos[4] = {(int)(rec.cr.z & 0xffffu), (int)(rec.cr.z >> 16), (int)(rec.cr.w & 0xffffu),
+                                (int)(rec.cr.w >> 16)};
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+              R* cp = contrib + pos[a];
+              int ch = 0;
+              if constexpr (HAS_G) {
+                cp[(ch + 0) * CS] = o.g[a].x;
+                cp[(ch + 1) * CS] = o.g[a].y;
+                cp[(ch + 2) * CS] = o.g[a].z;
+                ch += 3;
+              }
+              if constexpr (HAS_H) {
+                cp[(ch + 0) * CS] = o.h[a].x;
+                cp[(ch + 1) * CS] = o.h[a].y;
+                cp[(ch + 2) * CS] = o.h[a].z;
+                ch += 3;
+              }
+              if constexpr (HAS_HP) {
+                cp[(ch + 0) * CS] = o.hp[a].x;
+                cp[(ch + 1) * CS] = o.hp[a].y;
+                cp[(ch + 2) * CS] = o.hp[a].z;
+              }
+            }
+          }
+        }
+        if constexpr (SCATTER) {
+          __syncthreads();
+#pragma unroll
+          for (int k = 0; k < SPT; ++k) {
+            for (int q = q0[k]; q < q1[k]; q += tps) {
+#pragma unroll
+              for (int ch = 0; ch < NCH; ++ch) acc[k][ch] += contrib[ch * CS + q];
+            }
+          }
+          __syncthreads();
+        }
+      }
+
       if constexpr (SCATTER) {
-        const uint16_t* cc = co + c * (nloc + 1);
+        // combine the lanes of a slot, then one partial per slot, at the
+        // slot's place in the vertex-major order
+        if (lg > 0) {
+#pragma unroll
+          for (int k = 0; k < SPT; ++k)
+#pragma unroll
+            for (int ch = 0; ch < NCH; ++ch) acc[k][ch] += __shfl_xor(acc[k][ch], 1, 64);
+        }
+        if (lg > 1) {
+#pragma unroll
+          for (int k = 0; k < SPT; ++k)
+#pragma unroll
+            for (int ch = 0; ch < NCH; ++ch) acc[k][ch] += __shfl_xor(acc[k][ch], 2, 64);
+        }
+        const int32_t* dsts = A.loc_dst + sb;
 #pragma unroll
         for (int k = 0; k < SPT; ++k) {
           const int item = tid + k * NTH;
           const int s = item >> lg;
-          q0[k] = 0;
-          q1[k] = 0;
-          if (s < nloc) {
-            q0[k] = at(cc, (unsigned)s) + (item & (tps - 1));
-            q1[k] = at(cc, (unsigned)s + 1);
+          if (s < nloc && (item & (tps - 1)) == 0) {
+            const unsigned j = (unsigned)at(dsts, (unsigned)s) * NCH;
+#pragma unroll
+            for (int ch = 0; ch < NCH; ++ch) at(A.partial, j + ch) = acc[k][ch];
           }
         }
       }
-      if (tl < nt) {
-        if (c > 0) load_rec<MAT>(br, (unsigned)tl, rec);
-        const TetGeom<R> g = make_geom(rec.B, rec.dV);
-        const int sidx[4] = {(int)(rec.cr.x & 0xffffu), (int)(rec.cr.x >> 16), (int)(rec.cr.y & 0xffffu),
-                             (int)(rec.cr.y >> 16)};
-        // edge differences u_a - u_0 in the storage precision X, then R: F - I
-        // depends only on them, and forming them before rounding keeps F
-        // accurate relative to the strain rather than to |u|.
-        Vec4<X> uc[4];
-#pragma unroll
-        for (int a = 0; a < 4; ++a) uc[a] = *reinterpret_cast<const Vec4<X>*>(u_lds + 4 * sidx[a]);
-        V3<R> u[4], v[4];
-#pragma unroll
-        for (int a = 0; a < 4; ++a) {
-          u[a] = v3((R)(uc[a].v[0] - uc[0].v[0]), (R)(uc[a].v[1] - uc[0].v[1]), (R)(uc[a].v[2] - uc[0].v[2]));
-          if constexpr (NEEDV) {
-            const Vec4<R> vv = *reinterpret_cast<const Vec4<R>*>(v_lds + 4 * sidx[a]);
-            v[a] = v3(vv.v[0], vv.v[1], vv.v[2]);
-          } else {
-            v[a] = v3(R(0), R(0), R(0));
-          }
-        }
-        TetOut<R> o;
-        tet_eval<MAT, WHAT, R>(g, rec.mu, rec.lam, u, v, o);
-        if constexpr ((WHAT & W_E) != 0) e_acc += o.e;
-        if constexpr ((WHAT & W_HQ) != 0) q_acc += o.hq;
-        if constexpr ((WHAT & W_MIX) != 0) {
-          R* oa = A.out_a + t0;
-          R* ob = A.out_b + t0;
-          if (A.accumulate) {
-            at(oa, (unsigned)tl) += o.dmu;
-            if (MAT == MAT_SNH) at(ob, (unsigned)tl) += o.dlam;
-          } else {
-            at(oa, (unsigned)tl) = o.dmu;
-            if (MAT == MAT_SNH) at(ob, (unsigned)tl) = o.dlam;
-          }
-        }
-        if constexpr (SCATTER) {
-          const int pos[4] = {(int)(rec.cr.z & 0xffffu), (int)(rec.cr.z >> 16), (int)(rec.cr.w & 0xffffu),
-                              (int)(rec.cr.w >> 16)};
-#pragma unroll
-          for (int a = 0; a < 4; ++a) {
-            R* cp = contrib + pos[a];
-            int ch = 0;
-            if constexpr (HAS_G) {
-              cp[(ch + 0) * CS] = o.g[a].x;
-              cp[(ch + 1) * CS] = o.g[a].y;
-              cp[(ch + 2) * CS] = o.g[a].z;
-              ch += 3;
-            }
-            if constexpr (HAS_H) {
-              cp[(ch + 0) * CS] = o.h[a].x;
-              cp[(ch + 1) * CS] = o.h[a].y;
-              cp[(ch + 2) * CS] = o.h[a].z;
-              ch += 3;
-            }
-            if constexpr (HAS_HP) {
-              cp[(ch + 0) * CS] = o.hp[a].x;
-              cp[(ch + 1) * CS] = o.hp[a].y;
-              cp[(ch + 2) * CS] = o.hp[a].z;
-            }
-          }
+      if constexpr (RED) {
+        double e = (double)e_acc, q = (double)q_acc;
+        block_sum2(e, q, red);
+        if (tid == 0) {
+          A.blk_red[2 * b] = e;
+          A.blk_red[2 * b + 1] = q;
         }
       }
-      if constexpr (SCATTER) {
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < SPT; ++k) {
-          for (int q = q0[k]; q < q1[k]; q += tps) {
-#pragma unroll
-            for (int ch = 0; ch < NCH; ++ch) acc[k][ch] += contrib[ch * CS + q];
-          }
-        }
-        __syncthreads();
-      }
-    }
-
-    if constexpr (SCATTER) {
-      // combine the lanes of a slot, then one partial per slot, at the
-      // slot's place in the vertex-major order
-      if (lg > 0) {
-#pragma unroll
-        for (int k = 0; k < SPT; ++k)
-#pragma unroll
-          for (int ch = 0; ch < NCH; ++ch) acc[k][ch] += __shfl_xor(acc[k][ch], 1, 64);
-      }
-      if (lg > 1) {
-#pragma unroll
-        for (int k = 0; k < SPT; ++k)
-#pragma unroll
-          for (int ch = 0; ch < NCH; ++ch) acc[k][ch] += __shfl_xor(acc[k][ch], 2, 64);
-      }
-      const int32_t* dsts = A.loc_dst + sb;
-#pragma unroll
-      for (int k = 0; k < SPT; ++k) {
-        const int item = tid + k * NTH;
-        const int s = item >> lg;
-        if (s < nloc && (item & (tps - 1)) == 0) {
-          const unsigned j = (unsigned)at(dsts, (unsigned)s) * NCH;
-#pragma unroll
-          for (int ch = 0; ch < NCH; ++ch) at(A.partial, j + ch) = acc[k][ch];
-        }
-      }
-    }
-    if constexpr (RED) {
-      double e = (double)e_acc, q = (double)q_acc;
-      block_sum2(e, q, red);
-      if (tid == 0) {
-        A.blk_red[2 * b] = e;
-        A.blk_red[2 * b + 1] = q;
-      }
-    }
-    __syncthreads();
-  }
+      __syncthreads();
+      });
 }
 
 // Finish pass of an element pass, one launch: workgroups [0, ninst) sum one
@@ -507,20 +528,21 @@ static ElemArgs<R, X> make_args(apl_mesh* m, const void* mu, const void* lam, co
 }
 
 template <int MAT, unsigned WHAT, class R, class X>
-static size_t smem_bytes(const apl_mesh* m) {
+static size_t smem_bytes(const apl_mesh* m, bool filtered) {
   constexpr bool NEEDV = (WHAT & (W_HP | W_HQ | W_MIX)) != 0;
   constexpr int NCH = ((WHAT & W_G) ? 3 : 0) + ((WHAT & W_H) ? 3 : 0) + ((WHAT & W_HP) ? 3 : 0);
   size_t n = 8 * sizeof(double) + 4 * (size_t)m->L.nloc_max * (sizeof(X) + (NEEDV ? sizeof(R) : 0));
   n += sizeof(R) * (size_t)NCH * 4 * NTH;
+  if (filtered) n += sizeof(int) * NTH;  // active-block list
   return n;
 }
 
 template <int MAT, unsigned WHAT, class R, class X, int SPT>
-static int grid_for_spt(apl_mesh* m) {
+static int grid_for_spt(apl_mesh* m, bool filtered) {
   static_assert(WHAT < 64, "pass variant index");
-  int& cached = m->grid_cache[MAT][WHAT + (sizeof(X) != sizeof(R) ? 64 : 0)];
+  int& cached = m->grid_cache[MAT][WHAT + (sizeof(X) != sizeof(R) ? 64 : 0) + (filtered ? 128 : 0)];
   if (cached > 0) return cached;
-  const size_t smem = smem_bytes<MAT, WHAT, R, X>(m);
+  const size_t smem = smem_bytes<MAT, WHAT, R, X>(m, filtered);
   int occ = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, elem_kernel<MAT, WHAT, R, X, SPT>, NTH, smem) != hipSuccess ||
       occ < 1)
@@ -538,13 +560,13 @@ static int grid_for_spt(apl_mesh* m) {
 inline int spt_for(const apl_mesh* m) { return m->L.nloc_max <= NTH ? 1 : (m->L.nloc_max <= 2 * NTH ? 2 : 4); }
 
 template <int MAT, unsigned WHAT, class R, class X>
-static int grid_for(apl_mesh* m) {
+static int grid_for(apl_mesh* m, bool filtered) {
   constexpr bool SCATTER = (WHAT & (W_G | W_H | W_HP)) != 0;
-  if (!SCATTER) return grid_for_spt<MAT, WHAT, R, X, 1>(m);
+  if (!SCATTER) return grid_for_spt<MAT, WHAT, R, X, 1>(m, filtered);
   const int spt = spt_for(m);
-  if (spt == 1) return grid_for_spt<MAT, WHAT, R, X, 1>(m);
-  if (spt == 2) return grid_for_spt<MAT, WHAT, R, X, 2>(m);
-  return grid_for_spt<MAT, WHAT, R, X, 4>(m);
+  if (spt == 1) return grid_for_spt<MAT, WHAT, R, X, 1>(m, filtered);
+  if (spt == 2) return grid_for_spt<MAT, WHAT, R, X, 2>(m, filtered);
+  return grid_for_spt<MAT, WHAT, R, X, 4>(m, filtered);
 }
 
 template <int MAT, unsigned WHAT, class R, class X>
@@ -552,12 +574,13 @@ static int launch_one(apl_mesh* m, const void* mu, const void* lam, const void* 
                       const double* alpha, const int32_t* active, void* out_a, void* out_b,
                       int accumulate, int grid_override, hipStream_t s) {
   if (m->L.nblocks == 0) return 0;
-  const size_t smem = smem_bytes<MAT, WHAT, R, X>(m);
+  const bool filtered = active != nullptr;
+  const size_t smem = smem_bytes<MAT, WHAT, R, X>(m, filtered);
   if (smem > 65536) {
     set_error("element block needs more than 64 KiB of LDS; rebuild the mesh with a smaller nloc_cap");
     return -1;
   }
-  const int grid = grid_override > 0 ? ((grid_override + 7) / 8) * 8 : grid_for<MAT, WHAT, R, X>(m);
+  const int grid = grid_override > 0 ? ((grid_override + 7) / 8) * 8 : grid_for<MAT, WHAT, R, X>(m, filtered);
   ElemArgs<R, X> A = make_args<R, X>(m, mu, lam, x, p, alpha, active, out_a, out_b, accumulate);
   constexpr bool SCATTER = (WHAT & (W_G | W_H | W_HP)) != 0;
   const int spt = SCATTER ? spt_for(m) : 1;
@@ -620,21 +643,23 @@ int launch_elem(apl_mesh* m, int material, unsigned what, const void* mu, const 
 }
 
 template <int MAT>
-static int grid_mat(apl_mesh* m, unsigned what, int x_f64) {
+static int grid_mat(apl_mesh* m, unsigned what, int x_f64, bool filtered) {
   if (x_f64 && m->dtype == DT_F32) {
-    if (what == (W_E | W_G | W_H)) return grid_for<MAT, W_E | W_G | W_H, float, double>(m);
-    if (what == W_HQ) return grid_for<MAT, W_HQ, float, double>(m);
+    if (what == (W_E | W_G | W_H)) return grid_for<MAT, W_E | W_G | W_H, float, double>(m, filtered);
+    if (what == W_HQ) return grid_for<MAT, W_HQ, float, double>(m, filtered);
     return 0;
   }
 #define APL_CASE(W) \
-  if (what == (W)) return m->dtype == DT_F32 ? grid_for<MAT, (W), float, float>(m) : grid_for<MAT, (W), double, double>(m);
+  if (what == (W)) \
+    return m->dtype == DT_F32 ? grid_for<MAT, (W), float, float>(m, filtered) : grid_for<MAT, (W), double, double>(m, filtered);
   APL_WHAT_LIST(APL_CASE)
 #undef APL_CASE
   return 0;
 }
 
-int elem_grid(apl_mesh* m, int material, unsigned what, int x_f64) {
-  return material == MAT_SNH ? grid_mat<MAT_SNH>(m, what, x_f64) : grid_mat<MAT_ARAP>(m, what, x_f64);
+int elem_grid(apl_mesh* m, int material, unsigned what, int x_f64, bool filtered) {
+  return material == MAT_SNH ? grid_mat<MAT_SNH>(m, what, x_f64, filtered)
+                             : grid_mat<MAT_ARAP>(m, what, x_f64, filtered);
 }
 
 int launch_finish(apl_mesh* m, unsigned what, void* out_a, void* out_b, double* out_e, double* out_q,

@@ -61,7 +61,7 @@ struct apl_mesh {
   apl::Layout L;
   apl::DevLayout d;
   size_t device_bytes = 0;
-  int grid_cache[2][128] = {};  // persistent grid per (material, pass variant, FP64 displacements)
+  int grid_cache[2][256] = {};  // persistent grid per (material, pass variant, FP64 displacements, filtered)
   int refs = 1;                 // the caller's handle plus one per solver built on the mesh
 };
 
@@ -92,6 +92,6 @@ int launch_elem(apl_mesh* m, int material, unsigned what, const void* mu, const 
 // quadratic forms.
 int launch_finish(apl_mesh* m, unsigned what, void* out_a, void* out_b, double* out_e, double* out_q,
                   int accumulate, const int32_t* active, hipStream_t s);
-int elem_grid(apl_mesh* m, int material, unsigned what, int x_f64 = 0);
+int elem_grid(apl_mesh* m, int material, unsigned what, int x_f64 = 0, bool filtered = false);
 
 }  // namespace apl

@@ -32,6 +32,7 @@
 #include "../../include/apple_amd.h"
 #include "internal.h"
 #include "flatsum.h"
+#include "persist.h"
 #include "vsum.h"
 #include "tetmath.h"
 
@@ -106,6 +107,15 @@ __device__ inline void energy_share(const double* blk_red, const int64_t* inst_b
   for (long long b = lo + threadIdx.x; b < hi; b += NTH) acc += blk_red[2 * b];
 }
 
+// Persistent loop of a vector kernel over its vertex blocks blockIdx.x +
+// j gridDim.x (compacted to the active ones in a batch).
+template <class IsActive, class Body>
+__device__ inline void vertex_blocks(int nvblk, bool filter, int* list, IsActive is_active, Body body) {
+  const int b0 = blockIdx.x, G = gridDim.x;
+  const int n_cand = b0 < nvblk ? (nvblk - b0 + G - 1) / G : 0;
+  for_active<NTH>(n_cand, filter, list, [&](int j) { return b0 + j * G; }, is_active, body);
+}
+
 template <class R>
 struct VecArgs {
   const int32_t* vblk_off;
@@ -124,6 +134,8 @@ struct VecArgs {
   const int64_t* inst_blk_off;
   const int64_t* inst_vblk_off;
   StatePtrs st;
+  int nvblk;   // vertex blocks of the mesh
+  int filter;  // visit only blocks of active instances (batches)
 };
 
 template <class R>
@@ -168,10 +180,7 @@ __global__ void pncg_init_kernel(StatePtrs st, int n) {
 // block (one DOF per thread and stride, coalesced), plus the block's share of
 // the instance's element-block energies
 template <class R>
-__global__ __launch_bounds__(NTH) void pncg_reduce_kernel(VecArgs<R> A) {
-  __shared__ double sm[(NTH / 64) * N_DOTS];
-  __shared__ R win[VSUM_LDS_BYTES / sizeof(R)];
-  const int vb = blockIdx.x;
+__device__ inline void pncg_reduce_block(const VecArgs<R>& A, const int vb, double* sm, R* win) {
   const int inst = A.vblk_inst[vb];
   if (A.st.act_eval[inst] == 0) return;
   const int v0 = A.vblk_off[vb], v1 = A.vblk_off[vb + 1];
@@ -238,6 +247,15 @@ __global__ __launch_bounds__(NTH) void pncg_reduce_kernel(VecArgs<R> A) {
   }
   energy_share(A.blk_red, A.inst_blk_off, A.inst_vblk_off, inst, vb, d[D_E]);
   block_dots(d, sm, A.vblk_dots + (long long)vb * N_DOTS);
+}
+
+template <class R>
+__global__ __launch_bounds__(NTH) void pncg_reduce_kernel(VecArgs<R> A) {
+  __shared__ double sm[(NTH / 64) * N_DOTS];
+  __shared__ R win[VSUM_LDS_BYTES / sizeof(R)];
+  __shared__ int list[NTH];
+  vertex_blocks(A.nvblk, A.filter, list, [&](int vb) { return A.st.act_eval[A.vblk_inst[vb]] != 0; },
+                [&](int vb) { pncg_reduce_block<R>(A, vb, sm, win); });
 }
 
 // S: per-instance decision
@@ -352,8 +370,7 @@ __global__ __launch_bounds__(SCAL_TH) void pncg_scalar_kernel(StatePtrs st, Para
 
 // D: commit and new direction
 template <class R>
-__global__ __launch_bounds__(NTH) void pncg_commit_kernel(VecArgs<R> A) {
-  const int vb = blockIdx.x;
+__device__ inline void pncg_commit_block(const VecArgs<R>& A, const int vb) {
   const int inst = A.vblk_inst[vb];
   if (A.st.commit[inst] == 0) return;
   const double a = A.st.alpha_commit[inst];
@@ -377,6 +394,13 @@ __global__ __launch_bounds__(NTH) void pncg_commit_kernel(VecArgs<R> A) {
       A.p[i] = -Pc * gt + beta * pc;
     }
   }
+}
+
+template <class R>
+__global__ __launch_bounds__(NTH) void pncg_commit_kernel(VecArgs<R> A) {
+  __shared__ int list[NTH];
+  vertex_blocks(A.nvblk, A.filter, list, [&](int vb) { return A.st.commit[A.vblk_inst[vb]] != 0; },
+                [&](int vb) { pncg_commit_block<R>(A, vb); });
 }
 
 // S2: alpha from the quadratic form; clears the per-iteration flags
@@ -427,16 +451,15 @@ struct CgArgs {
   R* hd;
   double* vblk_dots;
   StatePtrs st;
+  int nvblk;   // vertex blocks of the mesh
+  int filter;  // visit only blocks of active instances (batches)
 };
 
 enum CgStage : int { CG_DIAG = 0, CG_PRECOND = 1, CG_Q = 2, CG_UPDATE = 3, CG_DIR = 4 };
 
 template <class R, int STAGE>
-__global__ __launch_bounds__(NTH) void cg_vec_kernel(CgArgs<R> A) {
-  __shared__ double sm[(NTH / 64) * N_DOTS];
+__device__ inline void cg_vec_block(const CgArgs<R>& A, const int vb, double* sm, R* win) {
   constexpr bool SUMS = STAGE == CG_DIAG || STAGE == CG_Q;
-  __shared__ R win[SUMS ? VSUM_LDS_BYTES / sizeof(R) : 1];
-  const int vb = blockIdx.x;
   const int inst = A.vblk_inst[vb];
   if (STAGE != CG_DIAG && STAGE != CG_PRECOND && A.st.act_eval[inst] == 0) return;
   const int v0 = A.vblk_off[vb], v1 = A.vblk_off[vb + 1];
@@ -506,6 +529,17 @@ __global__ __launch_bounds__(NTH) void cg_vec_kernel(CgArgs<R> A) {
     }
   }
   if constexpr (STAGE != CG_DIR) block_dots(dd, sm, A.vblk_dots + (long long)vb * N_DOTS);
+}
+
+template <class R, int STAGE>
+__global__ __launch_bounds__(NTH) void cg_vec_kernel(CgArgs<R> A) {
+  __shared__ double sm[(NTH / 64) * N_DOTS];
+  constexpr bool SUMS = STAGE == CG_DIAG || STAGE == CG_Q;
+  __shared__ R win[SUMS ? VSUM_LDS_BYTES / sizeof(R) : 1];
+  __shared__ int list[NTH];
+  constexpr bool ALL = STAGE == CG_DIAG || STAGE == CG_PRECOND;  // run before the status is known
+  vertex_blocks(A.nvblk, A.filter && !ALL, list, [&](int vb) { return A.st.act_eval[A.vblk_inst[vb]] != 0; },
+                [&](int vb) { cg_vec_block<R, STAGE>(A, vb, sm, win); });
 }
 
 // per-instance CG scalars; stage 0: after DIAG, 1: after PRECOND, 2: after Q, 3: after UPDATE
@@ -607,6 +641,8 @@ static VecArgs<R> vec_args(apl_solver* s, double* x, const void* fext, const uin
   A.inst_blk_off = m->d.inst_blk_off;
   A.inst_vblk_off = m->d.inst_vblk_off;
   A.st = s->st;
+  A.nvblk = (int)m->L.vblk_inst.size();
+  A.filter = m->L.n_inst > 1 ? 1 : 0;
   return A;
 }
 
@@ -625,6 +661,14 @@ static Params to_params(const apl_pncg_params* p) {
   return P;
 }
 
+// Persistent grid of the vector kernels: every vertex block its own
+// workgroup up to sixteen per CU, beyond that a grid-stride loop.
+static unsigned vec_grid(const apl_mesh* m) {
+  const long long nvb = (long long)m->L.vblk_inst.size();
+  const long long cap = 16LL * m->n_cu;
+  return (unsigned)(nvb < cap ? nvb : cap);
+}
+
 template <class R>
 static int pncg_iteration(apl_solver* s, const void* mu, const void* lam, double* x, const void* fext,
                           const uint8_t* mask, const Params& P, int grid, hipStream_t st) {
@@ -636,9 +680,10 @@ static int pncg_iteration(apl_solver* s, const void* mu, const void* lam, double
   if (launch_elem(m, s->material, W_E | W_G | W_H, mu, lam, x, A.p, s->st.alpha, s->st.act_eval, A.gE, A.hd, 0,
                   grid, st, x_f64))
     return -1;
-  if (nvb) hipLaunchKernelGGL((pncg_reduce_kernel<R>), dim3(nvb), dim3(NTH), 0, st, A);
+  const unsigned vg = vec_grid(m);
+  if (nvb) hipLaunchKernelGGL((pncg_reduce_kernel<R>), dim3(vg), dim3(NTH), 0, st, A);
   hipLaunchKernelGGL(pncg_scalar_kernel, dim3(ni), dim3(SCAL_TH), 0, st, s->st, P, m->d.vblk_dots, m->d.inst_vblk_off);
-  if (nvb) hipLaunchKernelGGL((pncg_commit_kernel<R>), dim3(nvb), dim3(NTH), 0, st, A);
+  if (nvb) hipLaunchKernelGGL((pncg_commit_kernel<R>), dim3(vg), dim3(NTH), 0, st, A);
   if (launch_elem(m, s->material, W_HQ, mu, lam, x, A.p, nullptr, s->st.act_dir, nullptr, nullptr, 0, grid, st,
                   x_f64))
     return -1;
@@ -679,8 +724,8 @@ static int pncg_solve_t(apl_solver* s, const void* mu, const void* lam, void* x,
   }
   const int grid = prm->grid_override;
   // occupancy-derived grids are queried (and cached) outside any capture
-  (void)elem_grid(m, s->material, W_E | W_G | W_H, x_f64);
-  (void)elem_grid(m, s->material, W_HQ, x_f64);
+  (void)elem_grid(m, s->material, W_E | W_G | W_H, x_f64, true);
+  (void)elem_grid(m, s->material, W_HQ, x_f64, true);
   // cached graph of K iterations for these pointers and parameters
   bool use_graph = prm->use_graph != 0;
   if (use_graph) {
@@ -785,6 +830,8 @@ static CgArgs<R> cg_args(apl_solver* s, const void* rhs, void* y, const uint8_t*
   A.hd = base + 8 * nv;
   A.vblk_dots = m->d.vblk_dots;
   A.st = s->st;
+  A.nvblk = (int)s->mesh->L.vblk_inst.size();
+  A.filter = s->mesh->L.n_inst > 1 ? 1 : 0;
   return A;
 }
 
@@ -795,13 +842,13 @@ static int cg_iteration(apl_solver* s, const void* mu, const void* lam, const vo
   const unsigned nvb = (unsigned)m->L.vblk_inst.size();
   const unsigned ni = (unsigned)m->L.n_inst;
   if (launch_elem(m, s->material, W_HP, mu, lam, x, A.d, nullptr, s->st.act_eval, A.q, nullptr, 0, 0, st)) return -1;
-  if (nvb) hipLaunchKernelGGL((cg_vec_kernel<R, CG_Q>), dim3(nvb), dim3(NTH), 0, st, A);
+  if (nvb) hipLaunchKernelGGL((cg_vec_kernel<R, CG_Q>), dim3(vec_grid(m)), dim3(NTH), 0, st, A);
   hipLaunchKernelGGL(cg_scalar_kernel, dim3(ni), dim3(SCAL_TH), 0, st, s->st, 2, rtol, max_iters, m->d.vblk_dots,
                      m->d.inst_vblk_off);
-  if (nvb) hipLaunchKernelGGL((cg_vec_kernel<R, CG_UPDATE>), dim3(nvb), dim3(NTH), 0, st, A);
+  if (nvb) hipLaunchKernelGGL((cg_vec_kernel<R, CG_UPDATE>), dim3(vec_grid(m)), dim3(NTH), 0, st, A);
   hipLaunchKernelGGL(cg_scalar_kernel, dim3(ni), dim3(SCAL_TH), 0, st, s->st, 3, rtol, max_iters, m->d.vblk_dots,
                      m->d.inst_vblk_off);
-  if (nvb) hipLaunchKernelGGL((cg_vec_kernel<R, CG_DIR>), dim3(nvb), dim3(NTH), 0, st, A);
+  if (nvb) hipLaunchKernelGGL((cg_vec_kernel<R, CG_DIR>), dim3(vec_grid(m)), dim3(NTH), 0, st, A);
   APL_CHECK_HIP(hipGetLastError());
   return 0;
 }
@@ -817,14 +864,14 @@ static int cg_solve_t(apl_solver* s, const void* mu, const void* lam, const void
   // zero the diagonal buffer (vertices with no tet keep 0) and run the Hdiag pass at x
   if (fill<R>(A.hd, 3 * (long long)m->L.V, R(0), st)) return -1;
   if (launch_elem(m, s->material, W_H, mu, lam, x, nullptr, nullptr, nullptr, nullptr, A.hd, 0, 0, st)) return -1;
-  if (nvb) hipLaunchKernelGGL((cg_vec_kernel<R, CG_DIAG>), dim3(nvb), dim3(NTH), 0, st, A);
+  if (nvb) hipLaunchKernelGGL((cg_vec_kernel<R, CG_DIAG>), dim3(vec_grid(m)), dim3(NTH), 0, st, A);
   hipLaunchKernelGGL(cg_scalar_kernel, dim3(n), dim3(SCAL_TH), 0, st, s->st, 0, rtol, max_iters, m->d.vblk_dots,
                      m->d.inst_vblk_off);
-  if (nvb) hipLaunchKernelGGL((cg_vec_kernel<R, CG_PRECOND>), dim3(nvb), dim3(NTH), 0, st, A);
+  if (nvb) hipLaunchKernelGGL((cg_vec_kernel<R, CG_PRECOND>), dim3(vec_grid(m)), dim3(NTH), 0, st, A);
   hipLaunchKernelGGL(cg_scalar_kernel, dim3(n), dim3(SCAL_TH), 0, st, s->st, 1, rtol, max_iters, m->d.vblk_dots,
                      m->d.inst_vblk_off);
   APL_CHECK_HIP(hipGetLastError());
-  (void)elem_grid(m, s->material, W_HP);
+  (void)elem_grid(m, s->material, W_HP, 0, true);
   const int K = 16;
   long long done = 0;
   while (true) {
