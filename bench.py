@@ -62,8 +62,9 @@ def make_cube_problem(n, material, dtype, device, seed_mu=1, seed_noise=0, noise
     return dict(mesh=m, region=region, pot=pot, f=t(f), fixed=fixed, x0=t(x0), mu=mu)
 
 
-def bench_pass(pot, u, what, iters):
-    """Mean element-kernel ms and full-pass ms over `iters` launches (HIP events)."""
+def bench_pass(pot, u, what, iters, kernel=True, full=True):
+    """(mean element-kernel ms, mean full-pass ms) over `iters` back-to-back
+    launches each, timed with HIP events on the stream (None when skipped)."""
     import torch
 
     from apple_amd import _lib
@@ -81,11 +82,11 @@ def bench_pass(pot, u, what, iters):
     _lib.check(
         _lib.lib().apl_bench_elem_pass(
             m._handle, int(pot.material), what, ptr(pot.materials["mu"]), ptr(lam) if lam is not None else None,
-            ptr(u), ptr(p), ptr(e), ptr(a), ptr(b), int(iters), ctypes.byref(k), ctypes.byref(tp),
-            current_stream(m.device),
+            ptr(u), ptr(p), ptr(e), ptr(a), ptr(b), int(iters), ctypes.byref(k) if kernel else None,
+            ctypes.byref(tp) if full else None, current_stream(m.device),
         )
     )
-    return k.value, tp.value
+    return (k.value if kernel else None), (tp.value if full else None)
 
 
 def cpu_baseline(prob, target_s=12.0, threads=1):
@@ -119,11 +120,45 @@ def cpu_baseline(prob, target_s=12.0, threads=1):
 
 
 def load_traffic(name):
+    """HBM bytes per launch measured by rocprofv3 (FETCH_SIZE and WRITE_SIZE
+    passes, scaled by the copy-kernel calibration); the summary lives under
+    profiles/ as text with a line 'bytes_per_launch: <int>'."""
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f)
+        for line in f:
+            if line.strip().startswith("bytes_per_launch:"):
+                return int(line.split(":", 1)[1].strip())
+    return None
+
+
+def bench_solve(prob, material, dev, rtol=1e-6):
+    """One timed PNCG solve of a cube problem after a short graph warm-up."""
+    import torch
+
+    from apple_amd.solver import DeviceSolver, PncgConfig, mask_bits
+
+    solver = DeviceSolver(prob["pot"])
+    mask = mask_bits(prob["fixed"], dev)
+    xw = prob["x0"].clone()
+    solver.solve(xw, prob["f"], mask, PncgConfig(rtol=rtol, max_steps=16))
+    x = prob["x0"].clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = solver.solve(x, prob["f"], mask, PncgConfig(rtol=rtol, max_steps=20000))[0]
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    log(f"[bench] PNCG {material}: {st.result.name} in {st.n_steps} steps, {dt:.3f} s")
+    return {
+        "solves_per_s": round(1.0 / dt, 3),
+        "solve_s": round(dt, 4),
+        "result": st.result.name,
+        "steps": st.n_steps,
+        "evals": st.n_evals,
+        "relative_grad_norm": st.relative_grad_norm,
+        "us_per_iteration": round(dt / max(st.n_evals, 1) * 1e6, 2),
+    }
 
 
 def main():
@@ -139,8 +174,11 @@ def main():
     ap.add_argument("--chunks", type=int, default=1, help="tet chunks of 256 per element block")
     ap.add_argument("--batched-only", action="store_true", help="only the batched forward+adjoint comparison")
     ap.add_argument("--batched-meshes", type=int, default=256)
+    ap.add_argument("--cpu-threads", type=int, default=16, help="threads of the multi-threaded CPU baseline")
     ap.add_argument("--profile-pass", action="store_true",
                     help="only run pass-A launches and a reference copy (for rocprofv3 counter runs)")
+    ap.add_argument("--profile-solve", type=int, default=0, metavar="STEPS",
+                    help="only run a PNCG solve of STEPS iterations (for rocprofv3 kernel traces)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (sharding is described in DESIGN.md)"}))
@@ -149,7 +187,6 @@ def main():
     import torch
 
     from apple_amd import _lib
-    from apple_amd.solver import DeviceSolver, PncgConfig, mask_bits
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
@@ -177,27 +214,43 @@ def main():
         torch.cuda.synchronize()
         log("[bench] profile-pass done")
         return
+    if args.profile_solve:
+        from apple_amd.solver import DeviceSolver, PncgConfig, mask_bits
+
+        solver = DeviceSolver(pot)
+        mask = mask_bits(prob["fixed"], dev)
+        x = prob["x0"].clone()
+        solver.solve(x, prob["f"], mask, PncgConfig(rtol=1e-6, max_steps=16))
+        x = prob["x0"].clone()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = solver.solve(x, prob["f"], mask, PncgConfig(rtol=1e-6, max_steps=args.profile_solve))[0]
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        log(f"[bench] profile-solve: {st.n_evals} evaluations in {dt * 1e3:.1f} ms")
+        return
 
     # ---------------------------------------------------------------- pass A (headline)
     log(f"[bench] warmup {args.warmup}, timed {args.steps} steps of pass A")
     bench_pass(pot, u, what_a, max(args.warmup, 1))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    ms_kernel, ms_pass = bench_pass(pot, u, what_a, args.steps)
+    _, ms_pass = bench_pass(pot, u, what_a, args.steps, kernel=False)  # the timed steps: whole passes
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    ms_kernel, _ = bench_pass(pot, u, what_a, args.steps, full=False)  # the element kernel alone
     ms_per_step = wall / args.steps * 1e3
     value = T / (ms_per_step / 1e3) / 1e6
     bytes_a = 64 * T + 36 * V
     achieved = bytes_a / (ms_kernel / 1e3) / 1e9
-    traffic = load_traffic("traffic_pass_a_snh64.json")
+    traffic = load_traffic("pass_a_snh64_traffic.md") if n == 64 else None
     roofline = {
         "bound": "hbm",
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": traffic.get("bytes_per_launch") if (traffic and n == 64) else None,
+        "traffic": traffic,
         "kernel": "elem_kernel<SNH, E|G|H, float>",
         "kernel_ms": round(ms_kernel, 5),
         "pass_ms": round(ms_pass, 5),
@@ -219,7 +272,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"pass A (energy+grad+Hdiag), stable Neo-Hookean, {n}^3 cube x 5 tets, "
+            "workload": f"pass A (energy+grad+Hdiag incl. vertex sums), stable Neo-Hookean, {n}^3 cube x 5 tets, "
             "mu lognormal(1e4, 0.5), lambda = 10 mu, seeded noisy displacement",
             "n_tets": T,
             "n_verts": V,
@@ -242,6 +295,7 @@ def main():
         "kernel_ms": round(k_hp, 5),
         "hbm_frac": round((64 * T + 36 * V) / (k_hp / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
     }
+    aprob = None
     if not args.quick:
         log("[bench] ARAP pass A")
         aprob = make_cube_problem(n, "arap", dtype, dev)
@@ -251,41 +305,22 @@ def main():
             "kernel_ms": round(ka, 5),
             "hbm_frac": round((60 * T + 36 * V) / (ka / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
         }
-        del aprob
 
-    # ---------------------------------------------------------------- PNCG solve
+    # ---------------------------------------------------------------- PNCG solves (config 2: SNH and ARAP)
     if not (args.quick or args.no_solve):
-        log("[bench] PNCG solve (SNH, rtol 1e-6)")
-        solver = DeviceSolver(pot)
-        mask = mask_bits(prob["fixed"], dev)
-        cfg = PncgConfig(rtol=1e-6, max_steps=20000)
-        xw = prob["x0"].clone()
-        solver.solve(xw, prob["f"], mask, PncgConfig(rtol=1e-6, max_steps=16))  # graph warmup
-        x = prob["x0"].clone()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        st = solver.solve(x, prob["f"], mask, cfg)[0]
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        out["pncg"] = {
-            "solves_per_s": round(1.0 / dt, 3),
-            "solve_s": round(dt, 4),
-            "result": st.result.name,
-            "steps": st.n_steps,
-            "evals": st.n_evals,
-            "relative_grad_norm": st.relative_grad_norm,
-            "ms_per_iteration": round(dt / max(st.n_evals, 1) * 1e3, 4),
-        }
-        log(f"[bench] PNCG: {st.result.name} in {st.n_steps} steps, {dt:.3f} s")
+        out["pncg_snh"] = bench_solve(prob, "snh", dev)
+        out["pncg_arap"] = bench_solve(aprob, "arap", dev)
+    del aprob
 
-    # ---------------------------------------------------------------- batched vs plain
+    # ---------------------------------------------------------------- batched (+adjoint) vs plain
     if not (args.quick or args.no_batch):
         out["batched"] = bench_batched(dev, dtype, n_meshes=args.batched_meshes)
 
     # ---------------------------------------------------------------- CPU baseline
     if not (args.quick or args.no_cpu):
-        log("[bench] CPU baseline (C restatement, 1 thread)")
-        out["cpu_baseline"] = cpu_baseline(prob, threads=1)
+        log(f"[bench] CPU baseline (C restatement, {args.cpu_threads} threads and 1 thread)")
+        out["cpu_baseline"] = cpu_baseline(prob, target_s=12.0, threads=args.cpu_threads)
+        out["cpu_baseline_1thread"] = cpu_baseline(prob, target_s=12.0, threads=1)
     print(json.dumps(out), flush=True)
 
 

@@ -91,11 +91,12 @@ int apl_elem_eval(apl_mesh* mesh, int material, const void* mu, const void* lam,
 int apl_ext_force_eval(apl_mesh* mesh, int64_t n, const int32_t* indices, const void* force, const void* u,
                        double* energy, void* grad, int accumulate, void* stream);
 
-/* Measurement support (no reference counterpart): run one element pass
- * `iters` times on `stream`, bracketing each element-kernel launch with HIP
- * events.  ms_kernel = mean element-kernel time, ms_pass = mean time of the
- * whole pass (kernel + shared-vertex sums + energy sums), both host outputs.
- * what: ENERGY|GRAD|HESS_DIAG (pass A), HESS_QUAD or HESS_PROD. */
+/* Measurement support (no reference counterpart).  ms_kernel (host, may be
+ * NULL): mean time of `iters` element-kernel launches run back to back
+ * between two HIP events on `stream`; ms_pass (host, may be NULL): mean
+ * time of `iters` whole passes (element kernel + vertex and energy sums)
+ * measured the same way.  what: ENERGY|GRAD|HESS_DIAG (pass A), HESS_QUAD or
+ * HESS_PROD. */
 int apl_bench_elem_pass(apl_mesh* mesh, int material, unsigned what, const void* mu, const void* lam, const void* u,
                         const void* p, double* energy, void* out_a, void* out_b, int iters, double* ms_kernel,
                         double* ms_pass, void* stream);
