@@ -47,11 +47,11 @@ static_assert(D_E < N_DOTS, "dot table too small");
 static_assert(D_PP < N_DOTS, "dot slots");
 
 struct StatePtrs {
-  double *f, *gd, *alpha, *alpha_commit, *beta, *gnorm0, *gnorm, *pmean, *f_first, *pHp, *gbest;
+  double *f, *gd, *alpha, *alpha_commit, *beta, *gnorm0, *gnorm, *pmean, *f_first, *pHp, *gbest, *f_mark;
   int32_t *status, *act_eval, *act_dir, *commit, *n_steps, *n_evals, *ls_step, *n_backtracks, *phase, *restarts,
       *best_step;
 };
-constexpr int N_SD = 11, N_SI = 11;
+constexpr int N_SD = 12, N_SI = 11;
 
 struct Params {
   double rtol, atol, overstep, c1, wolfe_delta, wolfe_eps, beta_max;
@@ -163,6 +163,7 @@ __global__ void pncg_init_kernel(StatePtrs st, int n) {
   st.f_first[i] = 0.0;
   st.pHp[i] = 0.0;
   st.gbest[i] = INFINITY;
+  st.f_mark[i] = INFINITY;
   st.best_step[i] = 0;
   st.status[i] = APL_RUNNING;
   st.act_eval[i] = 1;
@@ -297,10 +298,16 @@ __global__ __launch_bounds__(SCAL_TH) void pncg_scalar_kernel(StatePtrs st, Para
       st.n_steps[i] += 1;
       st.restarts[i] = 0;
     }
-    // progress marker: the last step that lowered the best gradient norm by 10%
-    if (gn < 0.9 * st.gbest[i]) {
-      st.gbest[i] = gn;
+    // progress marker: the last step that lowered the best gradient norm by
+    // 10% or the energy, since the previous marker, by more than the energy
+    // noise the approximate-Wolfe test tolerates (10 wolfe_eps relative)
+    const bool g_prog = gn < 0.9 * st.gbest[i];
+    const double f_tol = fmax(10.0 * P.wolfe_eps, 1e-12) * fabs(ft);
+    const bool f_prog = !(st.f_mark[i] - ft <= f_tol);
+    if (g_prog) st.gbest[i] = gn;
+    if (g_prog || f_prog) {
       st.best_step[i] = st.n_steps[i];
+      st.f_mark[i] = ft;
     }
     int status = APL_RUNNING;
     if (!isfinite(gn) || !isfinite(ft))
@@ -956,6 +963,7 @@ int apl_solver_create(apl_mesh* m, int material, apl_solver** out) {
   s->st.f_first = d + 8 * n;
   s->st.pHp = d + 9 * n;
   s->st.gbest = d + 10 * n;
+  s->st.f_mark = d + 11 * n;
   int32_t* ii = s->si;
   s->st.status = ii + 0 * n;
   s->st.act_eval = ii + 1 * n;

@@ -69,7 +69,7 @@ class PncgConfig:
     iters_per_launch: int = 16
     use_graph: bool = True
     grid: int = 0
-    stall_steps: int = 2000  # STAGNATION: this many steps without a 10% better gradient norm near rtol
+    stall_steps: int = 2000  # STAGNATION: this many steps without gradient or energy progress near rtol
 
     def to_c(self, dtype: torch.dtype) -> _lib.PncgParams:
         eps = self.wolfe_eps

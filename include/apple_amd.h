@@ -109,9 +109,10 @@ int apl_bench_elem_pass(apl_mesh* mesh, int material, unsigned what, const void*
  * (benches/bench_pncg_branching_backends.py:288-289), iters_per_launch 16,
  * use_graph 1, grid_override 0 (occupancy-derived persistent grid),
  * stall_steps 2000: an instance whose best relative gradient norm is within
- * three decades of rtol and has not dropped by 10% within that many accepted
- * steps ends with APL_STAGNATION (0 = never); this is where an FP32 solve
- * stops when its gradient reaches the rounding floor of the element forces. */
+ * three decades of rtol and that has lowered neither that norm by 10% nor
+ * its energy by 10 wolfe_eps relative within that many accepted steps ends with
+ * APL_STAGNATION (0 = never); this is where an FP32 solve stops when its
+ * gradient reaches the rounding floor of the element forces. */
 typedef struct {
   double rtol, atol;
   int32_t max_steps, max_backtracking;
