@@ -173,6 +173,10 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--chunks", type=int, default=1, help="tet chunks of 256 per element block")
     ap.add_argument("--batched-only", action="store_true", help="only the batched forward+adjoint comparison")
+    ap.add_argument("--diff-only", action="store_true", help="only the 32^3 adjoint-vs-finite-difference check")
+    ap.add_argument("--combined-only", action="store_true", help="only the FP32-vs-FP64 batched forward+adjoint")
+    ap.add_argument("--no-diff", action="store_true")
+    ap.add_argument("--no-combined", action="store_true")
     ap.add_argument("--batched-meshes", type=int, default=256)
     ap.add_argument("--cpu-threads", type=int, default=16, help="threads of the multi-threaded CPU baseline")
     ap.add_argument("--profile-pass", action="store_true",
@@ -194,6 +198,12 @@ def main():
     n = args.n
     if args.batched_only:
         print(json.dumps(bench_batched(dev, dtype, n_meshes=args.batched_meshes)), flush=True)
+        return
+    if args.diff_only:
+        print(json.dumps(bench_differentiation(dev)), flush=True)
+        return
+    if args.combined_only:
+        print(json.dumps(bench_combined(dev)), flush=True)
         return
     log(f"[bench] building {n}^3 x 5 cube (SNH, FP32)")
     prob = make_cube_problem(n, "snh", dtype, dev, chunks=args.chunks)
@@ -316,6 +326,12 @@ def main():
     if not (args.quick or args.no_batch):
         out["batched"] = bench_batched(dev, dtype, n_meshes=args.batched_meshes)
 
+    # ---------------------------------------------------------------- differentiation, FP32 vs FP64
+    if not (args.quick or args.no_diff):
+        out["differentiation"] = bench_differentiation(dev)
+    if not (args.quick or args.no_combined):
+        out["combined"] = bench_combined(dev)
+
     # ---------------------------------------------------------------- CPU baseline
     if not (args.quick or args.no_cpu):
         log(f"[bench] CPU baseline (C restatement, {args.cpu_threads} threads and 1 thread)")
@@ -430,6 +446,151 @@ def bench_batched(dev, dtype, n_meshes=256, lo=6, hi=32, seed_sizes=2, seed_mat=
                        f"{adj_rtol:g} on H p = -(x - x0), then the mixed pass for dL/d(mu, lambda)")
     res["max_n"] = max(sizes)
     res["setup_s"] = round(t_setup, 2)
+    return res
+
+
+def bench_differentiation(dev, n=32, n_fd=64, seed_true=4, seed_init=5, seed_pick=6):
+    """Config 4: d/dmu of L = ||x*(mu) - x_target||^2 on an n^3 x 5 cube by the
+    adjoint (FP32 forward + adjoint, and FP64), checked against FP64 central
+    differences on n_fd random tets.  lambda = 10 mu (tied, as in the plain
+    large-mesh case); x_target = the FP64 equilibrium for a seeded
+    ground-truth lognormal mu."""
+    import torch
+
+    from apple_amd import adjoint, fem
+    from apple_amd.mesh import Region, cube
+    from apple_amd.solver import PncgConfig, mask_bits
+
+    m = cube(n)
+    region = Region.from_mesh(m)
+    T = m.n_cells
+    f = np.zeros((m.n_points, 3))
+    np.add.at(f[:, 2], m.cells.reshape(-1), np.repeat(-9.8e3 * region.dV / 4, 4))
+    fixed = np.zeros((m.n_points, 3), bool)
+    fixed[m.points[:, 2] == 0] = True
+    mask = mask_bits(fixed, dev)
+    mu_true = 1e4 * np.exp(0.5 * np.random.default_rng(seed_true).standard_normal(T))
+    mu0 = 1e4 * np.exp(0.5 * np.random.default_rng(seed_init).standard_normal(T))
+    m.cell_data["mu"] = mu0
+    m.cell_data["lambda"] = 10 * mu0
+    pots = {dt: fem.StableNeoHookean.from_region(region, dtype=dt, device=dev) for dt in (torch.float32, torch.float64)}
+    f64 = torch.as_tensor(f, dtype=torch.float64, device=dev)
+    z64 = torch.zeros(m.n_points, 3, dtype=torch.float64, device=dev)
+    tight = PncgConfig(rtol=1e-12, max_steps=50000)
+    mt = torch.as_tensor(mu_true, dtype=torch.float64, device=dev)
+    with torch.no_grad():
+        x_t = adjoint.equilibrium(pots[torch.float64], mt, 10 * mt, f64, mask, z64, tight)
+
+    grads, times, xs = {}, {}, {}
+    for dt, cfg, acfg in ((torch.float32, PncgConfig(rtol=1e-6, max_steps=50000), adjoint.AdjointConfig(rtol=1e-6)),
+                          (torch.float64, tight, adjoint.AdjointConfig(rtol=1e-12))):
+        ft = torch.as_tensor(f, dtype=dt, device=dev)
+        xt = x_t.to(dt)
+        for rep in range(2):  # the first run builds the graphs
+            mu_t = torch.as_tensor(mu0, dtype=dt, device=dev).requires_grad_(True)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            x = adjoint.equilibrium(pots[dt], mu_t, 10 * mu_t, ft, mask, torch.zeros_like(ft), cfg, acfg)
+            ((x - xt) ** 2).sum().backward()
+            torch.cuda.synchronize()
+            times[dt] = time.perf_counter() - t0
+        grads[dt] = mu_t.grad.double().cpu().numpy()
+        xs[dt] = x.detach()
+
+    # FP64 central differences, warm-started at the FP64 equilibrium of mu0
+    x_eq = xs[torch.float64]
+    fd_cfg = PncgConfig(rtol=1e-12, max_steps=50000)
+
+    def loss(mu_np):
+        mtt = torch.as_tensor(mu_np, dtype=torch.float64, device=dev)
+        with torch.no_grad():
+            xx = adjoint.equilibrium(pots[torch.float64], mtt, 10 * mtt, f64, mask, x_eq, fd_cfg)
+        return float(((xx - x_t) ** 2).sum())
+
+    pick = np.random.default_rng(seed_pick).choice(T, n_fd, replace=False)
+    fd = np.empty(n_fd)
+    t0 = time.perf_counter()
+    for k, t in enumerate(pick):
+        h = 1e-4 * mu0[t]
+        mp, mm = mu0.copy(), mu0.copy()
+        mp[t] += h
+        mm[t] -= h
+        fd[k] = (loss(mp) - loss(mm)) / (2 * h)
+    t_fd = time.perf_counter() - t0
+    scale = np.abs(fd).max()
+    out = {"workload": f"{n}^3 x 5 cube ({T} tets), SNH, lambda = 10 mu, mu* lognormal (seed {seed_true}), "
+                       f"mu0 lognormal (seed {seed_init}), L = ||x*(mu) - x*(mu*)||^2, {n_fd} FD tets (seed {seed_pick})",
+           "fd_seconds": round(t_fd, 2)}
+    for dt, name in ((torch.float32, "fp32"), (torch.float64, "fp64")):
+        g = grads[dt][pick]
+        err = np.abs(g - fd)
+        out[name] = {
+            "forward_adjoint_s": round(times[dt], 4),
+            "max_abs_err_over_max_fd": float(err.max() / scale),
+            "median_rel_err": float(np.median(err / np.maximum(np.abs(fd), 1e-300))),
+            "norm_rel_err": float(np.linalg.norm(g - fd) / np.linalg.norm(fd)),
+        }
+        log(f"[bench] differentiation {name}: ||g - fd|| / ||fd|| = {out[name]['norm_rel_err']:.2e}, "
+            f"max |g - fd| / max |fd| = {out[name]['max_abs_err_over_max_fd']:.2e}, {times[dt]:.3f} s")
+    return out
+
+
+def bench_combined(dev, n_meshes=64, lo=16, hi=48, seed=7, adj_rtol32=1e-6, adj_rtol64=1e-10):
+    """Config 5: forward + adjoint of one batch of uneven meshes with FP32
+    element kernels (FP64 iterate and reductions) against the same batch run
+    all in FP64; per-instance errors of energy, dL/dmu and positions."""
+    import torch
+
+    from apple_amd import _lib, batch
+    from apple_amd.solver import PncgConfig
+
+    insts, sizes = make_cube_instances(n_meshes, lo, hi, seed, seed)
+    res = {}
+    runs = {}
+    for dt, cfg, ar in ((torch.float32, PncgConfig(rtol=1e-6, max_steps=50000), adj_rtol32),
+                        (torch.float64, PncgConfig(rtol=1e-10, max_steps=50000), adj_rtol64)):
+        b = batch.Batch(insts, _lib.APL_SNH, dtype=dt, device=dev)
+        forward_adjoint(b, PncgConfig(rtol=1e-6, max_steps=16), ar, adjoint=False)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        x, st, cs, dmu, dlam = forward_adjoint(b, cfg, ar)
+        torch.cuda.synchronize()
+        res[str(dt).split(".")[-1] + "_s"] = round(time.perf_counter() - t0, 3)
+        runs[dt] = (b, x.double(), st, cs, dmu.double(), dlam.double())
+        log(f"[bench] combined {dt}: {res[str(dt).split('.')[-1] + '_s']} s")
+    b32, x32, st32, cs32, dmu32, dlam32 = runs[torch.float32]
+    b64, x64, st64, cs64, dmu64, dlam64 = runs[torch.float64]
+    diag = np.sqrt(3.0)
+    e_err, g_err, x_err = [], [], []
+    for i in range(n_meshes):
+        sv = slice(int(b32.vtx_off[i]), int(b32.vtx_off[i + 1]))
+        stt = slice(int(b32.tet_off[i]), int(b32.tet_off[i + 1]))
+        e_err.append(abs(st32[i].f - st64[i].f) / max(abs(st64[i].f), 1e-300))
+        gg32 = torch.cat([dmu32[stt], dlam32[stt]])
+        gg64 = torch.cat([dmu64[stt], dlam64[stt]])
+        g_err.append(float(torch.linalg.norm(gg32 - gg64) / torch.linalg.norm(gg64)))
+        x_err.append(float((x32[sv] - x64[sv]).abs().max()) / diag)
+    e_err, g_err, x_err = map(np.asarray, (e_err, g_err, x_err))
+    ok = np.array([s.success for s in st32])
+    res.update({
+        "workload": f"{n_meshes} cubes, n ~ U{{{lo}..{hi}}} (seed {seed}), {b32.n_tets} tets, SNH, per-instance mu/nu "
+                    f"as the batched case; L = 1/2 ||x - x0||^2",
+        "fp32_converged": int(ok.sum()),
+        "fp32_stagnated": int(sum(s.result.name == "STAGNATION" for s in st32)),
+        "fp64_converged": int(sum(s.success for s in st64)),
+        "energy_rel_err_max": float(e_err.max()),
+        "grad_rel_err_max": float(g_err.max()),
+        "grad_rel_err_median": float(np.median(g_err)),
+        "pos_err_over_diag_max": float(x_err.max()),
+        "pos_err_over_diag_max_converged": float(x_err[ok].max()) if ok.any() else None,
+        "within_energy_1e-5": int((e_err <= 1e-5).sum()),
+        "within_grad_1e-4": int((g_err <= 1e-4).sum()),
+        "within_pos_1e-5_diag": int((x_err <= 1e-5).sum()),
+        "per_instance": [{"n": sizes[i], "fp32": st32[i].result.name, "e": float(e_err[i]), "g": float(g_err[i]),
+                          "x": float(x_err[i])} for i in range(n_meshes)],
+    })
+    log(f"[bench] combined: energy rel err max {e_err.max():.2e}, dL/dq rel err max {g_err.max():.2e}, "
+        f"positions max {x_err.max():.2e} of the bbox diagonal")
     return res
 
 
