@@ -91,6 +91,7 @@ def _declare(lib):
     ]
     lib.apl_mesh_destroy.argtypes = [c_vp]
     lib.apl_mesh_info.argtypes = [c_vp, ctypes.POINTER(MeshInfo)]
+    lib.apl_mesh_set_schedule.argtypes = [c_vp, ctypes.c_int]
     lib.apl_elem_eval.argtypes = [
         c_vp, ctypes.c_int, c_vp, c_vp, c_vp, c_vp, ctypes.c_uint, c_dp, c_vp, c_vp, c_vp, c_dp, c_vp, c_vp, c_vp,
     ]

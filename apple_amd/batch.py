@@ -43,7 +43,11 @@ class _BatchPotential:
 
 
 class Batch:
-    def __init__(self, instances: list[Instance], material: int, *, dtype=torch.float32, device=None, chunks=1):
+    def __init__(self, instances: list[Instance], material: int, *, dtype=torch.float32, device=None, chunks=1,
+                 schedule=None):
+        """schedule: None (chunks as given, default grid), "auto" (apple_amd.autotune
+        picks chunks and the element grid for this batch, cached per signature)
+        or an autotune.Schedule."""
         self.instances = instances
         self.material = int(material)
         self.dtype = as_dtype(dtype)
@@ -55,13 +59,24 @@ class Batch:
         cells = np.concatenate([inst.region.cells + self.vtx_off[i] for i, inst in enumerate(instances)])
         dminv = np.concatenate([inst.region.dminv for inst in instances])
         dV = np.concatenate([inst.region.dV for inst in instances])
-        self.mesh = DeviceMesh(cells, dminv, dV, int(self.vtx_off[-1]), dtype=self.dtype, device=self.device,
-                               inst_tet_off=self.tet_off, inst_vtx_off=self.vtx_off, chunks=chunks)
         t = lambda a: torch.as_tensor(np.concatenate(a), dtype=self.dtype, device=self.device).contiguous()
         self.mu = t([np.asarray(i.mu, np.float64) for i in instances])
         self.lam = (
             t([np.asarray(i.lam, np.float64) for i in instances]) if self.material == _lib.APL_SNH else None
         )
+        if isinstance(schedule, str) and schedule == "auto":
+            from . import autotune
+
+            schedule = autotune.tune(cells, dminv, dV, int(self.vtx_off[-1]), material=self.material, mu=self.mu,
+                                     lam=self.lam, dtype=self.dtype, device=self.device, inst_tet_off=self.tet_off,
+                                     inst_vtx_off=self.vtx_off)
+        self.schedule = schedule
+        if schedule is not None:
+            chunks = schedule.chunks
+        self.mesh = DeviceMesh(cells, dminv, dV, int(self.vtx_off[-1]), dtype=self.dtype, device=self.device,
+                               inst_tet_off=self.tet_off, inst_vtx_off=self.vtx_off, chunks=chunks)
+        if schedule is not None:
+            self.mesh.set_schedule(schedule.grid_pct)
         V = int(self.vtx_off[-1])
         f = np.zeros((V, 3))
         fm = np.zeros((V, 3), bool)

@@ -184,6 +184,17 @@ int apl_mesh_destroy(apl_mesh* mesh) {
   return 0;
 }
 
+int apl_mesh_set_schedule(apl_mesh* mesh, int grid_pct) {
+  if (mesh == nullptr || grid_pct < 25 || grid_pct > 800) {
+    set_error("grid_pct must be in [25, 800]");
+    return -1;
+  }
+  mesh->grid_pct = grid_pct;
+  for (auto& row : mesh->grid_cache)
+    for (int& g : row) g = 0;
+  return 0;
+}
+
 int apl_mesh_info(const apl_mesh* mesh, apl_mesh_info_t* info) {
   if (mesh == nullptr || info == nullptr) {
     set_error("NULL argument");

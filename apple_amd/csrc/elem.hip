@@ -547,7 +547,7 @@ static int grid_for_spt(apl_mesh* m, bool filtered) {
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, elem_kernel<MAT, WHAT, R, X, SPT>, NTH, smem) != hipSuccess ||
       occ < 1)
     occ = 1;
-  long long g = (long long)occ * m->n_cu;
+  long long g = (long long)occ * m->n_cu * m->grid_pct / 100;
   long long nb8 = ((m->L.nblocks + 7) / 8) * 8;
   if (g > nb8) g = nb8;
   g = ((g + 7) / 8) * 8;

@@ -63,6 +63,7 @@ struct apl_mesh {
   size_t device_bytes = 0;
   int grid_cache[2][256] = {};  // persistent grid per (material, pass variant, FP64 displacements, filtered)
   int refs = 1;                 // the caller's handle plus one per solver built on the mesh
+  int grid_pct = 100;           // persistent element grid, percent of occupancy x CUs (autotuned)
 };
 
 namespace apl {

@@ -655,6 +655,7 @@ static VecArgs<R> vec_args(apl_solver* s, double* x, const void* fext, const uin
 
 static Params to_params(const apl_pncg_params* p) {
   Params P;
+  std::memset(&P, 0, sizeof P);  // the bytes (padding included) key the cached graph
   P.rtol = p->rtol;
   P.atol = p->atol;
   P.overstep = p->overstep;
@@ -736,7 +737,7 @@ static int pncg_solve_t(apl_solver* s, const void* mu, const void* lam, void* x,
   // cached graph of K iterations for these pointers and parameters
   bool use_graph = prm->use_graph != 0;
   if (use_graph) {
-    std::vector<char> key(sizeof(void*) * 5 + sizeof(Params) + 2 * sizeof(int));
+    std::vector<char> key(sizeof(void*) * 5 + sizeof(Params) + 3 * sizeof(int));
     char* k = key.data();
     const void* ptrs[5] = {mu, lam, x, fext, mask};
     std::memcpy(k, ptrs, sizeof ptrs);
@@ -746,6 +747,8 @@ static int pncg_solve_t(apl_solver* s, const void* mu, const void* lam, void* x,
     std::memcpy(k, &K, sizeof K);
     k += sizeof K;
     std::memcpy(k, &grid, sizeof grid);
+    k += sizeof grid;
+    std::memcpy(k, &m->grid_pct, sizeof m->grid_pct);
     if (s->gexec == nullptr || key != s->gkey) {
       if (s->gexec) {
         drop_hip_error(hipGraphExecDestroy(s->gexec));

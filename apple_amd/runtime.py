@@ -120,6 +120,11 @@ class DeviceMesh:
     def handle(self) -> int:
         return self._handle.value
 
+    def set_schedule(self, grid_pct: int) -> None:
+        """Persistent element grid as a percentage of occupancy x CUs (autotuner knob)."""
+        _lib.check(_lib.lib().apl_mesh_set_schedule(self._handle, int(grid_pct)))
+        self.grid_pct = int(grid_pct)
+
     def info(self) -> dict:
         inf = _lib.MeshInfo()
         _lib.check(_lib.lib().apl_mesh_info(self._handle, ctypes.byref(inf)))

@@ -404,7 +404,7 @@ def bench_batched(dev, dtype, n_meshes=256, lo=6, hi=32, seed_sizes=2, seed_mat=
     t_setup = time.perf_counter()
     insts, sizes = make_cube_instances(n_meshes, lo, hi, seed_sizes, seed_mat)
     cfg = PncgConfig(rtol=1e-6, max_steps=20000)
-    b = batch.Batch(insts, _lib.APL_SNH, dtype=dtype, device=dev)
+    b = batch.Batch(insts, _lib.APL_SNH, dtype=dtype, device=dev, schedule="auto")
     singles = [batch.Batch([inst], _lib.APL_SNH, dtype=dtype, device=dev) for inst in insts]
     warm = PncgConfig(rtol=1e-6, max_steps=16)
     for bb in [b] + singles:
@@ -446,6 +446,8 @@ def bench_batched(dev, dtype, n_meshes=256, lo=6, hi=32, seed_sizes=2, seed_mat=
                        f"{adj_rtol:g} on H p = -(x - x0), then the mixed pass for dL/d(mu, lambda)")
     res["max_n"] = max(sizes)
     res["setup_s"] = round(t_setup, 2)
+    res["batch_schedule"] = {"chunks": b.schedule.chunks, "grid_pct": b.schedule.grid_pct,
+                             "tuned_us_per_iteration": b.schedule.us_per_iteration, "table_us": b.schedule.table}
     return res
 
 
@@ -476,14 +478,14 @@ def bench_differentiation(dev, n=32, n_fd=64, seed_true=4, seed_init=5, seed_pic
     pots = {dt: fem.StableNeoHookean.from_region(region, dtype=dt, device=dev) for dt in (torch.float32, torch.float64)}
     f64 = torch.as_tensor(f, dtype=torch.float64, device=dev)
     z64 = torch.zeros(m.n_points, 3, dtype=torch.float64, device=dev)
-    tight = PncgConfig(rtol=1e-12, max_steps=50000)
+    tight = PncgConfig(rtol=1e-11, max_steps=50000, stall_steps=500)
     mt = torch.as_tensor(mu_true, dtype=torch.float64, device=dev)
     with torch.no_grad():
         x_t = adjoint.equilibrium(pots[torch.float64], mt, 10 * mt, f64, mask, z64, tight)
 
     grads, times, xs = {}, {}, {}
     for dt, cfg, acfg in ((torch.float32, PncgConfig(rtol=1e-6, max_steps=50000), adjoint.AdjointConfig(rtol=1e-6)),
-                          (torch.float64, tight, adjoint.AdjointConfig(rtol=1e-12))):
+                          (torch.float64, tight, adjoint.AdjointConfig(rtol=1e-11))):
         ft = torch.as_tensor(f, dtype=dt, device=dev)
         xt = x_t.to(dt)
         for rep in range(2):  # the first run builds the graphs
@@ -499,7 +501,7 @@ def bench_differentiation(dev, n=32, n_fd=64, seed_true=4, seed_init=5, seed_pic
 
     # FP64 central differences, warm-started at the FP64 equilibrium of mu0
     x_eq = xs[torch.float64]
-    fd_cfg = PncgConfig(rtol=1e-12, max_steps=50000)
+    fd_cfg = PncgConfig(rtol=1e-11, max_steps=50000, stall_steps=500)
 
     def loss(mu_np):
         mtt = torch.as_tensor(mu_np, dtype=torch.float64, device=dev)
@@ -516,6 +518,8 @@ def bench_differentiation(dev, n=32, n_fd=64, seed_true=4, seed_init=5, seed_pic
         mp[t] += h
         mm[t] -= h
         fd[k] = (loss(mp) - loss(mm)) / (2 * h)
+        if k % 16 == 15:
+            log(f"[bench] differentiation: {k + 1} of {n_fd} central differences, {time.perf_counter() - t0:.1f} s")
     t_fd = time.perf_counter() - t0
     scale = np.abs(fd).max()
     out = {"workload": f"{n}^3 x 5 cube ({T} tets), SNH, lambda = 10 mu, mu* lognormal (seed {seed_true}), "

@@ -78,6 +78,14 @@ typedef struct {
 } apl_mesh_info_t;
 int apl_mesh_info(const apl_mesh* mesh, apl_mesh_info_t* info /* host */);
 
+/* Schedule knob (no reference counterpart; set by the autotuner,
+ * apple_amd/autotune.py): the persistent element-kernel grid as a percentage
+ * (25..800) of occupancy x compute units.  Together with the block size
+ * chosen at apl_mesh_create (chunks) it is the schedule the autotuner picks.
+ * Takes effect for later launches; a solver's captured graph keeps its grid
+ * until the solver's parameters change. */
+int apl_mesh_set_schedule(apl_mesh* mesh, int grid_pct);
+
 /* One fused element pass (replaces the five launches of warp/fem/_base.py:151-194).
  * u, p: (V,3); mu, lam: (T,) (lam ignored for ARAP); energy, hess_quad:
  * (n_inst,) float64; grad, hess_diag, hess_prod: (V,3); dmu, dlam: (T,).
