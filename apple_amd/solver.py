@@ -112,19 +112,23 @@ class DeviceSolver:
         except Exception:
             pass
 
-    def _mat_ptrs(self):
+    def _mat_ptrs(self, mu=None, lam=None):
+        """Per-tet materials: the given tensors, else the potential's."""
         m = self.mesh
         tet = dict(dtype=m.dtype, device=m.device, shape=(m.n_tets,))
-        mu = ptr(self.potential.materials["mu"], name="mu", **tet)
-        lam = self.potential.materials.get("lambda")
-        return mu, (ptr(lam, name="lambda", **tet) if lam is not None else None)
+        mu = self.potential.materials["mu"] if mu is None else mu
+        lam = self.potential.materials.get("lambda") if lam is None else lam
+        if self.potential.material != _lib.APL_SNH:
+            lam = None
+        return ptr(mu, name="mu", **tet), (ptr(lam, name="lambda", **tet) if lam is not None else None)
 
     def solve(self, x: torch.Tensor, f_ext: torch.Tensor | None = None, mask: torch.Tensor | None = None,
-              config: PncgConfig | None = None) -> list[SolveStats]:
-        """Minimise in place from ``x`` (fixed DOFs already at their values)."""
+              config: PncgConfig | None = None, *, mu=None, lam=None) -> list[SolveStats]:
+        """Minimise in place from ``x`` (fixed DOFs already at their values);
+        mu / lam override the potential's materials for this call."""
         config = config or PncgConfig()
         m = self.mesh
-        mu, lam = self._mat_ptrs()
+        mu, lam = self._mat_ptrs(mu, lam)
         params = config.to_c(m.dtype)
         stats = (_lib.PncgStats * m.n_inst)()
         _lib.check(
@@ -137,10 +141,10 @@ class DeviceSolver:
         return _stats(stats, m.n_inst)
 
     def cg(self, x: torch.Tensor, rhs: torch.Tensor, y: torch.Tensor, mask: torch.Tensor | None = None,
-           rtol: float = 1e-6, max_iters: int = 10000) -> list[SolveStats]:
-        """Solve H(x) y = rhs on the free DOFs (y overwritten)."""
+           rtol: float = 1e-6, max_iters: int = 10000, *, mu=None, lam=None) -> list[SolveStats]:
+        """Solve H(x) y = rhs on the free DOFs (y overwritten); mu / lam as in solve()."""
         m = self.mesh
-        mu, lam = self._mat_ptrs()
+        mu, lam = self._mat_ptrs(mu, lam)
         stats = (_lib.PncgStats * m.n_inst)()
         _lib.check(
             _lib.lib().apl_cg_solve(
