@@ -172,6 +172,9 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--chunks", type=int, default=1, help="tet chunks of 256 per element block")
+    ap.add_argument("--tune", action="store_true",
+                    help="autotune the headline mesh's schedule too (it gains little on one large cube; the batched "
+                         "comparison always autotunes its batch)")
     ap.add_argument("--batched-only", action="store_true", help="only the batched forward+adjoint comparison")
     ap.add_argument("--diff-only", action="store_true", help="only the 32^3 adjoint-vs-finite-difference check")
     ap.add_argument("--combined-only", action="store_true", help="only the FP32-vs-FP64 batched forward+adjoint")
@@ -210,6 +213,18 @@ def main():
     log(f"[bench] building {n}^3 x 5 cube (SNH, FP32)")
     prob = make_cube_problem(n, "snh", dtype, dev, chunks=args.chunks)
     pot = prob["pot"]
+    sched = None
+    if args.tune:
+        from apple_amd import autotune
+
+        r = prob["region"]
+        sched = autotune.tune(r.cells, r.dminv, r.dV, r.mesh.n_points, material=_lib.APL_SNH,
+                              mu=pot.materials["mu"], lam=pot.materials["lambda"], dtype=dtype, device=dev)
+        log(f"[bench] autotuned schedule: chunks {sched.chunks}, grid {sched.grid_pct}% ({sched.table})")
+        if sched.chunks != args.chunks:
+            prob = make_cube_problem(n, "snh", dtype, dev, chunks=sched.chunks)
+            pot = prob["pot"]
+        pot.mesh.set_schedule(sched.grid_pct)
     T, V = pot.mesh.n_tets, pot.mesh.n_verts
     info = pot.mesh.info()
     log(f"[bench] partition: {info['n_blocks']} blocks, nloc_max {info['nloc_max']}, {info['n_partials']} partials, "
@@ -291,6 +306,7 @@ def main():
             "n_blocks": info["n_blocks"],
             "nloc_max": info["nloc_max"],
             "n_partials": info["n_partials"],
+            "schedule": {"chunks": sched.chunks, "grid_pct": sched.grid_pct} if sched else "default",
             "parallelism": "single GPU",
         },
         "roofline": roofline,
