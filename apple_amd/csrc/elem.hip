@@ -405,13 +405,18 @@ __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R, X> A) {
           }
         }
       }
-      if constexpr (RED) {
+      if constexpr ((WHAT & W_E) != 0 && (WHAT & W_HQ) != 0) {
         double e = (double)e_acc, q = (double)q_acc;
         block_sum2(e, q, red);
         if (tid == 0) {
           A.blk_red[2 * b] = e;
           A.blk_red[2 * b + 1] = q;
         }
+      } else if constexpr (RED) {
+        // one of the two sums; its readers never look at the other column
+        constexpr int col = (WHAT & W_E) != 0 ? 0 : 1;
+        const double r = block_sum((double)(col == 0 ? e_acc : q_acc), red);
+        if (tid == 0) A.blk_red[2 * b + col] = r;
       }
       __syncthreads();
       });

@@ -145,6 +145,32 @@ __host__ __device__ inline M3<R> outer_sum(const TetGeom<R>& g, const V3<R> u[4]
   return G;
 }
 
+// G from edge differences u_a - u_0 (a = 1..3; the d_a sum to zero, so the
+// corner-0 term drops out)
+template <class R>
+__host__ __device__ inline M3<R> outer_sum_edges(const TetGeom<R>& g, const V3<R> e[4]) {
+  M3<R> G;
+  const R* ec[4] = {&e[0].x, &e[1].x, &e[2].x, &e[3].x};
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    R e1 = ec[1][i], e2 = ec[2][i], e3 = ec[3][i];
+    G.m[i][0] = e1 * g.d[1].x + e2 * g.d[2].x + e3 * g.d[3].x;
+    G.m[i][1] = e1 * g.d[1].y + e2 * g.d[2].y + e3 * g.d[3].y;
+    G.m[i][2] = e1 * g.d[1].z + e2 * g.d[2].z + e3 * g.d[3].z;
+  }
+  return G;
+}
+
+// M d_a for the four corners; corner 0 as minus the sum of the others (the
+// d_a sum to zero), which keeps each tet's corner forces summing to zero.
+template <class R>
+__host__ __device__ inline void corner_products(const M3<R>& M, const TetGeom<R>& g, V3<R> out[4]) {
+#pragma unroll
+  for (int a = 1; a < 4; ++a) out[a] = mul(M, g.d[a]);
+  out[0] = v3(-(out[1].x + out[2].x + out[3].x), -(out[1].y + out[2].y + out[3].y),
+              -(out[1].z + out[2].z + out[3].z));
+}
+
 // Invariants of F = I + G without cancellation.
 template <class R>
 struct Inv {
@@ -443,10 +469,12 @@ struct TetOut {
 
 // Evaluate the requested quantities for one tet.  u: displacements, v: the
 // direction for W_HP / W_HQ / W_MIX.
+// u: edge differences (u[a] = displacement of corner a minus that of corner
+// 0; u[0] is not read); v: the direction at the four corners.
 template <int MAT, unsigned WHAT, class R>
 __host__ __device__ inline void tet_eval(const TetGeom<R>& g, R mu, R lam, const V3<R> u[4],
                                          const V3<R> v[4], TetOut<R>& o) {
-  const M3<R> G = outer_sum(g, u);
+  const M3<R> G = outer_sum_edges(g, u);
   const Inv<R> I = invariants(G);
   constexpr bool needV = (WHAT & (W_HP | W_HQ | W_MIX)) != 0;
   M3<R> dF;
@@ -469,13 +497,14 @@ __host__ __device__ inline void tet_eval(const TetGeom<R>& g, R mu, R lam, const
         for (int i = 0; i < 3; ++i)
 #pragma unroll
           for (int j = 0; j < 3; ++j) P.m[i][j] = muV * D.m[i][j] + lj * C.m[i][j];
-#pragma unroll
-        for (int a = 0; a < 4; ++a) o.g[a] = mul(P, g.d[a]);
+        corner_products(P, g, o.g);
       }
       if constexpr ((WHAT & W_H) != 0) {
+        V3<R> wc[4];
+        corner_products(C, g, wc);
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
-          V3<R> w = mul(C, g.d[a]);
+          const V3<R> w = wc[a];
           R t = muV * dot(g.d[a], g.d[a]);
           R hx = lamV * w.x * w.x + t;
           R hy = lamV * w.y * w.y + t;
@@ -509,8 +538,7 @@ __host__ __device__ inline void tet_eval(const TetGeom<R>& g, R mu, R lam, const
           for (int i = 0; i < 3; ++i)
 #pragma unroll
             for (int j = 0; j < 3; ++j) T.m[i][j] = lc * C.m[i][j] + muV * dF.m[i][j] + sV * Hs.m[i][j];
-#pragma unroll
-          for (int a = 0; a < 4; ++a) o.hp[a] = mul(T, g.d[a]);
+          corner_products(T, g, o.hp);
         }
         if constexpr ((WHAT & W_MIX) != 0) {
           const M3<R> D = F_minus_cofF(G, I.trG);
@@ -526,8 +554,7 @@ __host__ __device__ inline void tet_eval(const TetGeom<R>& g, R mu, R lam, const
     if constexpr ((WHAT & W_G) != 0) {
       R d[3] = {muV * S.sm1[0], muV * S.sm1[1], muV * S.sm1[2]};
       const M3<R> P = udv(S, d);
-#pragma unroll
-      for (int a = 0; a < 4; ++a) o.g[a] = mul(P, g.d[a]);
+      corner_products(P, g, o.g);
     }
     R lk[3];
     if constexpr ((WHAT & (W_H | W_HP | W_HQ)) != 0) arap_lambdas(S.s, lk);
@@ -558,8 +585,7 @@ __host__ __device__ inline void tet_eval(const TetGeom<R>& g, R mu, R lam, const
         for (int i = 0; i < 3; ++i)
 #pragma unroll
           for (int j = 0; j < 3; ++j) T.m[i][j] = muV * (dF.m[i][j] - Qc.m[i][j]);
-#pragma unroll
-        for (int a = 0; a < 4; ++a) o.hp[a] = mul(T, g.d[a]);
+        corner_products(T, g, o.hp);
       }
       if constexpr ((WHAT & W_MIX) != 0) {
         o.dmu = g.dV * (S.sm1[0] * dM[0] + S.sm1[1] * dM[1] + S.sm1[2] * dM[2]);
