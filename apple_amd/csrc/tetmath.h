@@ -248,6 +248,56 @@ __host__ __device__ inline void jacobi_rotate(R (&a)[3][3], M3<R>& V, int p, int
   }
 }
 
+// FP32 rotation with the hardware reciprocal / square-root approximations
+// (about 1 ulp): used only to find a basis that FP64 sweeps then finish.
+__host__ __device__ inline float fast_rcp(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rcpf(x);
+#else
+  return 1.0f / x;
+#endif
+}
+__host__ __device__ inline float fast_sqrt(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_sqrtf(x);
+#else
+  return sqrtf(x);
+#endif
+}
+__host__ __device__ inline float fast_rsq(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rsqf(x);
+#else
+  return 1.0f / sqrtf(x);
+#endif
+}
+
+__host__ __device__ inline void jacobi_rotate_fast(float (&a)[3][3], M3<float>& V, int p, int q) {
+  const float apq = a[p][q];
+  const float scale = fabsf(a[p][p]) + fabsf(a[q][q]);
+  if (!(fabsf(apq) > 1e-30f + scale * 1e-9f)) {
+    a[p][q] = a[q][p] = 0.0f;
+    return;
+  }
+  const float tau = (a[q][q] - a[p][p]) * fast_rcp(2.0f * apq);
+  const float t = (tau >= 0.0f ? 1.0f : -1.0f) * fast_rcp(fabsf(tau) + fast_sqrt(1.0f + tau * tau));
+  const float c = fast_rsq(1.0f + t * t);
+  const float s = t * c;
+  a[p][p] -= t * apq;
+  a[q][q] += t * apq;
+  a[p][q] = a[q][p] = 0.0f;
+  const int r = 3 - p - q;
+  const float arp = a[r][p], arq = a[r][q];
+  a[r][p] = a[p][r] = c * arp - s * arq;
+  a[r][q] = a[q][r] = s * arp + c * arq;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const float vp = V.m[i][p], vq = V.m[i][q];
+    V.m[i][p] = c * vp - s * vq;
+    V.m[i][q] = s * vp + c * vq;
+  }
+}
+
 template <class R>
 __host__ __device__ inline void swap_cols(M3<R>& V, R (&e)[3], int i, int j) {
   R t = e[i];
@@ -281,10 +331,9 @@ struct Svd {
   R sm1[3];  // s_i - 1 computed without cancellation
 };
 
-// Rotation-variant SVD of F = I + G (see file header).
+// Symmetric A = F^T F - I = G + G^T + G^T G (no cancellation near F = I).
 template <class R>
-__host__ __device__ inline Svd<R> svd_rv(const M3<R>& G) {
-  R a[3][3];
+__host__ __device__ inline void ftf_minus_i(const M3<R>& G, R (&a)[3][3]) {
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -292,22 +341,36 @@ __host__ __device__ inline Svd<R> svd_rv(const M3<R>& G) {
       R gtg = G.m[0][i] * G.m[0][j] + G.m[1][i] * G.m[1][j] + G.m[2][i] * G.m[2][j];
       a[i][j] = G.m[i][j] + G.m[j][i] + gtg;
     }
-  Svd<R> r;
-  M3<R>& V = r.V;
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) V.m[i][j] = (i == j) ? R(1) : R(0);
-  const int sweeps = sizeof(R) == 4 ? 4 : 6;
+}
+
+template <class R>
+__host__ __device__ inline void jacobi_sweeps(R (&a)[3][3], M3<R>& V, int sweeps) {
   for (int sw = 0; sw < sweeps; ++sw) {
     jacobi_rotate(a, V, 0, 1);
     jacobi_rotate(a, V, 0, 2);
     jacobi_rotate(a, V, 1, 2);
   }
-  R e[3] = {a[0][0], a[1][1], a[2][2]};
+}
+
+// Rotation-variant SVD of F = I + G (see file header), from the symmetric
+// eigen-decomposition a = V diag(e) V^T of F^T F - I that the caller has
+// already brought to diagonal form.
+template <class R>
+__host__ __device__ inline void sort_eig(const R (&a)[3][3], M3<R>& V, R (&e)[3]) {
+  e[0] = a[0][0];
+  e[1] = a[1][1];
+  e[2] = a[2][2];
   if (e[0] < e[1]) swap_cols(V, e, 0, 1);
   if (e[0] < e[2]) swap_cols(V, e, 0, 2);
   if (e[1] < e[2]) swap_cols(V, e, 1, 2);
+}
+
+// U and sigma from the sorted eigenpairs (e, V) of F^T F - I.
+template <class R>
+__host__ __device__ inline Svd<R> qr_sigma(const M3<R>& G, const M3<R>& V0, const R (&e)[3]) {
+  Svd<R> r;
+  M3<R>& V = r.V;
+  V = V0;
   // U and the singular values by Gram-Schmidt QR of F V, F = I + G
   // (the QR step of McAdams et al., "Computing the SVD of 3x3 matrices with
   // minimal branching", 2011): sigma = diag(R) keeps absolute accuracy
@@ -344,28 +407,90 @@ __host__ __device__ inline Svd<R> svd_rv(const M3<R>& G) {
   return r;
 }
 
-// Rotation-variant SVD computed in precision C and rounded to R.  The FP32
-// element kernels run the ARAP SVD in FP64: the Jacobi/QR steps are the
-// accuracy bottleneck of FP32 ARAP (stress errors up to 1e-4 relative near
-// repeated singular values), while the kernel's memory traffic -- its
-// bound -- does not change.
-template <class R, class C>
-__host__ __device__ inline Svd<R> svd_rv_in(const M3<R>& G) {
-  M3<C> Gc;
+template <class R>
+__host__ __device__ inline Svd<R> svd_rv(const M3<R>& G) {
+  R a[3][3];
+  ftf_minus_i(G, a);
+  M3<R> V;
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
-    for (int j = 0; j < 3; ++j) Gc.m[i][j] = (C)G.m[i][j];
-  const Svd<C> Sc = svd_rv(Gc);
+    for (int j = 0; j < 3; ++j) V.m[i][j] = (i == j) ? R(1) : R(0);
+  jacobi_sweeps(a, V, sizeof(R) == 4 ? 4 : 6);
+  R e[3];
+  sort_eig(a, V, e);
+  return qr_sigma(G, V, e);
+}
+
+// The same SVD to FP64 accuracy at a fraction of the FP64 work: FP32 Jacobi
+// sweeps (hardware-approximate rotations) find the eigenvectors to about
+// FP32 accuracy, the FP64 matrix is rotated into that basis (nearly
+// diagonal), and FP64 sweeps -- whose rotations are tiny, so quadratic
+// convergence finishes them -- complete it.
+#ifndef REFINE_F32_SWEEPS
+#define REFINE_F32_SWEEPS 4
+#endif
+#ifndef REFINE_F64_SWEEPS
+#define REFINE_F64_SWEEPS 1
+#endif
+template <class R>
+__host__ __device__ inline Svd<R> svd_rv_refined(const M3<R>& G) {
+  M3<float> Gf;
+  M3<double> Gd;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      Gf.m[i][j] = (float)G.m[i][j];
+      Gd.m[i][j] = (double)G.m[i][j];
+    }
+  float af[3][3];
+  ftf_minus_i(Gf, af);
+  M3<float> Vf;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) Vf.m[i][j] = (i == j) ? 1.0f : 0.0f;
+  for (int sw = 0; sw < REFINE_F32_SWEEPS; ++sw) {
+    jacobi_rotate_fast(af, Vf, 0, 1);
+    jacobi_rotate_fast(af, Vf, 0, 2);
+    jacobi_rotate_fast(af, Vf, 1, 2);
+  }
+  double ad[3][3], t[3][3], b[3][3];
+  ftf_minus_i(Gd, ad);
+  // re-orthonormalise the FP32 basis in FP64 (Gram-Schmidt, det +1)
+  V3<double> c0 = v3((double)Vf.m[0][0], (double)Vf.m[1][0], (double)Vf.m[2][0]);
+  V3<double> c1 = v3((double)Vf.m[0][1], (double)Vf.m[1][1], (double)Vf.m[2][1]);
+  c0 = (1.0 / sqrt(dot(c0, c0))) * c0;
+  c1 = c1 - dot(c0, c1) * c0;
+  c1 = (1.0 / sqrt(dot(c1, c1))) * c1;
+  const V3<double> c2 = cross(c0, c1);
+  M3<double> V;
+  set_col(V, 0, c0);
+  set_col(V, 1, c1);
+  set_col(V, 2, c2);
+  // b = V^T ad V
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) t[i][j] = ad[i][0] * V.m[0][j] + ad[i][1] * V.m[1][j] + ad[i][2] * V.m[2][j];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) b[i][j] = V.m[0][i] * t[0][j] + V.m[1][i] * t[1][j] + V.m[2][i] * t[2][j];
+  jacobi_sweeps(b, V, REFINE_F64_SWEEPS);
+  double ed[3];
+  sort_eig(b, V, ed);
+  const Svd<double> Sd = qr_sigma(Gd, V, ed);
   Svd<R> S;
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    S.s[i] = (R)Sc.s[i];
-    S.sm1[i] = (R)Sc.sm1[i];
+    S.s[i] = (R)Sd.s[i];
+    S.sm1[i] = (R)Sd.sm1[i];
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      S.U.m[i][j] = (R)Sc.U.m[i][j];
-      S.V.m[i][j] = (R)Sc.V.m[i][j];
+      S.U.m[i][j] = (R)Sd.U.m[i][j];
+      S.V.m[i][j] = (R)Sd.V.m[i][j];
     }
   }
   return S;
@@ -548,7 +673,7 @@ __host__ __device__ inline void tet_eval(const TetGeom<R>& g, R mu, R lam, const
       }
     }
   } else {  // ARAP
-    const Svd<R> S = svd_rv_in<R, double>(G);
+    const Svd<R> S = sizeof(R) == 4 ? svd_rv_refined<R>(G) : svd_rv<R>(G);
     if constexpr ((WHAT & W_E) != 0)
       o.e = R(0.5) * muV * (S.sm1[0] * S.sm1[0] + S.sm1[1] * S.sm1[1] + S.sm1[2] * S.sm1[2]);
     if constexpr ((WHAT & W_G) != 0) {
