@@ -140,3 +140,51 @@ def test_batched_adjoint_equals_single():
         g_s = run(single, mu, tgt)
         np.testing.assert_allclose(g_b[off : off + mu.size], g_s, rtol=1e-6, atol=1e-12)
         off += mu.size
+
+
+def test_batched_fp32_vs_fp64_forward_and_adjoint():
+    """Small version of the combined case: one batch, FP32 element kernels with
+    the FP64 iterate and reductions against the same batch in FP64.  Errors
+    per instance: energy <= 1e-5 relative, positions <= 1e-5 of the bounding-box
+    diagonal, dL/d(mu, lambda) <= 2e-4 relative (measured 2e-5 .. 6e-5; the FP32
+    adjoint is limited by the conditioning of H at the FP32 rounding floor)."""
+    from apple_amd import _lib
+
+    insts = []
+    for k, n in enumerate((5, 7, 6, 8)):
+        m = cube(n)
+        region = Region.from_mesh(m)
+        rng = np.random.default_rng(40 + k)
+        mu = rng.uniform(1e4, 1e5) * np.ones(m.n_cells)
+        nu = rng.uniform(0.3, 0.45)
+        f = np.zeros((m.n_points, 3))
+        np.add.at(f[:, 2], m.cells.reshape(-1), np.repeat(-9.8e3 * region.dV / 4, 4))
+        fixed = np.zeros((m.n_points, 3), bool)
+        fixed[m.points[:, 2] == 0] = True
+        x0 = 0.1 / n * rng.standard_normal((m.n_points, 3))
+        x0[fixed] = 0
+        insts.append(batch.Instance(region, mu, 2 * mu * nu / (1 - 2 * nu), f, fixed, x0))
+    out = {}
+    for dt, rtol in ((torch.float32, 1e-6), (torch.float64, 1e-10)):
+        b = batch.Batch(insts, _lib.APL_SNH, dtype=dt, device="cuda")
+        solver = b.solver()
+        x = b.x0.clone()
+        st = solver.solve(x, b.f_ext, b.mask, PncgConfig(rtol=rtol, max_steps=20000))
+        p = torch.zeros_like(x)
+        cs = solver.cg(x, (b.x0 - x).contiguous(), p, b.mask, rtol=rtol, max_iters=20000)
+        assert all(s.result.name == "PRIMARY_SUCCESS" for s in st + cs), (st, cs)
+        dmu = torch.empty(b.n_tets, dtype=dt, device="cuda")
+        dlam = torch.empty_like(dmu)
+        b.mesh.eval(_lib.APL_SNH, b.mu, b.lam, x, p, dmu=dmu, dlam=dlam)
+        out[dt] = (b, x.double().cpu().numpy(), [s.f for s in st], torch.cat([dmu, dlam]).double().cpu().numpy())
+    b, x32, f32, g32 = out[torch.float32]
+    _, x64, f64, g64 = out[torch.float64]
+    T = b.n_tets
+    for i in range(len(insts)):
+        sv = slice(int(b.vtx_off[i]), int(b.vtx_off[i + 1]))
+        st_ = np.r_[b.tet_off[i]:b.tet_off[i + 1], T + b.tet_off[i]:T + b.tet_off[i + 1]]
+        e_err = abs(f32[i] - f64[i]) / abs(f64[i])
+        x_err = np.abs(x32[sv] - x64[sv]).max() / np.sqrt(3.0)
+        g_err = np.linalg.norm(g32[st_] - g64[st_]) / np.linalg.norm(g64[st_])
+        print(f"instance {i}: energy {e_err:.2e}, positions/diag {x_err:.2e}, dL/dq {g_err:.2e}")
+        assert e_err <= 1e-5 and x_err <= 1e-5 and g_err <= 2e-4
