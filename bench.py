@@ -190,7 +190,7 @@ def main():
                     help="only run a PNCG solve of STEPS iterations (for rocprofv3 kernel traces)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (sharding is described in DESIGN.md)"}))
+        print(json.dumps({"error": "apple_amd runs on one GPU: --gpus must be 1 (the element passes would shard by tet slabs with a one-layer vertex halo; not implemented)"}))
         sys.exit(2)
 
     import torch
