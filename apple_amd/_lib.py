@@ -15,7 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libapple_amd.so")
 
 APL_F32, APL_F64 = 0, 1
-APL_SNH, APL_ARAP = 0, 1
+APL_SNH, APL_ARAP, APL_SNH_MUSCLE = 0, 1, 2
 APL_ENERGY, APL_GRAD, APL_HESS_DIAG, APL_HESS_PROD, APL_HESS_QUAD, APL_MIXED = 1, 2, 4, 8, 16, 32
 APL_ACCUMULATE = 1 << 16
 
@@ -95,6 +95,8 @@ def _declare(lib):
     lib.apl_elem_eval.argtypes = [
         c_vp, ctypes.c_int, c_vp, c_vp, c_vp, c_vp, ctypes.c_uint, c_dp, c_vp, c_vp, c_vp, c_dp, c_vp, c_vp, c_vp,
     ]
+    lib.apl_mesh_set_activation.argtypes = [c_vp, c_vp]
+    lib.apl_elem_mixed_activation.argtypes = [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.c_int, c_vp]
     lib.apl_ext_force_eval.argtypes = [c_vp, c_i64, c_vp, c_vp, c_vp, c_dp, c_vp, ctypes.c_int, c_vp]
     lib.apl_bench_elem_pass.argtypes = [
         c_vp, ctypes.c_int, ctypes.c_uint, c_vp, c_vp, c_vp, c_vp, c_dp, c_vp, c_vp, ctypes.c_int,

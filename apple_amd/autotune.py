@@ -81,10 +81,10 @@ def _load(path):
         return {}
 
 
-def tune(cells, dminv, dV, n_verts, *, material, mu, lam=None, dtype=torch.float32, device=None,
+def tune(cells, dminv, dV, n_verts, *, material, mu, lam=None, activation=None, dtype=torch.float32, device=None,
          inst_tet_off=None, inst_vtx_off=None, iters=10, cache: str | None = DEFAULT_CACHE,
          chunks=CHUNKS, grid_pct=GRID_PCT) -> Schedule:
-    """Pick (chunks, grid_pct) for this mesh; mu / lam are per-tet device tensors."""
+    """Pick (chunks, grid_pct) for this mesh; mu / lam / activation are per-tet device tensors."""
     sig = signature(cells, n_verts, inst_tet_off, dtype, material, device)
     if cache:
         hit = _load(cache).get(sig)
@@ -96,6 +96,8 @@ def tune(cells, dminv, dV, n_verts, *, material, mu, lam=None, dtype=torch.float
     for c in chunks:
         mesh = DeviceMesh(cells, dminv, dV, n_verts, dtype=dtype, device=device, inst_tet_off=inst_tet_off,
                           inst_vtx_off=inst_vtx_off, chunks=c)
+        if activation is not None:
+            mesh.set_activation(activation)
         u = torch.as_tensor(1e-3 * rng.standard_normal((n_verts, 3)), dtype=mesh.dtype, device=mesh.device)
         for g in grid_pct:
             mesh.set_schedule(g)

@@ -29,17 +29,20 @@ class Instance:
     f_ext: np.ndarray | None = None  # (V,3)
     fixed_mask: np.ndarray | None = None  # (V,3) bool
     x0: np.ndarray | None = None  # (V,3) initial guess incl. fixed values
+    activation: np.ndarray | None = None  # (T, 6), muscle law
 
 
 class _BatchPotential:
     """Minimal potential view (materials + mesh) for the device solver."""
 
-    def __init__(self, mesh, material, mu, lam):
+    def __init__(self, mesh, material, mu, lam, activation=None):
         self.mesh = mesh
         self.material = material
         self.materials = {"mu": mu}
         if lam is not None:
             self.materials["lambda"] = lam
+        if activation is not None:
+            self.materials["activation"] = activation
 
 
 class Batch:
@@ -62,14 +65,18 @@ class Batch:
         t = lambda a: torch.as_tensor(np.concatenate(a), dtype=self.dtype, device=self.device).contiguous()
         self.mu = t([np.asarray(i.mu, np.float64) for i in instances])
         self.lam = (
-            t([np.asarray(i.lam, np.float64) for i in instances]) if self.material == _lib.APL_SNH else None
+            t([np.asarray(i.lam, np.float64) for i in instances]) if self.material != _lib.APL_ARAP else None
+        )
+        self.activation = (
+            t([np.asarray(i.activation, np.float64).reshape(-1, 6) for i in instances])
+            if self.material == _lib.APL_SNH_MUSCLE else None
         )
         if isinstance(schedule, str) and schedule == "auto":
             from . import autotune
 
             schedule = autotune.tune(cells, dminv, dV, int(self.vtx_off[-1]), material=self.material, mu=self.mu,
-                                     lam=self.lam, dtype=self.dtype, device=self.device, inst_tet_off=self.tet_off,
-                                     inst_vtx_off=self.vtx_off)
+                                     lam=self.lam, activation=self.activation, dtype=self.dtype, device=self.device,
+                                     inst_tet_off=self.tet_off, inst_vtx_off=self.vtx_off)
         self.schedule = schedule
         if schedule is not None:
             chunks = schedule.chunks
@@ -92,7 +99,7 @@ class Batch:
         self.f_ext = torch.as_tensor(f, dtype=self.dtype, device=self.device).contiguous()
         self.mask = mask_bits(fm, self.device)
         self.x0 = torch.as_tensor(x0, dtype=self.dtype, device=self.device).contiguous()
-        self.potential = _BatchPotential(self.mesh, self.material, self.mu, self.lam)
+        self.potential = _BatchPotential(self.mesh, self.material, self.mu, self.lam, self.activation)
         self._solver = None
 
     @property

@@ -226,15 +226,15 @@ int apl_elem_eval(apl_mesh* m, int material, const void* mu, const void* lam, co
     set_error("mesh is NULL");
     return -1;
   }
-  if (material != APL_SNH && material != APL_ARAP) {
-    set_error("material must be APL_SNH or APL_ARAP");
+  if (material != APL_SNH && material != APL_ARAP && material != APL_SNH_MUSCLE) {
+    set_error("material must be APL_SNH, APL_ARAP or APL_SNH_MUSCLE");
     return -1;
   }
   const unsigned w = to_internal(what);
   const int acc = (what & APL_ACCUMULATE) ? 1 : 0;
   if (w == 0) return 0;
-  if (m->L.T > 0 && (mu == nullptr || u == nullptr || (material == APL_SNH && lam == nullptr))) {
-    set_error("mu, lam (SNH) and u must be non-NULL");
+  if (m->L.T > 0 && (mu == nullptr || u == nullptr || (material != APL_ARAP && lam == nullptr))) {
+    set_error("mu, lam (SNH, SNH_MUSCLE) and u must be non-NULL");
     return -1;
   }
   if (m->L.T > 0 && (w & (W_HP | W_HQ | W_MIX)) && p == nullptr) {
@@ -244,7 +244,7 @@ int apl_elem_eval(apl_mesh* m, int material, const void* mu, const void* lam, co
   const bool has_v = m->L.V > 0, has_t = m->L.T > 0;
   if (((w & W_E) && !energy) || ((w & W_G) && has_v && !grad) || ((w & W_H) && has_v && !hess_diag) ||
       ((w & W_HP) && has_v && !hess_prod) || ((w & W_HQ) && !hess_quad) ||
-      ((w & W_MIX) && has_t && (!dmu || (material == APL_SNH && !dlam)))) {
+      ((w & W_MIX) && has_t && (!dmu || (material != APL_ARAP && !dlam)))) {
     set_error("a requested output pointer is NULL");
     return -1;
   }
@@ -277,6 +277,34 @@ int apl_elem_eval(apl_mesh* m, int material, const void* mu, const void* lam, co
     if (launch_elem(m, material, W_MIX, mu, lam, u, p, nullptr, nullptr, dmu, dlam, acc, 0, s)) return -1;
   }
   return 0;
+}
+
+int apl_mesh_set_activation(apl_mesh* mesh, const void* activation) {
+  if (mesh == nullptr) {
+    set_error("mesh is NULL");
+    return -1;
+  }
+  mesh->activation = activation;
+  return 0;
+}
+
+int apl_elem_mixed_activation(apl_mesh* m, const void* mu, const void* lam, const void* u, const void* p,
+                              void* dact, void* dmu, void* dlam, int accumulate, void* stream) {
+  if (m == nullptr) {
+    set_error("mesh is NULL");
+    return -1;
+  }
+  if (m->L.T == 0) return 0;
+  if (mu == nullptr || lam == nullptr || u == nullptr || p == nullptr || dact == nullptr ||
+      ((dmu == nullptr) != (dlam == nullptr))) {
+    set_error("mu, lam, u, p and dact must be non-NULL, and dmu / dlam both given or both NULL");
+    return -1;
+  }
+  APL_CHECK_HIP(hipSetDevice(m->device));
+  (void)hipGetLastError();  // drop errors left by unrelated earlier HIP calls
+  const unsigned w = dmu != nullptr ? (W_MIX | W_MIXA) : W_MIXA;
+  return launch_elem(m, MAT_MUSCLE, w, mu, lam, u, p, nullptr, nullptr, dmu, dlam, accumulate ? 1 : 0, 0,
+                     (hipStream_t)stream, 0, dact);
 }
 
 int apl_bench_elem_pass(apl_mesh* m, int material, unsigned what, const void* mu, const void* lam, const void* u,

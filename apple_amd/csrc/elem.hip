@@ -52,12 +52,14 @@ struct ElemArgs {
   const uint16_t* chunk_off;
   const R* mu;
   const R* lam;
+  const R* act;  // (T, 6) activation of the muscle law
   const X* x;  // displacements (X = R, or double when the solver keeps them in FP64)
   const R* p;
   const double* alpha;
   const int32_t* active;
   R* out_a;  // per-tet outputs of the mixed pass
   R* out_b;
+  R* out_c;  // (T, 6) activation derivatives of the mixed pass
   R* partial;  // vertex-major partial sums (scatter passes)
   double* blk_red;
   int T;
@@ -147,13 +149,14 @@ template <class R>
 struct TetRec {
   R B[9];
   R dV, mu, lam;
+  R act[6];  // muscle activation
   uint4 cr;  // local slots (x, y), positions (z, w): two u16 per word
 };
 
 // Resources of one block's tet data.  dminv is SoA (row kk at kk * T), so its
 // base moves with the block and the row offset stays below 2^32 bytes.
 struct BlockRes {
-  rsrc_t dm, dv, mu, lam, cr;
+  rsrc_t dm, dv, mu, lam, act, cr;
   unsigned row, t0b;  // row stride and block start, in bytes of R
 };
 
@@ -163,7 +166,8 @@ __device__ inline BlockRes block_res(const ElemArgs<R, X>& A, int t0) {
   br.dm = make_rsrc(A.dminv + t0);
   br.dv = make_rsrc(A.dV);
   br.mu = make_rsrc(A.mu);
-  br.lam = make_rsrc(MAT == MAT_SNH ? A.lam : A.mu);
+  br.lam = make_rsrc(MAT != MAT_ARAP ? A.lam : A.mu);
+  br.act = make_rsrc(MAT == MAT_MUSCLE ? A.act + 6 * (size_t)t0 : A.mu);  // block-relative (T x 24 B may pass 2^32)
   br.cr = make_rsrc(A.corner);
   br.row = (unsigned)A.T * (unsigned)sizeof(R);
   br.t0b = (unsigned)t0 * (unsigned)sizeof(R);
@@ -177,7 +181,11 @@ __device__ inline void load_rec(const BlockRes& br, unsigned tl, TetRec<R>& r) {
   for (int kk = 0; kk < 9; ++kk) r.B[kk] = bload<R>(br.dm, vo, (unsigned)kk * br.row);
   r.dV = bload<R>(br.dv, vo, br.t0b);
   r.mu = bload<R>(br.mu, vo, br.t0b);
-  r.lam = (MAT == MAT_SNH) ? bload<R>(br.lam, vo, br.t0b) : R(0);
+  r.lam = (MAT != MAT_ARAP) ? bload<R>(br.lam, vo, br.t0b) : R(0);
+  if constexpr (MAT == MAT_MUSCLE) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) r.act[k] = bload<R>(br.act, 6u * vo + (unsigned)k * (unsigned)sizeof(R), 0u);
+  }
   r.cr = bload_u4(br.cr, tl * 16u, br.t0b / (unsigned)sizeof(R) * 16u);
 }
 
@@ -199,7 +207,7 @@ __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R, X> A) {
   constexpr int NCH = (HAS_G ? 3 : 0) + (HAS_H ? 3 : 0) + (HAS_HP ? 3 : 0);
   constexpr bool SCATTER = NCH > 0;
   constexpr int NCHA = NCH > 0 ? NCH : 1;
-  constexpr bool NEEDV = (WHAT & (W_HP | W_HQ | W_MIX)) != 0;
+  constexpr bool NEEDV = (WHAT & (W_HP | W_HQ | W_MIX | W_MIXA)) != 0;
   constexpr bool RED = (WHAT & (W_E | W_HQ)) != 0;
   constexpr int CS = 4 * NTH;  // channel stride of the contribution buffer
   static_assert(NCH <= MAX_CH, "too many channels");
@@ -324,7 +332,7 @@ __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R, X> A) {
             }
           }
           TetOut<R> o;
-          tet_eval<MAT, WHAT, R>(g, rec.mu, rec.lam, u, v, o);
+          tet_eval<MAT, WHAT, R>(g, rec.mu, rec.lam, rec.act, u, v, o);
           if constexpr ((WHAT & W_E) != 0) e_acc += o.e;
           if constexpr ((WHAT & W_HQ) != 0) q_acc += o.hq;
           if constexpr ((WHAT & W_MIX) != 0) {
@@ -332,10 +340,20 @@ __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R, X> A) {
             R* ob = A.out_b + t0;
             if (A.accumulate) {
               at(oa, (unsigned)tl) += o.dmu;
-              if (MAT == MAT_SNH) at(ob, (unsigned)tl) += o.dlam;
+              if (MAT != MAT_ARAP) at(ob, (unsigned)tl) += o.dlam;
             } else {
               at(oa, (unsigned)tl) = o.dmu;
-              if (MAT == MAT_SNH) at(ob, (unsigned)tl) = o.dlam;
+              if (MAT != MAT_ARAP) at(ob, (unsigned)tl) = o.dlam;
+            }
+          }
+          if constexpr ((WHAT & W_MIXA) != 0) {
+            R* oc = A.out_c + 6 * (size_t)t0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+              if (A.accumulate)
+                at(oc, 6u * (unsigned)tl + (unsigned)k) += o.dact[k];
+              else
+                at(oc, 6u * (unsigned)tl + (unsigned)k) = o.dact[k];
             }
           }
           if constexpr (SCATTER) {
@@ -502,7 +520,7 @@ __global__ __launch_bounds__(NTH) void finish_kernel(FinishArgs<R> A) {
 template <class R, class X>
 static ElemArgs<R, X> make_args(apl_mesh* m, const void* mu, const void* lam, const void* x, const void* p,
                                 const double* alpha, const int32_t* active, void* out_a, void* out_b,
-                                int accumulate) {
+                                int accumulate, void* out_c) {
   ElemArgs<R, X> A;
   A.dminv = (const R*)m->d.dminv;
   A.dV = (const R*)m->d.dV;
@@ -516,12 +534,14 @@ static ElemArgs<R, X> make_args(apl_mesh* m, const void* mu, const void* lam, co
   A.chunk_off = m->d.chunk_off;
   A.mu = (const R*)mu;
   A.lam = (const R*)lam;
+  A.act = (const R*)m->activation;
   A.x = (const X*)x;
   A.p = (const R*)p;
   A.alpha = alpha;
   A.active = active;
   A.out_a = (R*)out_a;
   A.out_b = (R*)out_b;
+  A.out_c = (R*)out_c;
   A.partial = (R*)m->d.partial;
   A.blk_red = m->d.blk_red;
   A.T = (int)m->L.T;
@@ -534,7 +554,7 @@ static ElemArgs<R, X> make_args(apl_mesh* m, const void* mu, const void* lam, co
 
 template <int MAT, unsigned WHAT, class R, class X>
 static size_t smem_bytes(const apl_mesh* m, bool filtered) {
-  constexpr bool NEEDV = (WHAT & (W_HP | W_HQ | W_MIX)) != 0;
+  constexpr bool NEEDV = (WHAT & (W_HP | W_HQ | W_MIX | W_MIXA)) != 0;
   constexpr int NCH = ((WHAT & W_G) ? 3 : 0) + ((WHAT & W_H) ? 3 : 0) + ((WHAT & W_HP) ? 3 : 0);
   size_t n = 8 * sizeof(double) + 4 * (size_t)m->L.nloc_max * (sizeof(X) + (NEEDV ? sizeof(R) : 0));
   n += sizeof(R) * (size_t)NCH * 4 * NTH;
@@ -544,8 +564,8 @@ static size_t smem_bytes(const apl_mesh* m, bool filtered) {
 
 template <int MAT, unsigned WHAT, class R, class X, int SPT>
 static int grid_for_spt(apl_mesh* m, bool filtered) {
-  static_assert(WHAT < 64, "pass variant index");
-  int& cached = m->grid_cache[MAT][WHAT + (sizeof(X) != sizeof(R) ? 64 : 0) + (filtered ? 128 : 0)];
+  static_assert(WHAT < 128, "pass variant index");
+  int& cached = m->grid_cache[MAT][WHAT + (sizeof(X) != sizeof(R) ? 128 : 0) + (filtered ? 256 : 0)];
   if (cached > 0) return cached;
   const size_t smem = smem_bytes<MAT, WHAT, R, X>(m, filtered);
   int occ = 0;
@@ -577,7 +597,7 @@ static int grid_for(apl_mesh* m, bool filtered) {
 template <int MAT, unsigned WHAT, class R, class X>
 static int launch_one(apl_mesh* m, const void* mu, const void* lam, const void* x, const void* p,
                       const double* alpha, const int32_t* active, void* out_a, void* out_b,
-                      int accumulate, int grid_override, hipStream_t s) {
+                      int accumulate, int grid_override, hipStream_t s, void* out_c = nullptr) {
   if (m->L.nblocks == 0) return 0;
   const bool filtered = active != nullptr;
   const size_t smem = smem_bytes<MAT, WHAT, R, X>(m, filtered);
@@ -586,7 +606,7 @@ static int launch_one(apl_mesh* m, const void* mu, const void* lam, const void* 
     return -1;
   }
   const int grid = grid_override > 0 ? ((grid_override + 7) / 8) * 8 : grid_for<MAT, WHAT, R, X>(m, filtered);
-  ElemArgs<R, X> A = make_args<R, X>(m, mu, lam, x, p, alpha, active, out_a, out_b, accumulate);
+  ElemArgs<R, X> A = make_args<R, X>(m, mu, lam, x, p, alpha, active, out_a, out_b, accumulate, out_c);
   constexpr bool SCATTER = (WHAT & (W_G | W_H | W_HP)) != 0;
   const int spt = SCATTER ? spt_for(m) : 1;
   if (spt == 1)
@@ -612,7 +632,14 @@ static int launch_one(apl_mesh* m, const void* mu, const void* lam, const void* 
 template <int MAT, class R>
 static int dispatch_what(apl_mesh* m, unsigned what, const void* mu, const void* lam, const void* x,
                          const void* p, const double* alpha, const int32_t* active, void* out_a,
-                         void* out_b, int accumulate, int grid_override, int x_f64, hipStream_t s) {
+                         void* out_b, int accumulate, int grid_override, int x_f64, hipStream_t s, void* out_c) {
+  if constexpr (MAT == MAT_MUSCLE) {
+    // activation derivatives, alone or with the mu / lambda ones
+    if (what == W_MIXA)
+      return launch_one<MAT, W_MIXA, R, R>(m, mu, lam, x, p, alpha, active, out_a, out_b, accumulate, grid_override, s, out_c);
+    if (what == (W_MIX | W_MIXA))
+      return launch_one<MAT, W_MIX | W_MIXA, R, R>(m, mu, lam, x, p, alpha, active, out_a, out_b, accumulate, grid_override, s, out_c);
+  }
   if (x_f64 && sizeof(R) == 4) {
     // solver passes over FP64 displacements
     if (what == (W_E | W_G | W_H))
@@ -632,16 +659,21 @@ static int dispatch_what(apl_mesh* m, unsigned what, const void* mu, const void*
 
 int launch_elem(apl_mesh* m, int material, unsigned what, const void* mu, const void* lam, const void* x,
                 const void* p, const double* alpha, const int32_t* active, void* out_a, void* out_b,
-                int accumulate, int grid_override, hipStream_t s, int x_f64) {
-  if (material == MAT_SNH) {
+                int accumulate, int grid_override, hipStream_t s, int x_f64, void* out_c) {
+  auto go = [&](auto mat_c) -> int {
+    constexpr int MAT = decltype(mat_c)::value;
     if (m->dtype == DT_F32)
-      return dispatch_what<MAT_SNH, float>(m, what, mu, lam, x, p, alpha, active, out_a, out_b, accumulate, grid_override, x_f64, s);
-    return dispatch_what<MAT_SNH, double>(m, what, mu, lam, x, p, alpha, active, out_a, out_b, accumulate, grid_override, 0, s);
-  }
-  if (material == MAT_ARAP) {
-    if (m->dtype == DT_F32)
-      return dispatch_what<MAT_ARAP, float>(m, what, mu, lam, x, p, alpha, active, out_a, out_b, accumulate, grid_override, x_f64, s);
-    return dispatch_what<MAT_ARAP, double>(m, what, mu, lam, x, p, alpha, active, out_a, out_b, accumulate, grid_override, 0, s);
+      return dispatch_what<MAT, float>(m, what, mu, lam, x, p, alpha, active, out_a, out_b, accumulate, grid_override, x_f64, s, out_c);
+    return dispatch_what<MAT, double>(m, what, mu, lam, x, p, alpha, active, out_a, out_b, accumulate, grid_override, 0, s, out_c);
+  };
+  if (material == MAT_SNH) return go(std::integral_constant<int, MAT_SNH>{});
+  if (material == MAT_ARAP) return go(std::integral_constant<int, MAT_ARAP>{});
+  if (material == MAT_MUSCLE) {
+    if (m->activation == nullptr && m->L.T > 0) {
+      set_error("the muscle law needs the activation: call apl_mesh_set_activation first");
+      return -1;
+    }
+    return go(std::integral_constant<int, MAT_MUSCLE>{});
   }
   set_error("unknown material");
   return -1;
@@ -663,6 +695,7 @@ static int grid_mat(apl_mesh* m, unsigned what, int x_f64, bool filtered) {
 }
 
 int elem_grid(apl_mesh* m, int material, unsigned what, int x_f64, bool filtered) {
+  if (material == MAT_MUSCLE) return grid_mat<MAT_MUSCLE>(m, what, x_f64, filtered);
   return material == MAT_SNH ? grid_mat<MAT_SNH>(m, what, x_f64, filtered)
                              : grid_mat<MAT_ARAP>(m, what, x_f64, filtered);
 }

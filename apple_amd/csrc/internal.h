@@ -61,9 +61,10 @@ struct apl_mesh {
   apl::Layout L;
   apl::DevLayout d;
   size_t device_bytes = 0;
-  int grid_cache[2][256] = {};  // persistent grid per (material, pass variant, FP64 displacements, filtered)
+  int grid_cache[3][512] = {};  // persistent grid per (material, pass variant, FP64 displacements, filtered)
   int refs = 1;                 // the caller's handle plus one per solver built on the mesh
   int grid_pct = 100;           // persistent element grid, percent of occupancy x CUs (autotuned)
+  const void* activation = nullptr;  // caller-owned (T, 6) muscle activation, read at every muscle launch
 };
 
 namespace apl {
@@ -85,7 +86,8 @@ inline size_t real_size(const apl_mesh* m) { return m->dtype == DT_F64 ? 8 : 4; 
 // x_f64: x holds FP64 displacements while the mesh is FP32 (solver passes)
 int launch_elem(apl_mesh* m, int material, unsigned what, const void* mu, const void* lam,
                 const void* x, const void* p, const double* alpha, const int32_t* active,
-                void* out_a, void* out_b, int accumulate, int grid_override, hipStream_t s, int x_f64 = 0);
+                void* out_a, void* out_b, int accumulate, int grid_override, hipStream_t s, int x_f64 = 0,
+                void* out_c = nullptr);
 // vertex sums of the slot partials into the (V,3) outputs
 // Finish pass of an element pass in one launch: vertex sums of the slot
 // partials into out_a / out_b (what: the scatter channels of the pass) and,

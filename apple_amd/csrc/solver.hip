@@ -737,9 +737,9 @@ static int pncg_solve_t(apl_solver* s, const void* mu, const void* lam, void* x,
   // cached graph of K iterations for these pointers and parameters
   bool use_graph = prm->use_graph != 0;
   if (use_graph) {
-    std::vector<char> key(sizeof(void*) * 5 + sizeof(Params) + 3 * sizeof(int));
+    std::vector<char> key(sizeof(void*) * 6 + sizeof(Params) + 3 * sizeof(int));
     char* k = key.data();
-    const void* ptrs[5] = {mu, lam, x, fext, mask};
+    const void* ptrs[6] = {mu, lam, x, fext, mask, m->activation};
     std::memcpy(k, ptrs, sizeof ptrs);
     k += sizeof ptrs;
     std::memcpy(k, &P, sizeof P);
@@ -926,8 +926,8 @@ int apl_solver_create(apl_mesh* m, int material, apl_solver** out) {
     set_error("NULL argument");
     return -1;
   }
-  if (material != APL_SNH && material != APL_ARAP) {
-    set_error("material must be APL_SNH or APL_ARAP");
+  if (material != APL_SNH && material != APL_ARAP && material != APL_SNH_MUSCLE) {
+    set_error("material must be APL_SNH, APL_ARAP or APL_SNH_MUSCLE");
     return -1;
   }
   *out = nullptr;
@@ -1002,7 +1002,7 @@ int apl_solver_destroy(apl_solver* s) {
 int apl_pncg_solve(apl_solver* s, const void* mu, const void* lam, void* x, const void* f_ext,
                    const uint8_t* fixed_mask, const apl_pncg_params* params, apl_pncg_stats* stats, void* stream) {
   if (s == nullptr || params == nullptr || (s->mesh->L.V > 0 && x == nullptr) ||
-      (s->mesh->L.T > 0 && (mu == nullptr || (s->material == APL_SNH && lam == nullptr)))) {
+      (s->mesh->L.T > 0 && (mu == nullptr || (s->material != APL_ARAP && lam == nullptr)))) {
     set_error("NULL argument");
     return -1;
   }
@@ -1020,7 +1020,7 @@ int apl_pncg_solve(apl_solver* s, const void* mu, const void* lam, void* x, cons
 int apl_cg_solve(apl_solver* s, const void* mu, const void* lam, const void* x, const void* rhs, void* y,
                  const uint8_t* fixed_mask, double rtol, int max_iters, apl_pncg_stats* stats, void* stream) {
   if (s == nullptr || (s->mesh->L.V > 0 && (x == nullptr || rhs == nullptr || y == nullptr)) ||
-      (s->mesh->L.T > 0 && (mu == nullptr || (s->material == APL_SNH && lam == nullptr)))) {
+      (s->mesh->L.T > 0 && (mu == nullptr || (s->material != APL_ARAP && lam == nullptr)))) {
     set_error("NULL argument");
     return -1;
   }

@@ -13,6 +13,11 @@
 //     J - 1 = tr G + c2(G) + det G, c2 = sum of principal 2x2 minors,
 //     cof F = I + tr(G) I - G^T + cof G,
 //     P = mu (F - cof F) + lam (J-1) cof F,  F - cof F = G + G^T - tr(G) I - cof G.
+//   SNH muscle (warp/fem/_stable_neo_hookean_muscle.py:18-113): the SNH law on
+//     G_A = F A with A = I + S, S the symmetric matrix of the per-tet
+//     activation (a0 a3 a4; a3 a1 a5; a4 a5 a2) (warp/fem/func/_misc.py:48-53).
+//     With d'_a = A d_a (the rows of dhdX A), G_A - I = S + sum_a u_a d'_a^T,
+//     so it is the SNH evaluation on the geometry d' plus the offset S.
 //   ARAP (warp/fem/_arap.py:17-76, func/_misc.py:31-70)
 //     F^T F = I + E with E = G + G^T + G^T G; E's eigenpairs give V and
 //     sigma_i^2 = 1 + e_i, so sigma_i - 1 = e_i / (1 + sqrt(1 + e_i)) without
@@ -32,7 +37,7 @@
 
 namespace apl {
 
-enum Material : int { MAT_SNH = 0, MAT_ARAP = 1 };
+enum Material : int { MAT_SNH = 0, MAT_ARAP = 1, MAT_MUSCLE = 2, N_MAT = 3 };
 
 template <class R>
 struct V3 {
@@ -580,6 +585,7 @@ enum : unsigned {
   W_HP = 8u,     // Hessian-vector product (4 x vec3)
   W_HQ = 16u,    // clamped quadratic form v^T H v
   W_MIX = 32u,   // mixed derivatives p^T dg/dmu, p^T dg/dlam (per tet)
+  W_MIXA = 64u,  // mixed derivatives p^T dg/da_k, k = 0..5 (muscle activation, per tet)
 };
 
 template <class R>
@@ -590,39 +596,64 @@ struct TetOut {
   V3<R> hp[4];  // dV H v
   R hq;         // max(dV v^T H v, 0)
   R dmu, dlam;  // dV (dP/dmu):dF(v), dV (dP/dlam):dF(v)
+  R dact[6];    // dV d(P:dF(v))/da_k (muscle)
 };
 
-// Evaluate the requested quantities for one tet.  u: displacements, v: the
-// direction for W_HP / W_HQ / W_MIX.
+// Evaluate the requested quantities for one tet.
 // u: edge differences (u[a] = displacement of corner a minus that of corner
-// 0; u[0] is not read); v: the direction at the four corners.
+// 0; u[0] is not read); v: the direction at the four corners for W_HP /
+// W_HQ / W_MIX / W_MIXA; act: the six activation values (muscle only).
 template <int MAT, unsigned WHAT, class R>
-__host__ __device__ inline void tet_eval(const TetGeom<R>& g, R mu, R lam, const V3<R> u[4],
+__host__ __device__ inline void tet_eval(const TetGeom<R>& g0, R mu, R lam, const R* act, const V3<R> u[4],
                                          const V3<R> v[4], TetOut<R>& o) {
-  const M3<R> G = outer_sum_edges(g, u);
+  constexpr bool needV = (WHAT & (W_HP | W_HQ | W_MIX | W_MIXA)) != 0;
+  static_assert(MAT == MAT_MUSCLE || (WHAT & W_MIXA) == 0, "activation derivatives need the muscle law");
+  // muscle: the SNH law on the geometry d'_a = A d_a, G offset by S = A - I
+  TetGeom<R> g = g0;
+  M3<R> S;
+  if constexpr (MAT == MAT_MUSCLE) {
+    S.m[0][0] = act[0];
+    S.m[1][1] = act[1];
+    S.m[2][2] = act[2];
+    S.m[0][1] = S.m[1][0] = act[3];
+    S.m[0][2] = S.m[2][0] = act[4];
+    S.m[1][2] = S.m[2][1] = act[5];
+#pragma unroll
+    for (int a = 1; a < 4; ++a) g.d[a] = g0.d[a] + mul(S, g0.d[a]);
+    g.d[0] = v3(-(g.d[1].x + g.d[2].x + g.d[3].x), -(g.d[1].y + g.d[2].y + g.d[3].y),
+                -(g.d[1].z + g.d[2].z + g.d[3].z));
+  }
+  M3<R> G = outer_sum_edges(g, u);
+  if constexpr (MAT == MAT_MUSCLE) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) G.m[i][j] += S.m[i][j];
+  }
   const Inv<R> I = invariants(G);
-  constexpr bool needV = (WHAT & (W_HP | W_HQ | W_MIX)) != 0;
-  M3<R> dF;
+  M3<R> dF;  // dF(v) A for the muscle (the direction in G)
   if constexpr (needV) dF = outer_sum(g, v);
   // dV folded into the moduli once: every output below is per unit volume
   // times dV (warp/fem/_base.py multiplies each result by dV separately)
   const R muV = g.dV * mu;
-  if constexpr (MAT == MAT_SNH) {
+  if constexpr (MAT != MAT_ARAP) {
     const R lamV = g.dV * lam;
     if constexpr ((WHAT & W_E) != 0)
       o.e = R(0.5) * muV * I.G2 - muV * (I.c2 + I.detG) + R(0.5) * lamV * I.Jm1 * I.Jm1;
-    constexpr bool needC = (WHAT & (W_G | W_H | W_HP | W_HQ | W_MIX)) != 0;
+    constexpr bool needC = (WHAT & (W_G | W_H | W_HP | W_HQ | W_MIX | W_MIXA)) != 0;
     if constexpr (needC) {
       const M3<R> C = cofF_from_G(G, I.trG);  // g3 = cof F
-      if constexpr ((WHAT & W_G) != 0) {
+      M3<R> P;   // first Piola-Kirchhoff stress times dV
+      M3<R> Tm;  // Hessian applied to dF, times dV
+      if constexpr ((WHAT & (W_G | W_MIXA)) != 0) {
         const M3<R> D = F_minus_cofF(G, I.trG);
-        M3<R> P;
         const R lj = lamV * I.Jm1;
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
           for (int j = 0; j < 3; ++j) P.m[i][j] = muV * D.m[i][j] + lj * C.m[i][j];
-        corner_products(P, g, o.g);
+        // muscle: P_F = P_G A^T, and P_F d_a = P_G d'_a
+        if constexpr ((WHAT & W_G) != 0) corner_products(P, g, o.g);
       }
       if constexpr ((WHAT & W_H) != 0) {
         V3<R> wc[4];
@@ -637,7 +668,7 @@ __host__ __device__ inline void tet_eval(const TetGeom<R>& g, R mu, R lam, const
           o.h[a] = v3(hx > R(0) ? hx : R(0), hy > R(0) ? hy : R(0), hz > R(0) ? hz : R(0));
         }
       }
-      if constexpr ((WHAT & (W_HP | W_HQ | W_MIX)) != 0) {
+      if constexpr ((WHAT & (W_HP | W_HQ | W_MIX | W_MIXA)) != 0) {
         const R sV = lamV * I.Jm1 - muV;
         const R cd = ddot(C, dF);
         if constexpr ((WHAT & W_HQ) != 0) {
@@ -647,7 +678,7 @@ __host__ __device__ inline void tet_eval(const TetGeom<R>& g, R mu, R lam, const
           R q = lamV * cd * cd + muV * ddot(dF, dF) + sV * h6;
           o.hq = q > R(0) ? q : R(0);
         }
-        if constexpr ((WHAT & W_HP) != 0) {
+        if constexpr ((WHAT & (W_HP | W_MIXA)) != 0) {
           // Hp = d/dt cof(F + t dF) (h6_prod, func/_hess_prod.py:61-77)
           V3<R> f0 = v3(R(1) + G.m[0][0], G.m[1][0], G.m[2][0]);
           V3<R> f1 = v3(G.m[0][1], R(1) + G.m[1][1], G.m[2][1]);
@@ -657,18 +688,37 @@ __host__ __device__ inline void tet_eval(const TetGeom<R>& g, R mu, R lam, const
           set_col(Hs, 0, cross(f1, p2) - cross(f2, p1));
           set_col(Hs, 1, cross(f2, p0) - cross(f0, p2));
           set_col(Hs, 2, cross(f0, p1) - cross(f1, p0));
-          M3<R> T;
           const R lc = lamV * cd;
 #pragma unroll
           for (int i = 0; i < 3; ++i)
 #pragma unroll
-            for (int j = 0; j < 3; ++j) T.m[i][j] = lc * C.m[i][j] + muV * dF.m[i][j] + sV * Hs.m[i][j];
-          corner_products(T, g, o.hp);
+            for (int j = 0; j < 3; ++j) Tm.m[i][j] = lc * C.m[i][j] + muV * dF.m[i][j] + sV * Hs.m[i][j];
+          if constexpr ((WHAT & W_HP) != 0) corner_products(Tm, g, o.hp);
         }
         if constexpr ((WHAT & W_MIX) != 0) {
           const M3<R> D = F_minus_cofF(G, I.trG);
           o.dmu = g.dV * ddot(D, dF);
           o.dlam = g.dV * I.Jm1 * cd;
+        }
+        if constexpr ((WHAT & W_MIXA) != 0) {
+          // d/da_k [P_G(F A) : (dF0 A)] = (F^T H[dG] + dF0^T P_G) : E_k with
+          // E_k = dA/da_k and F, dF0 in the rest geometry (H symmetric:
+          // H[F E_k] : dG = H[dG] : F E_k)
+          const M3<R> F0 = outer_sum_edges(g0, u);
+          const M3<R> dF0 = outer_sum(g0, v);
+          M3<R> N;
+#pragma unroll
+          for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+              N.m[i][j] = Tm.m[i][j] + (F0.m[0][i] * Tm.m[0][j] + F0.m[1][i] * Tm.m[1][j] + F0.m[2][i] * Tm.m[2][j]) +
+                          (dF0.m[0][i] * P.m[0][j] + dF0.m[1][i] * P.m[1][j] + dF0.m[2][i] * P.m[2][j]);
+          o.dact[0] = N.m[0][0];
+          o.dact[1] = N.m[1][1];
+          o.dact[2] = N.m[2][2];
+          o.dact[3] = N.m[0][1] + N.m[1][0];
+          o.dact[4] = N.m[0][2] + N.m[2][0];
+          o.dact[5] = N.m[1][2] + N.m[2][1];
         }
       }
     }
@@ -718,6 +768,14 @@ __host__ __device__ inline void tet_eval(const TetGeom<R>& g, R mu, R lam, const
       }
     }
   }
+}
+
+// SNH and ARAP need no activation
+template <int MAT, unsigned WHAT, class R>
+__host__ __device__ inline void tet_eval(const TetGeom<R>& g, R mu, R lam, const V3<R> u[4], const V3<R> v[4],
+                                         TetOut<R>& o) {
+  static_assert(MAT != MAT_MUSCLE, "the muscle law needs the activation");
+  tet_eval<MAT, WHAT, R>(g, mu, lam, static_cast<const R*>(nullptr), u, v, o);
 }
 
 }  // namespace apl

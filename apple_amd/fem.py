@@ -34,7 +34,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .common import FRACTION, LAMBDA, MU, default_name
+from .common import ACTIVATION, FRACTION, LAMBDA, MU, default_name
 from .mesh import Mesh, Region
 from .runtime import DeviceMesh, as_dtype
 
@@ -146,29 +146,36 @@ class FemPotential:
     def _lam(self):
         return self.materials.get("lambda")
 
+    def _prepare(self) -> None:
+        """Hook run before every element pass (binds per-mesh material arrays)."""
+
+    def _eval(self, u, p=None, **outputs) -> None:
+        self._prepare()
+        self.mesh.eval(self.material, self._mu(), self._lam(), u, p, **outputs)
+
     # ------------------------------------------------------------ reference protocol
 
     def _add_scalar(self, output: torch.Tensor, which: str, u, p=None) -> None:
         if output.dtype == torch.float64 and output.is_contiguous() and output.numel() == 1:
             kw = {which: output}
-            self.mesh.eval(self.material, self._mu(), self._lam(), u, p, accumulate=True, **kw)
+            self._eval(u, p, accumulate=True, **kw)
             return
         self._energy.zero_()
         kw = {which: self._energy}
-        self.mesh.eval(self.material, self._mu(), self._lam(), u, p, accumulate=True, **kw)
+        self._eval(u, p, accumulate=True, **kw)
         output += self._energy.to(output.dtype)
 
     def fun(self, u: torch.Tensor, output: torch.Tensor) -> None:
         self._add_scalar(output, "energy", u)
 
     def grad(self, u: torch.Tensor, output: torch.Tensor) -> None:
-        self.mesh.eval(self.material, self._mu(), self._lam(), u, grad=output, accumulate=True)
+        self._eval(u, grad=output, accumulate=True)
 
     def hess_diag(self, u: torch.Tensor, output: torch.Tensor) -> None:
-        self.mesh.eval(self.material, self._mu(), self._lam(), u, hess_diag=output, accumulate=True)
+        self._eval(u, hess_diag=output, accumulate=True)
 
     def hess_prod(self, u: torch.Tensor, p: torch.Tensor, output: torch.Tensor) -> None:
-        self.mesh.eval(self.material, self._mu(), self._lam(), u, p, hess_prod=output, accumulate=True)
+        self._eval(u, p, hess_prod=output, accumulate=True)
 
     def hess_quad(self, u: torch.Tensor, p: torch.Tensor, output: torch.Tensor) -> None:
         self._add_scalar(output, "hess_quad", u, p)
@@ -177,16 +184,15 @@ class FemPotential:
 
     def fused(self, u, energy=None, grad=None, hess_diag=None, *, accumulate=True) -> None:
         """Energy (float64 (1,)), gradient and Hessian diagonal in one element pass."""
-        self.mesh.eval(
-            self.material, self._mu(), self._lam(), u, energy=energy, grad=grad, hess_diag=hess_diag,
-            accumulate=accumulate,
+        self._eval(
+            u, energy=energy, grad=grad, hess_diag=hess_diag, accumulate=accumulate,
         )
 
     def mixed_derivative_prod(self, u: torch.Tensor, p: torch.Tensor) -> dict[str, torch.Tensor]:
         """Per-tet p^T d(grad)/d(mu) (and d/d(lambda) for SNH)."""
         dmu = torch.empty(self.n_cells, dtype=self.dtype, device=self.device)
-        dlam = torch.empty_like(dmu) if self.material == _lib.APL_SNH else None
-        self.mesh.eval(self.material, self._mu(), self._lam(), u, p, dmu=dmu, dlam=dlam)
+        dlam = torch.empty_like(dmu) if self.material != _lib.APL_ARAP else None
+        self._eval(u, p, dmu=dmu, dlam=dlam)
         out = {"mu": dmu}
         if dlam is not None:
             out["lambda"] = dlam
@@ -220,3 +226,47 @@ class Arap(FemPotential):
     @classmethod
     def materials_from_region(cls, region, mesh):
         return {"mu": cls._cell_tensor(region, MU.vtk, mesh)}
+
+
+class StableNeoHookeanMuscle(StableNeoHookean):
+    """Stable Neo-Hookean on G = F A (warp/fem/_stable_neo_hookean_muscle.py:18-171),
+    A = I + [[a0, a3, a4], [a3, a1, a5], [a4, a5, a2]] from the per-tet
+    activation (warp/fem/func/_misc.py:48-53); materials activation (T, 6),
+    lambda, mu (Materials, :120-133)."""
+
+    material = _lib.APL_SNH_MUSCLE
+    material_names = ("activation", "lambda", "mu")
+
+    @classmethod
+    def materials_from_region(cls, region, mesh):
+        # utils/_material.py:15-16 reads cell_data[ACTIVATION] (the enum value)
+        key = ACTIVATION.value
+        if key not in region.cell_data:
+            raise KeyError(f"cell_data[{key!r}] is required")
+        act = np.asarray(region.cell_data[key], np.float64).reshape(region.n_cells, 6)
+        out = super().materials_from_region(region, mesh)
+        out["activation"] = torch.as_tensor(act, dtype=mesh.dtype, device=mesh.device).contiguous()
+        return out
+
+    def set_materials(self, **values: torch.Tensor) -> None:
+        act = values.pop("activation", None)
+        if act is not None:
+            act = torch.as_tensor(act, dtype=self.mesh.dtype, device=self.mesh.device).reshape(-1, 6).contiguous()
+            if act.shape[0] != self.mesh.n_tets:
+                raise ValueError("activation must have one row of 6 per tet")
+            self.materials["activation"] = act
+        super().set_materials(**values)
+
+    def _prepare(self) -> None:
+        self.mesh.set_activation(self.materials["activation"])
+
+    def mixed_derivative_prod(self, u: torch.Tensor, p: torch.Tensor) -> dict[str, torch.Tensor]:
+        """Per-tet p^T d(grad)/d(mu), d/d(lambda) and d/d(activation) (T, 6):
+        the mixed term of the activation inverse
+        (exp/2026/01/28/smas/src/31-inverse-activation-stable-neo-hookean.py:472-486)."""
+        self._prepare()
+        dmu = torch.empty(self.n_cells, dtype=self.dtype, device=self.device)
+        dlam = torch.empty_like(dmu)
+        dact = torch.empty(self.n_cells, 6, dtype=self.dtype, device=self.device)
+        self.mesh.mixed_activation(self._mu(), self._lam(), u, p, dact, dmu=dmu, dlam=dlam)
+        return {"mu": dmu, "lambda": dlam, "activation": dact}

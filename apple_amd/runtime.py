@@ -125,6 +125,14 @@ class DeviceMesh:
         _lib.check(_lib.lib().apl_mesh_set_schedule(self._handle, int(grid_pct)))
         self.grid_pct = int(grid_pct)
 
+    def set_activation(self, activation: torch.Tensor | None) -> None:
+        """Bind the (T, 6) activation that muscle-law launches read (the mesh
+        keeps the pointer; this object keeps the tensor alive)."""
+        if activation is not None:
+            ptr(activation, dtype=self.dtype, device=self.device, shape=(self.n_tets, 6), name="activation")
+        _lib.check(_lib.lib().apl_mesh_set_activation(self._handle, activation.data_ptr() if activation is not None else None))
+        self._activation = activation
+
     def info(self) -> dict:
         inf = _lib.MeshInfo()
         _lib.check(_lib.lib().apl_mesh_info(self._handle, ctypes.byref(inf)))
@@ -186,7 +194,7 @@ class DeviceMesh:
                 self._handle,
                 int(material),
                 ptr(mu, name="mu", **tet),
-                ptr(lam, name="lambda", **tet) if material == _lib.APL_SNH else None,
+                ptr(lam, name="lambda", **tet) if material != _lib.APL_ARAP else None,
                 ptr(u, **self.vec("u")),
                 ptr(p, **self.vec("p")),
                 what,
@@ -197,6 +205,25 @@ class DeviceMesh:
                 ptr(hess_quad, name="hess_quad", **sc),
                 ptr(dmu, name="dmu", **tet),
                 ptr(dlam, name="dlam", **tet),
+                current_stream(self.device),
+            )
+        )
+
+    def mixed_activation(self, mu, lam, u, p, dact, *, dmu=None, dlam=None, accumulate=False) -> None:
+        """Muscle law: per-tet p^T d(grad)/d(activation) into dact (T, 6), and
+        optionally p^T d(grad)/d(mu), d/d(lambda) in the same pass."""
+        tet = dict(dtype=self.dtype, device=self.device, shape=(self.n_tets,))
+        _lib.check(
+            _lib.lib().apl_elem_mixed_activation(
+                self._handle,
+                ptr(mu, name="mu", **tet),
+                ptr(lam, name="lambda", **tet),
+                ptr(u, **self.vec("u")),
+                ptr(p, **self.vec("p")),
+                ptr(dact, dtype=self.dtype, device=self.device, shape=(self.n_tets, 6), name="dact"),
+                ptr(dmu, name="dmu", **tet),
+                ptr(dlam, name="dlam", **tet),
+                1 if accumulate else 0,
                 current_stream(self.device),
             )
         )

@@ -112,23 +112,26 @@ class DeviceSolver:
         except Exception:
             pass
 
-    def _mat_ptrs(self, mu=None, lam=None):
-        """Per-tet materials: the given tensors, else the potential's."""
+    def _mat_ptrs(self, mu=None, lam=None, activation=None):
+        """Per-tet materials: the given tensors, else the potential's (the
+        muscle activation is bound to the mesh)."""
         m = self.mesh
         tet = dict(dtype=m.dtype, device=m.device, shape=(m.n_tets,))
         mu = self.potential.materials["mu"] if mu is None else mu
         lam = self.potential.materials.get("lambda") if lam is None else lam
-        if self.potential.material != _lib.APL_SNH:
+        if self.potential.material == _lib.APL_ARAP:
             lam = None
+        if self.potential.material == _lib.APL_SNH_MUSCLE:
+            m.set_activation(self.potential.materials["activation"] if activation is None else activation)
         return ptr(mu, name="mu", **tet), (ptr(lam, name="lambda", **tet) if lam is not None else None)
 
     def solve(self, x: torch.Tensor, f_ext: torch.Tensor | None = None, mask: torch.Tensor | None = None,
-              config: PncgConfig | None = None, *, mu=None, lam=None) -> list[SolveStats]:
+              config: PncgConfig | None = None, *, mu=None, lam=None, activation=None) -> list[SolveStats]:
         """Minimise in place from ``x`` (fixed DOFs already at their values);
-        mu / lam override the potential's materials for this call."""
+        mu / lam / activation override the potential's materials for this call."""
         config = config or PncgConfig()
         m = self.mesh
-        mu, lam = self._mat_ptrs(mu, lam)
+        mu, lam = self._mat_ptrs(mu, lam, activation)
         params = config.to_c(m.dtype)
         stats = (_lib.PncgStats * m.n_inst)()
         _lib.check(
@@ -141,10 +144,10 @@ class DeviceSolver:
         return _stats(stats, m.n_inst)
 
     def cg(self, x: torch.Tensor, rhs: torch.Tensor, y: torch.Tensor, mask: torch.Tensor | None = None,
-           rtol: float = 1e-6, max_iters: int = 10000, *, mu=None, lam=None) -> list[SolveStats]:
-        """Solve H(x) y = rhs on the free DOFs (y overwritten); mu / lam as in solve()."""
+           rtol: float = 1e-6, max_iters: int = 10000, *, mu=None, lam=None, activation=None) -> list[SolveStats]:
+        """Solve H(x) y = rhs on the free DOFs (y overwritten); materials as in solve()."""
         m = self.mesh
-        mu, lam = self._mat_ptrs(mu, lam)
+        mu, lam = self._mat_ptrs(mu, lam, activation)
         stats = (_lib.PncgStats * m.n_inst)()
         _lib.check(
             _lib.lib().apl_cg_solve(

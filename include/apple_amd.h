@@ -25,8 +25,10 @@ typedef struct apl_mesh apl_mesh;
 typedef struct apl_solver apl_solver;
 
 enum { APL_F32 = 0, APL_F64 = 1 };
-/* materials: warp/fem/_stable_neo_hookean.py, warp/fem/_arap.py */
-enum { APL_SNH = 0, APL_ARAP = 1 };
+/* materials: warp/fem/_stable_neo_hookean.py, warp/fem/_arap.py,
+ * warp/fem/_stable_neo_hookean_muscle.py (SNH on F A, A = I + sym(activation);
+ * the activation is bound to the mesh by apl_mesh_set_activation) */
+enum { APL_SNH = 0, APL_ARAP = 1, APL_SNH_MUSCLE = 2 };
 
 /* output selection for apl_elem_eval */
 enum {
@@ -86,6 +88,15 @@ int apl_mesh_info(const apl_mesh* mesh, apl_mesh_info_t* info /* host */);
  * until the solver's parameters change. */
 int apl_mesh_set_schedule(apl_mesh* mesh, int grid_pct);
 
+/* Activation of the muscle law (StableNeoHookeanMuscle.Materials.activation,
+ * warp/fem/_stable_neo_hookean_muscle.py:120-133, warp/fem/utils/_material.py:15-16):
+ * a device array (T, 6) in the mesh dtype, row t = (a0..a5) with
+ * A_t = I + [[a0, a3, a4], [a3, a1, a5], [a4, a5, a2]] (warp/fem/func/_misc.py:48-53).
+ * The mesh keeps the pointer, not a copy: every later APL_SNH_MUSCLE launch on
+ * the mesh (element passes, solves) reads the array's current contents, so it
+ * must stay allocated while such work is queued.  NULL unbinds it. */
+int apl_mesh_set_activation(apl_mesh* mesh, const void* activation);
+
 /* One fused element pass (replaces the five launches of warp/fem/_base.py:151-194).
  * u, p: (V,3); mu, lam: (T,) (lam ignored for ARAP); energy, hess_quad:
  * (n_inst,) float64; grad, hess_diag, hess_prod: (V,3); dmu, dlam: (T,).
@@ -93,6 +104,14 @@ int apl_mesh_set_schedule(apl_mesh* mesh, int grid_pct);
 int apl_elem_eval(apl_mesh* mesh, int material, const void* mu, const void* lam, const void* u, const void* p,
                   unsigned what, double* energy, void* grad, void* hess_diag, void* hess_prod, double* hess_quad,
                   void* dmu, void* dlam, void* stream);
+
+/* Mixed derivatives of the muscle law for the activation adjoint
+ * (mixed_derivative_prod of exp/2026/01/28/smas/src/
+ * 31-inverse-activation-stable-neo-hookean.py:472-486): per tet,
+ * dact[t, k] = p^T d(grad)/d(a_k) (T, 6), and optionally dmu, dlam (T,) as
+ * APL_MIXED computes them.  accumulate: add into the outputs. */
+int apl_elem_mixed_activation(apl_mesh* mesh, const void* mu, const void* lam, const void* u, const void* p,
+                              void* dact, void* dmu, void* dlam, int accumulate, void* stream);
 
 /* ExternalForce (warp/potential/_ext_force.py:17-39): E -= f . u[idx],
  * grad[idx] -= f for n entries with unique indices.  energy: (1,) float64. */

@@ -186,6 +186,87 @@ def snh_dpk1_dparams(F):
 
 
 # --------------------------------------------------------------------------
+# stable Neo-Hookean muscle: warp/fem/_stable_neo_hookean_muscle.py:18-113
+# --------------------------------------------------------------------------
+
+#: index pairs of the activation vector (warp/fem/func/_misc.py:48-53)
+ACTIVATION_PAIRS = ((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))
+
+
+def activation_matrix(a):
+    """A = I + [[a0, a3, a4], [a3, a1, a5], [a4, a5, a2]] -- func/_misc.py:48-53."""
+    a = np.asarray(a, np.float64).reshape(-1, 6)
+    A = np.repeat(np.eye(3)[None], a.shape[0], axis=0)
+    for k, (i, j) in enumerate(ACTIVATION_PAIRS):
+        A[:, i, j] += a[:, k]
+        if i != j:
+            A[:, j, i] += a[:, k]
+    return A
+
+
+def muscle_energy_density(F, mu, lam, a):
+    """SNH energy of G = F A -- :18-30."""
+    return snh_energy_density(F @ activation_matrix(a), mu, lam)
+
+
+def muscle_pk1(F, mu, lam, a):
+    """(dPsi/dI2 g2(G) + dPsi/dI3 g3(G)) A^T -- :33-43."""
+    A = activation_matrix(a)
+    return snh_pk1(F @ A, mu, lam) @ np.swapaxes(A, 1, 2)
+
+
+def muscle_hess_diag(F, dhdX, mu, lam, a):
+    """The SNH diagonal terms with G = F A and dhdX A -- :46-75."""
+    A = activation_matrix(a)
+    return snh_hess_diag(F @ A, dhdX @ A, mu, lam)
+
+
+def muscle_hess_prod(F, p_cell, dhdX, mu, lam, a):
+    """lam h3_prod + (mu/2) h5_prod + s h6_prod on (G, dhdX A) -- :78-92."""
+    A = activation_matrix(a)
+    return snh_hess_prod(F @ A, p_cell, dhdX @ A, mu, lam)
+
+
+def muscle_hess_quad(F, p_cell, dhdX, mu, lam, a):
+    """lam h3_quad + (mu/2) h5_quad + s h6_quad on (G, dhdX A) -- :95-113."""
+    A = activation_matrix(a)
+    return snh_hess_quad(F @ A, p_cell, dhdX @ A, mu, lam)
+
+
+def snh_hess_bilinear(F, X, Y, mu, lam):
+    """X : d^2 Psi/dF^2 : Y of the SNH law (per tet), from its three terms.
+
+    lam (g3 : X)(g3 : Y) + mu X : Y + s Y : h6(F, X), the bilinear form whose
+    diagonal h6_quad and whose matrix form h6_prod are (func/_hess_prod.py,
+    _hess_quad.py).
+    """
+    J = I3(F)
+    s = -mu + lam * (J - 1.0)
+    G3 = g3(F)
+    return lam * ddot(G3, X) * ddot(G3, Y) + mu * ddot(X, Y) + s * ddot(Y, h6_matrix(F, X))
+
+
+def muscle_mixed(F, dF, mu, lam, a):
+    """p^T d(grad)/dq per tet (without dV) for q = mu, lambda, a_0..a_5.
+
+    grad . p = P_G(F A) : (dF A); its derivatives are
+    d/dmu = (dP_G/dmu) : dF A, d/dlam likewise, and for a_k with
+    E_k = dA/da_k:  H_G[F E_k] : (dF A) + P_G : (dF E_k).
+    """
+    A = activation_matrix(a)
+    G = F @ A
+    dG = dF @ A
+    dmu, dlam = snh_dpk1_dparams(G)
+    P = snh_pk1(G, mu, lam)
+    dact = np.zeros((F.shape[0], 6))
+    for k, (i, j) in enumerate(ACTIVATION_PAIRS):
+        E = np.zeros((3, 3))
+        E[i, j] = E[j, i] = 1.0
+        dact[:, k] = snh_hess_bilinear(G, F @ E, dG, mu, lam) + ddot(P, dF @ E)
+    return ddot(dmu, dG), ddot(dlam, dG), dact
+
+
+# --------------------------------------------------------------------------
 # ARAP: warp/fem/_arap.py:17-76, warp/math/_rotation.py, func/_misc.py:31-70
 # --------------------------------------------------------------------------
 
@@ -324,6 +405,7 @@ def arap_dpk1_dmu(F):
 
 SNH = "snh"
 ARAP = "arap"
+MUSCLE = "snh_muscle"
 
 
 class FemPotential:
@@ -335,13 +417,16 @@ class FemPotential:
     the quadratic form per tet (:379-380), scatter-add to vertices.
     """
 
-    def __init__(self, material, cells, dhdX, dV, mu, lam=None):
+    def __init__(self, material, cells, dhdX, dV, mu, lam=None, activation=None):
         self.material = material
         self.cells = np.asarray(cells)
         self.dhdX = np.asarray(dhdX, np.float64)
         self.dV = np.asarray(dV, np.float64)
         self.mu = np.asarray(mu, np.float64)
         self.lam = None if lam is None else np.asarray(lam, np.float64)
+        self.activation = None if activation is None else np.asarray(activation, np.float64).reshape(-1, 6)
+        if material == MUSCLE and (self.lam is None or self.activation is None):
+            raise ValueError("the muscle law needs lam and activation")
 
     @property
     def n_cells(self):
@@ -354,12 +439,16 @@ class FemPotential:
         F = self.F(u)
         if self.material == SNH:
             return snh_energy_density(F, self.mu, self.lam)
+        if self.material == MUSCLE:
+            return muscle_energy_density(F, self.mu, self.lam, self.activation)
         return arap_energy_density(F, self.mu)
 
     def pk1(self, u):
         F = self.F(u)
         if self.material == SNH:
             return snh_pk1(F, self.mu, self.lam)
+        if self.material == MUSCLE:
+            return muscle_pk1(F, self.mu, self.lam, self.activation)
         return arap_pk1(F, self.mu)
 
     def _scatter(self, n_points, cell_vals):
@@ -382,6 +471,8 @@ class FemPotential:
         F = self.F(u)
         if self.material == SNH:
             h = snh_hess_diag(F, self.dhdX, self.mu, self.lam)
+        elif self.material == MUSCLE:
+            h = muscle_hess_diag(F, self.dhdX, self.mu, self.lam, self.activation)
         else:
             h = arap_hess_diag(F, self.dhdX, self.mu)
         return np.maximum(h * self.dV[:, None, None], 0.0)
@@ -395,6 +486,8 @@ class FemPotential:
         pc = cell_values(p, self.cells)
         if self.material == SNH:
             h = snh_hess_prod(F, pc, self.dhdX, self.mu, self.lam)
+        elif self.material == MUSCLE:
+            h = muscle_hess_prod(F, pc, self.dhdX, self.mu, self.lam, self.activation)
         else:
             h = arap_hess_prod(F, pc, self.dhdX, self.mu)
         return h * self.dV[:, None, None]
@@ -408,6 +501,8 @@ class FemPotential:
         pc = cell_values(p, self.cells)
         if self.material == SNH:
             q = snh_hess_quad(F, pc, self.dhdX, self.mu, self.lam)
+        elif self.material == MUSCLE:
+            q = muscle_hess_quad(F, pc, self.dhdX, self.mu, self.lam, self.activation)
         else:
             q = arap_hess_quad(F, pc, self.dhdX, self.mu)
         return np.maximum(q * self.dV, 0.0)
@@ -427,7 +522,19 @@ class FemPotential:
         if self.material == SNH:
             dmu, dlam = snh_dpk1_dparams(F)
             return self.dV * ddot(dmu, dF), self.dV * ddot(dlam, dF)
+        if self.material == MUSCLE:
+            dmu, dlam, _ = muscle_mixed(F, dF, self.mu, self.lam, self.activation)
+            return self.dV * dmu, self.dV * dlam
         return self.dV * ddot(arap_dpk1_dmu(F), dF), np.zeros(self.n_cells)
+
+    def mixed_activation_cells(self, u, p):
+        """Per-tet p^T d(grad)/d(a_k), (T, 6): the muscle activation adjoint's
+        mixed term (exp/2026/01/28/smas/src/31-inverse-activation-stable-neo-
+        hookean.py:472-486)."""
+        F = self.F(u)
+        dF = jvp(self.dhdX, cell_values(p, self.cells))
+        _, _, dact = muscle_mixed(F, dF, self.mu, self.lam, self.activation)
+        return self.dV[:, None] * dact
 
     def element_hessians(self, u):
         """Exact 12x12 element Hessians (T,12,12) from hess_prod on unit vectors."""
@@ -439,6 +546,8 @@ class FemPotential:
             e[:, col // 3, col % 3] = 1.0
             if self.material == SNH:
                 h = snh_hess_prod(F, e, self.dhdX, self.mu, self.lam)
+            elif self.material == MUSCLE:
+                h = muscle_hess_prod(F, e, self.dhdX, self.mu, self.lam, self.activation)
             else:
                 h = arap_hess_prod(F, e, self.dhdX, self.mu)
             H[:, :, col] = (h * self.dV[:, None, None]).reshape(T, 12)

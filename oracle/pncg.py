@@ -246,3 +246,14 @@ def adjoint_grad(problem: Problem, fem, x_eq, dLdx_free):
     u = problem.dof.to_full(x_eq)
     pf = problem.dof.to_full_grad(p)
     return fem.mixed_cells(u, pf)
+
+
+def adjoint_grad_activation(problem: Problem, fem, x_eq, dLdx_free):
+    """dL/dmu, dL/dlam and dL/da (T, 6) of the muscle law by the same adjoint
+    (exp/2026/01/28/smas/src/31-inverse-activation-stable-neo-hookean.py:472-486)."""
+    H = problem.hessian(x_eq)
+    p = spla.spsolve(H.tocsc(), -np.asarray(dLdx_free))
+    u = problem.dof.to_full(x_eq)
+    pf = problem.dof.to_full_grad(p)
+    dmu, dlam = fem.mixed_cells(u, pf)
+    return dmu, dlam, fem.mixed_activation_cells(u, pf)

@@ -33,12 +33,17 @@ def _rot(rng, T):
     )
 
 
+def _activation(rng, T, scale=0.15):
+    return scale * rng.standard_normal((T, 6))
+
+
 def _fem(material, mesh, rng=RNG):
     dhdX, dV = of.region(mesh.points, mesh.cells)
     T = mesh.n_cells
     mu = 1e4 * np.exp(0.3 * rng.standard_normal(T))
-    lam = 10.0 * mu if material == of.SNH else None
-    return of.FemPotential(material, mesh.cells, dhdX, dV, mu, lam)
+    lam = 10.0 * mu if material != of.ARAP else None
+    act = _activation(rng, T) if material == of.MUSCLE else None
+    return of.FemPotential(material, mesh.cells, dhdX, dV, mu, lam, act)
 
 
 # ----------------------------------------------------------------- setup
@@ -165,7 +170,7 @@ def _stretch_mesh_state(rng, m, A_diag=(1.3, 1.2, 1.25)):
     return m.points @ (A - np.eye(3)).T + 0.003 * rng.standard_normal(m.points.shape)
 
 
-@pytest.mark.parametrize("material", [of.SNH, of.ARAP])
+@pytest.mark.parametrize("material", [of.SNH, of.ARAP, of.MUSCLE])
 def test_element_hessian_matches_fd(material):
     rng = np.random.default_rng(7)
     m = cube(2)
@@ -182,7 +187,7 @@ def test_element_hessian_matches_fd(material):
         np.testing.assert_allclose(He[t], _fd_element_hessian(pot, u, t), rtol=1e-5, atol=1e-7)
 
 
-@pytest.mark.parametrize("material", [of.SNH, of.ARAP])
+@pytest.mark.parametrize("material", [of.SNH, of.ARAP, of.MUSCLE])
 def test_global_grad_and_hess_prod_fd(material):
     rng = np.random.default_rng(11)
     m = cube(2)
@@ -198,12 +203,13 @@ def test_global_grad_and_hess_prod_fd(material):
     np.testing.assert_allclose(pot.hess_prod(u, v), hv_fd, rtol=1e-5, atol=1e-6 * np.abs(hv_fd).max())
 
 
-def test_snh_hess_diag_is_exact_diagonal():
+@pytest.mark.parametrize("material", [of.SNH, of.MUSCLE])
+def test_snh_hess_diag_is_exact_diagonal(material):
     """SNH: every diag term is >= 0 and h6_diag == 0 is exact, so the clamped
     reference diagonal equals the diagonal of the exact assembled Hessian."""
     rng = np.random.default_rng(5)
     m = cube(2)
-    pot = _fem(of.SNH, m, rng)
+    pot = _fem(material, m, rng)
     u = 0.1 * rng.standard_normal(m.points.shape)
     hd = pot.hess_diag(u)
     He = pot.element_hessians(u)
@@ -227,7 +233,7 @@ def test_arap_hess_diag_matches_fd_when_unclamped():
         np.testing.assert_allclose(hd_cells[t], np.maximum(fd, 0.0), rtol=1e-5, atol=1e-8)
 
 
-@pytest.mark.parametrize("material", [of.SNH, of.ARAP])
+@pytest.mark.parametrize("material", [of.SNH, of.ARAP, of.MUSCLE])
 def test_hess_quad_clamp_and_consistency(material):
     rng = np.random.default_rng(3)
     m = cube(3)
@@ -278,7 +284,7 @@ def test_reference_arap_hess_prod_as_written_is_not_hv():
     assert np.abs(good - bad).max() > 1e-3 * np.abs(good).max()
 
 
-@pytest.mark.parametrize("material", [of.SNH, of.ARAP])
+@pytest.mark.parametrize("material", [of.SNH, of.ARAP, of.MUSCLE])
 def test_mixed_derivative_fd(material):
     """p^T dg/dmu (and dg/dlam) against FD of the gradient in the parameter."""
     rng = np.random.default_rng(17)
@@ -287,7 +293,7 @@ def test_mixed_derivative_fd(material):
     u = 0.05 * rng.standard_normal(m.points.shape)
     p = rng.standard_normal(u.shape)
     dmu, dlam = pot.mixed_cells(u, p)
-    h = 1e-3
+    h = 1.0  # the gradient is linear in mu and lambda: central differences are exact up to rounding
     for t in (0, 7, 19):
         mu0 = pot.mu[t]
         pot.mu[t] = mu0 + h
@@ -296,7 +302,7 @@ def test_mixed_derivative_fd(material):
         gm = pot.grad(u)
         pot.mu[t] = mu0
         assert dmu[t] == pytest.approx(np.sum((gp - gm) / (2 * h) * p), rel=1e-6)
-        if material == of.SNH:
+        if material != of.ARAP:
             l0 = pot.lam[t]
             pot.lam[t] = l0 + h
             gp = pot.grad(u)
@@ -304,6 +310,91 @@ def test_mixed_derivative_fd(material):
             gm = pot.grad(u)
             pot.lam[t] = l0
             assert dlam[t] == pytest.approx(np.sum((gp - gm) / (2 * h) * p), rel=1e-6, abs=1e-12)
+
+
+# ----------------------------------------------------------------- SNH muscle
+
+
+def test_muscle_zero_activation_is_snh():
+    rng = np.random.default_rng(31)
+    m = cube(2)
+    snh = _fem(of.SNH, m, rng)
+    mus = of.FemPotential(of.MUSCLE, m.cells, snh.dhdX, snh.dV, snh.mu, snh.lam, np.zeros((m.n_cells, 6)))
+    u = 0.1 * rng.standard_normal(m.points.shape)
+    v = rng.standard_normal(u.shape)
+    assert mus.fun(u) == pytest.approx(snh.fun(u), rel=1e-14)
+    for name in ("grad", "hess_diag"):
+        np.testing.assert_allclose(getattr(mus, name)(u), getattr(snh, name)(u), rtol=1e-13, atol=1e-9)
+    np.testing.assert_allclose(mus.hess_prod(u, v), snh.hess_prod(u, v), rtol=1e-13, atol=1e-9)
+    assert mus.hess_quad(u, v) == pytest.approx(snh.hess_quad(u, v), rel=1e-13)
+
+
+def test_muscle_uniform_scale_closed_form():
+    """u = 0, A = s I: G = s I, Psi = mu/2 (3 s^2 - 3) - mu (s^3 - 1) + lam/2 (s^3 - 1)^2,
+    P_F = (mu s + (lam (s^3 - 1) - mu) s^2) s I."""
+    T = 3
+    s = np.array([0.8, 1.0, 1.3])
+    mu = np.array([2.0, 3.0, 5.0])
+    lam = np.array([7.0, 11.0, 13.0])
+    a = np.zeros((T, 6))
+    a[:, :3] = (s - 1.0)[:, None]
+    F = np.broadcast_to(np.eye(3), (T, 3, 3)).copy()
+    J = s**3
+    psi = 0.5 * mu * (3 * s**2 - 3) - mu * (J - 1) + 0.5 * lam * (J - 1) ** 2
+    np.testing.assert_allclose(of.muscle_energy_density(F, mu, lam, a), psi, rtol=1e-14)
+    P = of.muscle_pk1(F, mu, lam, a)
+    c = (mu * s + (lam * (J - 1) - mu) * s**2) * s
+    np.testing.assert_allclose(P, c[:, None, None] * np.eye(3), rtol=1e-14, atol=1e-14)
+
+
+def test_muscle_equals_snh_on_activated_rest_shape():
+    """G = F A = dx/dY for the rest shape Y = A^-1 X: tet by tet, the muscle law
+    at displacement u is SNH on rest corners A^-1 X at displacement X + u - A^-1 X
+    (same dV).  Pins where A enters (F A, dhdX A, the A^T of the stress)."""
+    rng = np.random.default_rng(41)
+    m = cube(2)
+    T = m.n_cells
+    # one tet per element: independent corners, so each tet may have its own A
+    X = m.points[m.cells].reshape(-1, 3)
+    cells = np.arange(4 * T).reshape(T, 4)
+    dhdX, dV = of.region(X, cells)
+    act = _activation(rng, T, 0.2)
+    A = of.activation_matrix(act)
+    Y = np.einsum("tij,taj->tai", np.linalg.inv(A), X.reshape(T, 4, 3)).reshape(-1, 3)
+    dhdY, _ = of.region(Y, cells)
+    mu = 1e4 * np.exp(0.3 * rng.standard_normal(T))
+    lam = 10.0 * mu
+    mus = of.FemPotential(of.MUSCLE, cells, dhdX, dV, mu, lam, act)
+    snh = of.FemPotential(of.SNH, cells, dhdY, dV, mu, lam)
+    u = 0.1 * rng.standard_normal(X.shape)
+    uY = X + u - Y
+    v = rng.standard_normal(X.shape)
+    np.testing.assert_allclose(mus.energy_density(u), snh.energy_density(uY), rtol=1e-10, atol=1e-10)
+    np.testing.assert_allclose(mus.grad(u), snh.grad(uY), rtol=1e-10, atol=1e-8)
+    np.testing.assert_allclose(mus.hess_diag(u), snh.hess_diag(uY), rtol=1e-10, atol=1e-8)
+    np.testing.assert_allclose(mus.hess_prod(u, v), snh.hess_prod(uY, v), rtol=1e-10, atol=1e-8)
+    np.testing.assert_allclose(mus.hess_quad_cells(u, v), snh.hess_quad_cells(uY, v), rtol=1e-10, atol=1e-8)
+
+
+def test_muscle_mixed_activation_fd():
+    """p^T dg/da_k against central differences of the gradient in a_k."""
+    rng = np.random.default_rng(43)
+    m = cube(2)
+    pot = _fem(of.MUSCLE, m, rng)
+    u = 0.05 * rng.standard_normal(m.points.shape)
+    p = rng.standard_normal(u.shape)
+    dact = pot.mixed_activation_cells(u, p)
+    h = 1e-6
+    for t in (0, 9, 23):
+        for k in range(6):
+            a0 = pot.activation[t, k]
+            pot.activation[t, k] = a0 + h
+            gp = pot.grad(u)
+            pot.activation[t, k] = a0 - h
+            gm = pot.grad(u)
+            pot.activation[t, k] = a0
+            fd = np.sum((gp - gm) / (2 * h) * p)
+            assert dact[t, k] == pytest.approx(fd, rel=1e-6, abs=1e-9 * np.abs(dact).max())
 
 
 def test_external_force_sign_convention():
