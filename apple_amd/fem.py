@@ -1,6 +1,6 @@
 """FEM energy potentials with liblaf/apple's interface, on the HIP kernels.
 
-``StableNeoHookean`` and ``Arap`` keep the methods of
+``StableNeoHookean``, ``Arap`` and ``StableNeoHookeanMuscle`` keep the methods of
 ``WarpPotentialFem`` (warp/fem/_base.py:39-194) and the protocol of
 ``WarpPotential`` (warp/model/_potential.py:9-32):
 

@@ -37,8 +37,9 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_cube_problem(n, material, dtype, device, seed_mu=1, seed_noise=0, noise_frac=0.1, chunks=1):
-    """Config-2 style cube: lognormal mu, lambda = 10 mu, gravity, z = 0 pinned, noisy start."""
+def make_cube_problem(n, material, dtype, device, seed_mu=1, seed_noise=0, noise_frac=0.1, chunks=1, seed_act=8):
+    """Config-2 style cube: lognormal mu, lambda = 10 mu, gravity, z = 0 pinned, noisy start
+    (muscle: per-tet activation, normal with standard deviation 0.1 per component)."""
     import torch
 
     from apple_amd import fem
@@ -50,7 +51,9 @@ def make_cube_problem(n, material, dtype, device, seed_mu=1, seed_noise=0, noise
     m.cell_data["mu"] = mu
     m.cell_data["lambda"] = 10.0 * mu
     region = Region.from_mesh(m)
-    cls = fem.StableNeoHookean if material == "snh" else fem.Arap
+    if material == "muscle":
+        region.cell_data["activation"] = 0.1 * np.random.default_rng(seed_act).standard_normal((m.n_cells, 6))
+    cls = {"snh": fem.StableNeoHookean, "arap": fem.Arap, "muscle": fem.StableNeoHookeanMuscle}[material]
     pot = cls.from_region(region, dtype=dtype, device=device, chunks=chunks)
     f = np.zeros((m.n_points, 3))
     np.add.at(f[:, 2], m.cells.reshape(-1), np.repeat(-9.8 * 1e3 * region.dV / 4.0, 4))
@@ -76,6 +79,7 @@ def bench_pass(pot, u, what, iters, kernel=True, full=True):
     b = torch.empty_like(a)
     e = torch.zeros(1, dtype=torch.float64, device=m.device)
     lam = pot.materials.get("lambda")
+    pot._prepare()  # binds per-mesh material arrays (the muscle activation)
     p = u if what != _lib.APL_ENERGY | _lib.APL_GRAD | _lib.APL_HESS_DIAG else None
     k = ctypes.c_double()
     tp = ctypes.c_double()
@@ -334,10 +338,22 @@ def main():
             "hbm_frac": round((60 * T + 36 * V) / (ka / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
         }
 
-    # ---------------------------------------------------------------- PNCG solves (config 2: SNH and ARAP)
+    mprob = None
+    if not args.quick:
+        log("[bench] SNH muscle pass A")
+        mprob = make_cube_problem(n, "muscle", dtype, dev)
+        km, pm = bench_pass(mprob["pot"], mprob["x0"], what_a, 20)
+        out["muscle_pass_a"] = {
+            "mtet_evals_per_s": round(T / (pm / 1e3) / 1e6, 1),
+            "kernel_ms": round(km, 5),
+            "hbm_frac": round((88 * T + 36 * V) / (km / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+        }
+
+    # ---------------------------------------------------------------- PNCG solves (config 2: SNH, ARAP, muscle)
     if not (args.quick or args.no_solve):
         out["pncg_snh"] = bench_solve(prob, "snh", dev)
         out["pncg_arap"] = bench_solve(aprob, "arap", dev)
+        out["pncg_muscle"] = bench_solve(mprob, "muscle", dev)
     del aprob
 
     # ---------------------------------------------------------------- batched (+adjoint) vs plain
