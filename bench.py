@@ -186,6 +186,8 @@ def main():
     ap.add_argument("--fd-tets", type=int, default=16,
                     help="central-difference tets of the differentiation check in the full run (--diff-only: 64)")
     ap.add_argument("--no-combined", action="store_true")
+    ap.add_argument("--no-inverse", action="store_true")
+    ap.add_argument("--inverse-only", action="store_true", help="only the muscle-activation inverse loop")
     ap.add_argument("--batched-meshes", type=int, default=256)
     ap.add_argument("--cpu-threads", type=int, default=16, help="threads of the multi-threaded CPU baseline")
     ap.add_argument("--profile-pass", action="store_true",
@@ -213,6 +215,9 @@ def main():
         return
     if args.combined_only:
         print(json.dumps(bench_combined(dev)), flush=True)
+        return
+    if args.inverse_only:
+        print(json.dumps(bench_inverse(dev)), flush=True)
         return
     log(f"[bench] building {n}^3 x 5 cube (SNH, FP32)")
     prob = make_cube_problem(n, "snh", dtype, dev, chunks=args.chunks)
@@ -365,6 +370,8 @@ def main():
         out["differentiation"] = bench_differentiation(dev, n_fd=args.fd_tets)
     if not (args.quick or args.no_combined):
         out["combined"] = bench_combined(dev)
+    if not (args.quick or args.no_inverse):
+        out["inverse"] = bench_inverse(dev)
 
     # ---------------------------------------------------------------- CPU baseline
     if not (args.quick or args.no_cpu):
@@ -662,6 +669,64 @@ def bench_combined(dev, n_meshes=64, lo=16, hi=48, seed=7, adj_rtol32=1e-6, adj_
     log(f"[bench] combined: energy rel err max {e_err.max():.2e}, dL/dq rel err max {g_err.max():.2e}, "
         f"positions max {x_err.max():.2e} of the bbox diagonal")
     return res
+
+
+def bench_inverse(dev, n=32, steps=12, prestrain=0.9, lr=0.05, seed=9):
+    """Muscle-activation inverse loop (FP32): per-tet log activation, surface
+    point-to-point loss against a forward solution for a smooth true field,
+    Adam; every step = warm-started PNCG + adjoint PCG + mixed pass."""
+    import torch
+
+    from apple_amd import fem, inverse
+    from apple_amd.mesh import Region, cube
+    from apple_amd.solver import DeviceSolver, PncgConfig, mask_bits
+
+    m = cube(n)
+    rng = np.random.default_rng(seed)
+    mu = 1e4 * np.exp(0.3 * rng.standard_normal(m.n_cells))
+    m.cell_data["mu"] = mu
+    m.cell_data["lambda"] = 10 * mu
+    m.cell_data["activation"] = np.zeros((m.n_cells, 6))
+    region = Region.from_mesh(m)
+    pot = fem.StableNeoHookeanMuscle.from_region(region, dtype=torch.float32, device=dev)
+    fixed = np.zeros((m.n_points, 3), bool)
+    fixed[m.points[:, 0] == 0] = True
+    mask = mask_bits(fixed, dev)
+    surface = np.flatnonzero((m.points == 0).any(1) | (m.points == 1).any(1))
+    centroid = m.points[m.cells].mean(1)
+    T = m.n_cells
+    true = torch.as_tensor(0.2 + 0.1 * np.sin(2 * np.pi * centroid[:, 1]), dtype=torch.float32, device=dev)
+    mk = lambda p: {"activation": inverse.muscle_x_activation(p, prestrain=prestrain).contiguous()}
+    x_t = torch.zeros(m.n_points, 3, dtype=torch.float32, device=dev)
+    DeviceSolver(pot).solve(x_t, None, mask, PncgConfig(rtol=1e-6, max_steps=50000), activation=mk(true)["activation"])
+    idx = torch.as_tensor(surface, device=dev)
+    inv = inverse.Inverse(pot, [inverse.PointToPointLoss(idx, x_t[idx])], mk, mask=mask, lr=lr,
+                          forward=PncgConfig(rtol=1e-6, max_steps=50000))
+    log(f"[bench] inverse: {T} tets, {len(surface)} surface targets, {steps} Adam steps")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    p = inv.solve(torch.zeros(T, dtype=torch.float32, device=dev), steps=steps)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    h = inv.history
+    err0 = float(true.abs().mean())
+    err = float((p - true).abs().mean())
+    log(f"[bench] inverse: {dt / steps * 1e3:.1f} ms per step, loss {h[0].loss:.3e} -> {h[-1].loss:.3e}, "
+        f"mean |p - p_true| {err0:.3f} -> {err:.3f}")
+    return {
+        "workload": f"muscle activation inverse, {n}^3 cube x 5 tets ({T} per-tet parameters), FP32, "
+        f"surface point-to-point loss, Adam lr {lr}",
+        "steps": steps,
+        "ms_per_step": round(dt / steps * 1e3, 2),
+        "first_step_ms": round(h[0].seconds * 1e3, 2),
+        "forward_steps_first": h[0].forward_steps,
+        "forward_steps_mean_warm": round(float(np.mean([r.forward_steps for r in h[1:]])), 1),
+        "adjoint_iters_mean": round(float(np.mean([r.adjoint_iters for r in h])), 1),
+        "loss_first": h[0].loss,
+        "loss_last": h[-1].loss,
+        "param_mean_abs_err_start": round(err0, 4),
+        "param_mean_abs_err_end": round(err, 4),
+    }
 
 
 if __name__ == "__main__":
