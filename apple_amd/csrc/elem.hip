@@ -15,8 +15,8 @@
 //    positions), addressed by 32-bit offsets from per-block uniform base
 //    pointers: every load is coalesced;
 //  * each corner's contribution goes to LDS at its position (corners ranked
-//    by slot, then tet, then corner), so the corners of a local vertex form a
-//    contiguous run; one to four lanes sum each run in a fixed order
+//    by slot, then tet, then corner; channels packed into 16/8-byte records),
+//    so the corners of a local vertex form a contiguous run; one to four lanes sum each run in a fixed order
 //    (bitwise reproducible) and one partial per slot goes into a
 //    vertex-major buffer (a vertex's partials contiguous, in block order);
 //    the vertex pass adds each vertex's contiguous run;
@@ -194,6 +194,23 @@ template <class T>
 struct alignas(4 * sizeof(T)) Vec4 {
   T v[4];
 };
+template <class T>
+struct alignas(2 * sizeof(T)) Vec2 {
+  T v[2];
+};
+
+// Channel packing of the contribution buffer: one region per group of 4
+// channels ([position][4]: one 16/32-byte LDS access per corner and per
+// position), then a region for the remainder (a pair, a single channel, or
+// 3 channels padded to 4).  Pass A's 6 channels take a 16-byte and an 8-byte
+// access instead of six 4-byte ones.
+template <int NCH>
+struct Pack {
+  static constexpr int G4 = NCH / 4;
+  static constexpr int REM = NCH % 4;
+  static constexpr int REMW = REM == 3 ? 4 : REM;  // stored width of the remainder
+  static constexpr int NCHP = 4 * G4 + REMW;       // stored channels per position
+};
 
 // SPT: slot items per thread.  A block with nloc slots splits each slot's
 // position range over tps = 4, 2 or 1 adjacent lanes (the most that fit
@@ -216,7 +233,8 @@ __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R, X> A) {
   double* red = smem_d;
   X* u_lds = reinterpret_cast<X*>(smem_d + 8);  // 4 per slot (padded)
   R* v_lds = reinterpret_cast<R*>(u_lds + 4 * A.nloc_max);
-  R* contrib = v_lds + (NEEDV ? 4 * A.nloc_max : 0);  // [channel][position]
+  R* contrib = v_lds + (NEEDV ? 4 * A.nloc_max : 0);  // [channel group][position][channel in group]
+  using PK = Pack<NCHA>;
 
   const int tid = threadIdx.x;
   const int pid = blockIdx.x;
@@ -239,7 +257,7 @@ __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R, X> A) {
   // only those of still-active instances.
   const int n_cand = k0 < nb_x ? (nb_x - k0 + K - 1) / K : 0;
   // the active-block list (solves only) sits after the contribution buffer
-  int* act_list = reinterpret_cast<int*>(contrib + (SCATTER ? NCH * CS : 0));
+  int* act_list = reinterpret_cast<int*>(contrib + (SCATTER ? PK::NCHP * CS : 0));
   for_active<NTH>(
       n_cand, A.active != nullptr, act_list, [&](int j) { return xcd_block(k0 + j * K); },
       [&](int bb) { return A.active[A.blk_inst[bb]] != 0; }, [&](const int b) {
@@ -361,24 +379,40 @@ __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R, X> A) {
                                 (int)(rec.cr.w >> 16)};
 #pragma unroll
             for (int a = 0; a < 4; ++a) {
-              R* cp = contrib + pos[a];
+              R vals[NCHA];
               int ch = 0;
               if constexpr (HAS_G) {
-                cp[(ch + 0) * CS] = o.g[a].x;
-                cp[(ch + 1) * CS] = o.g[a].y;
-                cp[(ch + 2) * CS] = o.g[a].z;
+                vals[ch + 0] = o.g[a].x;
+                vals[ch + 1] = o.g[a].y;
+                vals[ch + 2] = o.g[a].z;
                 ch += 3;
               }
               if constexpr (HAS_H) {
-                cp[(ch + 0) * CS] = o.h[a].x;
-                cp[(ch + 1) * CS] = o.h[a].y;
-                cp[(ch + 2) * CS] = o.h[a].z;
+                vals[ch + 0] = o.h[a].x;
+                vals[ch + 1] = o.h[a].y;
+                vals[ch + 2] = o.h[a].z;
                 ch += 3;
               }
               if constexpr (HAS_HP) {
-                cp[(ch + 0) * CS] = o.hp[a].x;
-                cp[(ch + 1) * CS] = o.hp[a].y;
-                cp[(ch + 2) * CS] = o.hp[a].z;
+                vals[ch + 0] = o.hp[a].x;
+                vals[ch + 1] = o.hp[a].y;
+                vals[ch + 2] = o.hp[a].z;
+              }
+#pragma unroll
+              for (int gi = 0; gi < PK::G4; ++gi) {
+                const Vec4<R> t = {{vals[4 * gi], vals[4 * gi + 1], vals[4 * gi + 2], vals[4 * gi + 3]}};
+                *reinterpret_cast<Vec4<R>*>(contrib + gi * 4 * CS + 4 * pos[a]) = t;
+              }
+              R* rem = contrib + PK::G4 * 4 * CS;
+              constexpr int c0 = 4 * PK::G4;
+              if constexpr (PK::REM == 3) {
+                const Vec4<R> t = {{vals[c0], vals[c0 + 1], vals[c0 + 2], R(0)}};
+                *reinterpret_cast<Vec4<R>*>(rem + 4 * pos[a]) = t;
+              } else if constexpr (PK::REM == 2) {
+                const Vec2<R> t = {{vals[c0], vals[c0 + 1]}};
+                *reinterpret_cast<Vec2<R>*>(rem + 2 * pos[a]) = t;
+              } else if constexpr (PK::REM == 1) {
+                rem[pos[a]] = vals[c0];
               }
             }
           }
@@ -387,9 +421,42 @@ __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R, X> A) {
           __syncthreads();
 #pragma unroll
           for (int k = 0; k < SPT; ++k) {
-            for (int q = q0[k]; q < q1[k]; q += tps) {
+            // one position's channels: a 4-channel group per 16/32-byte load, then the remainder
+            auto load_pos = [&](int q, R (&t)[PK::NCHP]) {
 #pragma unroll
-              for (int ch = 0; ch < NCH; ++ch) acc[k][ch] += contrib[ch * CS + q];
+              for (int gi = 0; gi < PK::G4; ++gi) {
+                const Vec4<R> w = *reinterpret_cast<const Vec4<R>*>(contrib + gi * 4 * CS + 4 * q);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) t[4 * gi + i] = w.v[i];
+              }
+              const R* rem = contrib + PK::G4 * 4 * CS;
+              constexpr int c0 = 4 * PK::G4;
+              if constexpr (PK::REM == 3) {
+                const Vec4<R> w = *reinterpret_cast<const Vec4<R>*>(rem + 4 * q);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) t[c0 + i] = w.v[i];
+              } else if constexpr (PK::REM == 2) {
+                const Vec2<R> w = *reinterpret_cast<const Vec2<R>*>(rem + 2 * q);
+                t[c0] = w.v[0];
+                t[c0 + 1] = w.v[1];
+              } else if constexpr (PK::REM == 1) {
+                t[c0] = rem[q];
+              }
+            };
+            int q = q0[k];
+            // two positions per step, both loads first; the sums keep the sequential order
+            for (; q + tps < q1[k]; q += 2 * tps) {
+              R ta[PK::NCHP], tb[PK::NCHP];
+              load_pos(q, ta);
+              load_pos(q + tps, tb);
+#pragma unroll
+              for (int ch = 0; ch < NCH; ++ch) acc[k][ch] = (acc[k][ch] + ta[ch]) + tb[ch];
+            }
+            if (q < q1[k]) {
+              R ta[PK::NCHP];
+              load_pos(q, ta);
+#pragma unroll
+              for (int ch = 0; ch < NCH; ++ch) acc[k][ch] += ta[ch];
             }
           }
           __syncthreads();
@@ -557,7 +624,7 @@ static size_t smem_bytes(const apl_mesh* m, bool filtered) {
   constexpr bool NEEDV = (WHAT & (W_HP | W_HQ | W_MIX | W_MIXA)) != 0;
   constexpr int NCH = ((WHAT & W_G) ? 3 : 0) + ((WHAT & W_H) ? 3 : 0) + ((WHAT & W_HP) ? 3 : 0);
   size_t n = 8 * sizeof(double) + 4 * (size_t)m->L.nloc_max * (sizeof(X) + (NEEDV ? sizeof(R) : 0));
-  n += sizeof(R) * (size_t)NCH * 4 * NTH;
+  n += sizeof(R) * (size_t)(NCH > 0 ? Pack<NCH>::NCHP : 0) * 4 * NTH;
   if (filtered) n += sizeof(int) * NTH;  // active-block list
   return n;
 }
