@@ -89,6 +89,11 @@ def _declare(lib):
         ctypes.c_int, c_i64, c_i64, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, ctypes.c_int, ctypes.c_int,
         ctypes.POINTER(c_vp),
     ]
+    lib.apl_mesh_create_from_points.argtypes = [
+        ctypes.c_int, c_i64, c_i64, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, ctypes.c_int, ctypes.c_int,
+        ctypes.POINTER(c_vp),
+    ]
+    lib.apl_mesh_region.argtypes = [c_vp, c_vp, c_vp, c_vp]
     lib.apl_mesh_destroy.argtypes = [c_vp]
     lib.apl_mesh_info.argtypes = [c_vp, ctypes.POINTER(MeshInfo)]
     lib.apl_mesh_set_schedule.argtypes = [c_vp, ctypes.c_int]

@@ -71,9 +71,15 @@ const char* apl_last_error(void) { return g_err.c_str(); }
 
 int apl_version(void) { return 10000; }
 
-int apl_mesh_create(int dtype, int64_t n_tets, int64_t n_verts, const int32_t* cells, const double* dminv,
-                    const double* dv, int64_t n_inst, const int64_t* inst_tet_off, const int64_t* inst_vtx_off,
-                    int chunks, int device, apl_mesh** out) {
+}  // extern "C"
+
+namespace apl {
+
+// Everything of a mesh except the per-tet element data (inverse rest shapes
+// and volumes): validation, block partition, layout upload, workspaces.
+static int create_common(int dtype, int64_t n_tets, int64_t n_verts, const int32_t* cells, int64_t n_inst,
+                         const int64_t* inst_tet_off, const int64_t* inst_vtx_off, int chunks, int device,
+                         apl_mesh** out) {
   if (out == nullptr) {
     set_error("out is NULL");
     return -1;
@@ -83,8 +89,8 @@ int apl_mesh_create(int dtype, int64_t n_tets, int64_t n_verts, const int32_t* c
     set_error("dtype must be APL_F32 or APL_F64");
     return -1;
   }
-  if (n_tets > 0 && (cells == nullptr || dminv == nullptr || dv == nullptr)) {
-    set_error("cells, dminv and dv must be non-NULL");
+  if (n_tets > 0 && cells == nullptr) {
+    set_error("cells must be non-NULL");
     return -1;
   }
   for (int64_t i = 0; i < 4 * n_tets; ++i)
@@ -124,29 +130,8 @@ int apl_mesh_create(int dtype, int64_t n_tets, int64_t n_verts, const int32_t* c
   DevLayout& d = m->d;
   const size_t rs = dtype == APL_F64 ? 8 : 4;
   int rc = 0;
-  // element data, SoA, in the mesh precision
-  {
-    std::vector<char> buf(9 * (size_t)n_tets * rs);
-    for (int64_t t = 0; t < n_tets; ++t)
-      for (int k = 0; k < 9; ++k) {
-        double v = dminv[t * 9 + k];
-        if (rs == 8)
-          reinterpret_cast<double*>(buf.data())[k * n_tets + t] = v;
-        else
-          reinterpret_cast<float*>(buf.data())[k * n_tets + t] = (float)v;
-      }
-    rc |= alloc_bytes(m, &d.dminv, buf.size());
-    if (!rc && !buf.empty() && hipMemcpy(d.dminv, buf.data(), buf.size(), hipMemcpyHostToDevice) != hipSuccess) rc = -1;
-    std::vector<char> bv((size_t)n_tets * rs);
-    for (int64_t t = 0; t < n_tets; ++t) {
-      if (rs == 8)
-        reinterpret_cast<double*>(bv.data())[t] = dv[t];
-      else
-        reinterpret_cast<float*>(bv.data())[t] = (float)dv[t];
-    }
-    rc |= alloc_bytes(m, &d.dV, bv.size());
-    if (!rc && !bv.empty() && hipMemcpy(d.dV, bv.data(), bv.size(), hipMemcpyHostToDevice) != hipSuccess) rc = -1;
-  }
+  rc |= alloc_bytes(m, &d.dminv, 9 * (size_t)n_tets * rs);
+  rc |= alloc_bytes(m, &d.dV, (size_t)n_tets * rs);
   rc |= upload(m, &d.corner, L.corner.data(), L.corner.size());
   rc |= upload(m, &d.blk_tet_off, L.blk_tet_off.data(), L.blk_tet_off.size());
   rc |= upload(m, &d.blk_loc_off, L.blk_loc_off.data(), L.blk_loc_off.size());
@@ -176,6 +161,78 @@ int apl_mesh_create(int dtype, int64_t n_tets, int64_t n_verts, const int32_t* c
   L.loc_dst = std::vector<int32_t>();
   *out = m;
   return 0;
+}
+
+}  // namespace apl
+
+extern "C" {
+
+int apl_mesh_create(int dtype, int64_t n_tets, int64_t n_verts, const int32_t* cells, const double* dminv,
+                    const double* dv, int64_t n_inst, const int64_t* inst_tet_off, const int64_t* inst_vtx_off,
+                    int chunks, int device, apl_mesh** out) {
+  if (n_tets > 0 && (dminv == nullptr || dv == nullptr)) {
+    set_error("cells, dminv and dv must be non-NULL");
+    if (out != nullptr) *out = nullptr;
+    return -1;
+  }
+  apl_mesh* m = nullptr;
+  if (create_common(dtype, n_tets, n_verts, cells, n_inst, inst_tet_off, inst_vtx_off, chunks, device, &m)) return -1;
+  // element data, SoA, in the mesh precision
+  const size_t rs = dtype == APL_F64 ? 8 : 4;
+  std::vector<char> buf(9 * (size_t)n_tets * rs);
+  for (int64_t t = 0; t < n_tets; ++t)
+    for (int k = 0; k < 9; ++k) {
+      double v = dminv[t * 9 + k];
+      if (rs == 8)
+        reinterpret_cast<double*>(buf.data())[k * n_tets + t] = v;
+      else
+        reinterpret_cast<float*>(buf.data())[k * n_tets + t] = (float)v;
+    }
+  std::vector<char> bv((size_t)n_tets * rs);
+  for (int64_t t = 0; t < n_tets; ++t) {
+    if (rs == 8)
+      reinterpret_cast<double*>(bv.data())[t] = dv[t];
+    else
+      reinterpret_cast<float*>(bv.data())[t] = (float)dv[t];
+  }
+  if ((!buf.empty() && hipMemcpy(m->d.dminv, buf.data(), buf.size(), hipMemcpyHostToDevice) != hipSuccess) ||
+      (!bv.empty() && hipMemcpy(m->d.dV, bv.data(), bv.size(), hipMemcpyHostToDevice) != hipSuccess)) {
+    set_error("upload of the element data failed");
+    (void)hipGetLastError();
+    free_mesh(m);
+    return -1;
+  }
+  *out = m;
+  return 0;
+}
+
+int apl_mesh_create_from_points(int dtype, int64_t n_tets, int64_t n_verts, const int32_t* cells,
+                                const double* points, const double* fraction, int64_t n_inst,
+                                const int64_t* inst_tet_off, const int64_t* inst_vtx_off, int chunks, int device,
+                                apl_mesh** out) {
+  if (n_tets > 0 && points == nullptr) {
+    set_error("points must be non-NULL");
+    if (out != nullptr) *out = nullptr;
+    return -1;
+  }
+  apl_mesh* m = nullptr;
+  if (create_common(dtype, n_tets, n_verts, cells, n_inst, inst_tet_off, inst_vtx_off, chunks, device, &m)) return -1;
+  if (n_tets > 0 && region_from_points(m, cells, points, fraction) != 0) {
+    free_mesh(m);
+    return -1;
+  }
+  *out = m;
+  return 0;
+}
+
+int apl_mesh_region(apl_mesh* mesh, void* dminv, void* dv, void* stream) {
+  if (mesh == nullptr || (mesh->L.T > 0 && dminv == nullptr && dv == nullptr)) {
+    set_error("mesh and at least one output must be non-NULL");
+    return -1;
+  }
+  APL_CHECK_HIP(hipSetDevice(mesh->device));
+  (void)hipGetLastError();  // drop errors left by unrelated earlier HIP calls
+  return region_readback(mesh, dminv, dv, (hipStream_t)stream);
 }
 
 int apl_mesh_destroy(apl_mesh* mesh) {

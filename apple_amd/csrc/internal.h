@@ -96,5 +96,10 @@ int launch_elem(apl_mesh* m, int material, unsigned what, const void* mu, const 
 int launch_finish(apl_mesh* m, unsigned what, void* out_a, void* out_b, double* out_e, double* out_q,
                   int accumulate, const int32_t* active, hipStream_t s);
 int elem_grid(apl_mesh* m, int material, unsigned what, int x_f64 = 0, bool filtered = false);
+// device setup (setup.hip): the element data (inverse rest shapes, dV *
+// fraction) of a created mesh from host cells / points / fraction, and the
+// read-back of the element data in the caller's tet order (T x 9, T)
+int region_from_points(apl_mesh* m, const int32_t* cells, const double* points, const double* fraction);
+int region_readback(apl_mesh* m, void* dminv_out, void* dv_out, hipStream_t s);
 
 }  // namespace apl

@@ -66,6 +66,7 @@ class FemPotential:
         self.name = name if name is not None else default_name(self)
         self.requires_grad = tuple(requires_grad)
         self.region = region
+        self.cells_host = region.cells if region is not None else None
         self._energy = torch.zeros(1, dtype=torch.float64, device=mesh.device)
 
     # ------------------------------------------------------------ construction
@@ -91,10 +92,42 @@ class FemPotential:
         return cls(mesh, materials, name=name, requires_grad=requires_grad, region=region)
 
     @classmethod
-    def from_pyvista(cls, obj: Any, **kwargs):
-        return cls.from_region(Region.from_mesh(Mesh.from_pyvista(obj)), **kwargs)
+    def from_pyvista(cls, obj: Any, *, setup: str = "host", **kwargs):
+        """setup="host": the region (dhdX, dV) in numpy, as the reference's
+        Region; setup="device": the element data computed by a kernel from the
+        rest positions (``apl_mesh_create_from_points``); no host region."""
+        mesh = Mesh.from_pyvista(obj)
+        if setup == "host":
+            return cls.from_region(Region.from_mesh(mesh), **kwargs)
+        if setup != "device":
+            raise ValueError("setup must be 'host' or 'device'")
+        return cls.from_points(mesh, **kwargs)
 
     from_mesh = from_pyvista
+
+    @classmethod
+    def from_points(
+        cls,
+        mesh: Mesh,
+        *,
+        requires_grad: Sequence[str] = (),
+        name: str | None = None,
+        dtype=torch.float32,
+        device=None,
+        chunks: int = 1,
+    ):
+        """Device setup: the mesh's rest positions and cells go to
+        ``apl_mesh_create_from_points``; materials come from its cell data."""
+        dtype = as_dtype(dtype)
+        f = mesh.cell_data.get(FRACTION.vtk)
+        dmesh = DeviceMesh(
+            mesh.cells, None, None, mesh.n_points, dtype=dtype, device=device, chunks=chunks, points=mesh.points,
+            fraction=None if f is None else np.asarray(f, np.float64),
+        )
+        materials = cls.materials_from_region(mesh, dmesh)
+        pot = cls(dmesh, materials, name=name, requires_grad=requires_grad)
+        pot.cells_host = np.asarray(mesh.cells)
+        return pot
 
     @classmethod
     def materials_from_region(cls, region: Region, mesh: DeviceMesh) -> dict[str, torch.Tensor]:

@@ -85,3 +85,18 @@ def gravity(region: Region, density, g=(0.0, 0.0, -9.8), **kwargs) -> ExternalFo
     np.add.at(mass, region.cells.reshape(-1), np.repeat(rho * region.dV / 4.0, 4))
     force = mass[:, None] * np.asarray(g, np.float64)[None, :]
     return ExternalForce(np.arange(region.mesh.n_points), force, region.mesh.n_points, **kwargs)
+
+
+def gravity_from_potential(potential, density, g=(0.0, 0.0, -9.8), **kwargs) -> ExternalForce:
+    """Lumped gravity load for a potential built without a host region (device
+    setup): f_v = rho g sum_{t ni v} dV_t / 4 with the device's dV (which
+    includes the tets' Fraction), read back once."""
+    mesh = potential.mesh
+    _, dV = mesh.region()
+    dV = dV.double().cpu().numpy()
+    cells = potential.cells_host
+    rho = np.broadcast_to(np.asarray(density, np.float64), (dV.shape[0],))
+    mass = np.zeros(mesh.n_verts)
+    np.add.at(mass, cells.reshape(-1), np.repeat(rho * dV / 4.0, 4))
+    force = mass[:, None] * np.asarray(g, np.float64)[None, :]
+    return ExternalForce(np.arange(mesh.n_verts), force, mesh.n_verts, dtype=mesh.dtype, device=mesh.device, **kwargs)

@@ -62,7 +62,9 @@ class DeviceMesh:
 
     ``cells`` (T,4) int, ``dminv`` (T,3,3) = inverse of dX/dr, ``dV`` (T,),
     all host arrays; ``inst_tet_off`` / ``inst_vtx_off`` describe a batch of
-    independent meshes concatenated into one (tet and vertex ranges).
+    independent meshes concatenated into one (tet and vertex ranges).  With
+    ``points`` (V,3) (and optionally ``fraction`` (T,)) instead of dminv / dV,
+    the element data are computed on the device (``apl_mesh_create_from_points``).
     """
 
     def __init__(
@@ -77,15 +79,28 @@ class DeviceMesh:
         inst_tet_off=None,
         inst_vtx_off=None,
         chunks: int = 1,
+        points=None,
+        fraction=None,
     ):
         self.dtype = as_dtype(dtype)
         self.device = normalize_device(device)
         cells = np.ascontiguousarray(cells, dtype=np.int32)
-        dminv = np.ascontiguousarray(dminv, dtype=np.float64).reshape(-1, 9)
-        dV = np.ascontiguousarray(dV, dtype=np.float64).reshape(-1)
         T = cells.shape[0]
-        if cells.shape != (T, 4) or dminv.shape[0] != T or dV.shape[0] != T:
-            raise ValueError("cells (T,4), dminv (T,3,3) and dV (T,) must agree on T")
+        if points is not None:
+            points = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+            if points.shape[0] != int(n_verts):
+                raise ValueError("points must be (n_verts, 3)")
+            if fraction is not None:
+                fraction = np.ascontiguousarray(fraction, dtype=np.float64).reshape(-1)
+                if fraction.shape[0] != T:
+                    raise ValueError("fraction must have one value per tet")
+            if cells.shape != (T, 4):
+                raise ValueError("cells must be (T,4)")
+        else:
+            dminv = np.ascontiguousarray(dminv, dtype=np.float64).reshape(-1, 9)
+            dV = np.ascontiguousarray(dV, dtype=np.float64).reshape(-1)
+            if cells.shape != (T, 4) or dminv.shape[0] != T or dV.shape[0] != T:
+                raise ValueError("cells (T,4), dminv (T,3,3) and dV (T,) must agree on T")
         self.n_tets = T
         self.n_verts = int(n_verts)
         if inst_tet_off is None:
@@ -98,22 +113,21 @@ class DeviceMesh:
             self.n_inst = self.inst_tet_off.size - 1
         self._handle = ctypes.c_void_p()
         L = _lib.lib()
-        _lib.check(
-            L.apl_mesh_create(
-                _DT[self.dtype],
-                T,
-                self.n_verts,
-                cells.ctypes.data,
-                dminv.ctypes.data,
-                dV.ctypes.data,
-                self.n_inst,
-                self.inst_tet_off.ctypes.data,
-                self.inst_vtx_off.ctypes.data,
-                int(chunks),
-                self.device.index,
-                ctypes.byref(self._handle),
+        common = (self.n_inst, self.inst_tet_off.ctypes.data, self.inst_vtx_off.ctypes.data, int(chunks),
+                  self.device.index, ctypes.byref(self._handle))
+        if points is not None:
+            _lib.check(
+                L.apl_mesh_create_from_points(
+                    _DT[self.dtype], T, self.n_verts, cells.ctypes.data, points.ctypes.data,
+                    fraction.ctypes.data if fraction is not None else None, *common,
+                )
             )
-        )
+        else:
+            _lib.check(
+                L.apl_mesh_create(
+                    _DT[self.dtype], T, self.n_verts, cells.ctypes.data, dminv.ctypes.data, dV.ctypes.data, *common,
+                )
+            )
         self._scratch = torch.zeros(max(self.n_inst, 1) * 2, dtype=torch.float64, device=self.device)
 
     @property
@@ -132,6 +146,13 @@ class DeviceMesh:
             ptr(activation, dtype=self.dtype, device=self.device, shape=(self.n_tets, 6), name="activation")
         _lib.check(_lib.lib().apl_mesh_set_activation(self._handle, activation.data_ptr() if activation is not None else None))
         self._activation = activation
+
+    def region(self) -> tuple[torch.Tensor, torch.Tensor]:
+        """The element data in tet order: (dminv (T,3,3), dV (T,)) device tensors."""
+        dminv = torch.empty(self.n_tets, 3, 3, dtype=self.dtype, device=self.device)
+        dV = torch.empty(self.n_tets, dtype=self.dtype, device=self.device)
+        _lib.check(_lib.lib().apl_mesh_region(self._handle, ptr(dminv), ptr(dV), current_stream(self.device)))
+        return dminv, dV
 
     def info(self) -> dict:
         inf = _lib.MeshInfo()

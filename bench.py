@@ -372,6 +372,8 @@ def main():
         out["combined"] = bench_combined(dev)
     if not (args.quick or args.no_inverse):
         out["inverse"] = bench_inverse(dev)
+    if not args.quick:
+        out["setup"] = bench_setup(dev, n)
 
     # ---------------------------------------------------------------- CPU baseline
     if not (args.quick or args.no_cpu):
@@ -669,6 +671,35 @@ def bench_combined(dev, n_meshes=64, lo=16, hi=48, seed=7, adj_rtol32=1e-6, adj_
     log(f"[bench] combined: energy rel err max {e_err.max():.2e}, dL/dq rel err max {g_err.max():.2e}, "
         f"positions max {x_err.max():.2e} of the bbox diagonal")
     return res
+
+
+def bench_setup(dev, n):
+    """Mesh setup of the n^3 x 5 cube: the host region (numpy inverse / det,
+    as the reference's Region) + apl_mesh_create, against
+    apl_mesh_create_from_points (element data by a device kernel); both
+    include the host block partition and the uploads."""
+    import torch
+
+    from apple_amd.mesh import Region, cube
+    from apple_amd.runtime import DeviceMesh
+
+    m = cube(n)
+    DeviceMesh(m.cells, None, None, m.n_points, dtype=torch.float32, device=dev, points=m.points).close()  # warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = Region.from_mesh(m)
+    t1 = time.perf_counter()
+    a = DeviceMesh(r.cells, r.dminv, r.dV, m.n_points, dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    b = DeviceMesh(m.cells, None, None, m.n_points, dtype=torch.float32, device=dev, points=m.points)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    a.close()
+    b.close()
+    log(f"[bench] setup {n}^3: host region {t1 - t0:.3f} s + mesh {t2 - t1:.3f} s; device setup {t3 - t2:.3f} s")
+    return {"n_tets": m.n_cells, "host_region_s": round(t1 - t0, 4), "host_mesh_create_s": round(t2 - t1, 4),
+            "device_setup_s": round(t3 - t2, 4)}
 
 
 def bench_inverse(dev, n=32, steps=12, prestrain=0.9, lr=0.05, seed=9):

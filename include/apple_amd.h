@@ -68,6 +68,21 @@ int apl_version(void);
 int apl_mesh_create(int dtype, int64_t n_tets, int64_t n_verts, const int32_t* cells, const double* dminv,
                     const double* dv, int64_t n_inst, const int64_t* inst_tet_off, const int64_t* inst_vtx_off,
                     int chunks, int device, apl_mesh** out);
+/* Setup on the device: as apl_mesh_create, but the element data are computed
+ * by a kernel from the rest positions, replacing Region's shape gradients
+ * (jax/fem/region/_region.py:84-108: dXdr = X_cell^T dhdr, its inverse,
+ * dV = det(dXdr) / 6) and the Fraction factor (warp/fem/_base.py:104).
+ * points (host, V x 3), fraction (host, T, or NULL for 1).  Tets are used as
+ * given (an inverted tet gets dV < 0, as in the reference). */
+int apl_mesh_create_from_points(int dtype, int64_t n_tets, int64_t n_verts, const int32_t* cells,
+                                const double* points, const double* fraction, int64_t n_inst,
+                                const int64_t* inst_tet_off, const int64_t* inst_vtx_off, int chunks, int device,
+                                apl_mesh** out);
+/* Read back the element data in the caller's tet order: dminv (T x 3 x 3,
+ * the inverse of dX/dr = rows 1..3 of the region's dhdX) and dv (T, dV
+ * including Fraction), device arrays in the mesh dtype; either may be NULL. */
+int apl_mesh_region(apl_mesh* mesh, void* dminv, void* dv, void* stream);
+
 /* Releases the caller's handle.  Solvers built on the mesh hold their own
  * reference: the device memory is freed when the last of them is destroyed,
  * so the handles may be destroyed in any order. */

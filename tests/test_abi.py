@@ -35,3 +35,13 @@ def test_missing_library_fails_loudly(monkeypatch):
     monkeypatch.setattr(_lib, "LIB_PATH", "/nonexistent/libapple_amd.so")
     with pytest.raises(RuntimeError, match="no CPU fallback"):
         _lib.lib()
+
+
+def test_setup_mode_is_validated():
+    import pytest
+
+    from apple_amd import fem
+    from apple_amd.mesh import cube
+
+    with pytest.raises(ValueError, match="setup"):
+        fem.StableNeoHookean.from_pyvista(cube(2), setup="gpu")
