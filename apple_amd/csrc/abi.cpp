@@ -147,7 +147,7 @@ static int create_common(int dtype, int64_t n_tets, int64_t n_verts, const int32
   rc |= upload(m, &d.inst_vblk_off, L.inst_vblk_off.data(), L.inst_vblk_off.size());
   rc |= upload(m, &d.inst_vtx_off, inst_vtx_off, (size_t)n_inst + 1);
   rc |= alloc_bytes(m, &d.partial, (size_t)MAX_CH * std::max<int64_t>(L.n_slots, 1) * rs);
-  rc |= alloc_bytes(m, (void**)&d.blk_red, 2 * (size_t)std::max<int64_t>(L.nblocks, 1) * sizeof(double));
+  rc |= alloc_bytes(m, (void**)&d.blk_red, RED_STRIDE * (size_t)std::max<int64_t>(L.nblocks, 1) * sizeof(double));
   rc |= alloc_bytes(m, (void**)&d.vblk_dots,
                     (size_t)N_DOTS * std::max<size_t>(L.vblk_inst.size(), 1) * sizeof(double));
   if (rc) {

@@ -21,7 +21,8 @@
 //    vertex-major buffer (a vertex's partials contiguous, in block order);
 //    the vertex pass adds each vertex's contiguous run;
 //  * energies and quadratic forms are summed per thread, then in FP64 by a
-//    fixed wave-shuffle tree and a per-instance pass over the block sums;
+//    fixed wave-shuffle tree into one partial per wave (no block barrier),
+//    and a per-instance pass adds the wave partials in a fixed order;
 //  * the grid is persistent and XCD-aware: workgroup i runs on XCD i % 8, and
 //    XCD x walks every eighth chunk of up to 16 consecutive blocks, so
 //    neighbouring blocks (which share vertices) mostly hit the same L2 while
@@ -69,44 +70,44 @@ struct ElemArgs {
   int accumulate;
 };
 
-__device__ inline double block_sum(double v, double* red) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (lane == 0) red[w] = v;
-  __syncthreads();
-  double r = 0.0;
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int i = 0; i < NTH / 64; ++i) r += red[i];
-  }
-  __syncthreads();
-  return r;
+// Cross-lane moves by DPP (VALU; no LDS round trip as with ds_bpermute).
+// Lanes whose DPP source does not exist (row_shr at a row start, rows the
+// row mask leaves out) read `old` = 0.
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ float dpp_f32(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, ROW_MASK, 0xF, false));
 }
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, ROW_MASK, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xF, false);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+template <class T, int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ T dpp(T v) {
+  if constexpr (sizeof(T) == 4)
+    return dpp_f32<CTRL, ROW_MASK>(v);
+  else
+    return dpp_f64<CTRL, ROW_MASK>(v);
+}
+// gfx9 DPP controls
+constexpr int DPP_QUAD_XOR1 = 0xB1;  // quad_perm [1,0,3,2]: the partner lane ^ 1
+constexpr int DPP_QUAD_XOR2 = 0x4E;  // quad_perm [2,3,0,1]: the partner lane ^ 2
+constexpr int DPP_ROW_SHR1 = 0x111, DPP_ROW_SHR2 = 0x112, DPP_ROW_SHR4 = 0x114, DPP_ROW_SHR8 = 0x118;
+constexpr int DPP_ROW_BCAST15 = 0x142, DPP_ROW_BCAST31 = 0x143;
 
-// Two block sums in one fixed-order tree (result valid in thread 0).
-__device__ inline void block_sum2(double& a, double& b, double* red) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    a += __shfl_down(a, off, 64);
-    b += __shfl_down(b, off, 64);
-  }
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (lane == 0) {
-    red[2 * w] = a;
-    red[2 * w + 1] = b;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    a = 0.0;
-    b = 0.0;
-#pragma unroll
-    for (int i = 0; i < NTH / 64; ++i) {
-      a += red[2 * i];
-      b += red[2 * i + 1];
-    }
-  }
-  __syncthreads();
+// Sum over the 64 lanes of a wave by a fixed DPP tree: prefix sums within
+// each row of 16 lanes, then row 0's total into row 1 and row 2's into
+// row 3, then lane 31's into rows 2-3.  The total is in lane 63.
+__device__ inline double wave_sum(double v) {
+  v += dpp_f64<DPP_ROW_SHR1>(v);
+  v += dpp_f64<DPP_ROW_SHR2>(v);
+  v += dpp_f64<DPP_ROW_SHR4>(v);
+  v += dpp_f64<DPP_ROW_SHR8>(v);
+  v += dpp_f64<DPP_ROW_BCAST15, 0xA>(v);
+  v += dpp_f64<DPP_ROW_BCAST31, 0xC>(v);
+  return v;
 }
 
 // Element i of a uniform base pointer through a 32-bit byte offset: maps to
@@ -230,7 +231,7 @@ __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R, X> A) {
   static_assert(NCH <= MAX_CH, "too many channels");
 
   extern __shared__ double smem_d[];
-  double* red = smem_d;
+  double* red = smem_d;                          // wave partials of the block sums
   X* u_lds = reinterpret_cast<X*>(smem_d + 8);  // 4 per slot (padded)
   R* v_lds = reinterpret_cast<R*>(u_lds + 4 * A.nloc_max);
   R* contrib = v_lds + (NEEDV ? 4 * A.nloc_max : 0);  // [channel group][position][channel in group]
@@ -470,13 +471,13 @@ __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R, X> A) {
 #pragma unroll
           for (int k = 0; k < SPT; ++k)
 #pragma unroll
-            for (int ch = 0; ch < NCH; ++ch) acc[k][ch] += __shfl_xor(acc[k][ch], 1, 64);
+            for (int ch = 0; ch < NCH; ++ch) acc[k][ch] += dpp<R, DPP_QUAD_XOR1>(acc[k][ch]);
         }
         if (lg > 1) {
 #pragma unroll
           for (int k = 0; k < SPT; ++k)
 #pragma unroll
-            for (int ch = 0; ch < NCH; ++ch) acc[k][ch] += __shfl_xor(acc[k][ch], 2, 64);
+            for (int ch = 0; ch < NCH; ++ch) acc[k][ch] += dpp<R, DPP_QUAD_XOR2>(acc[k][ch]);
         }
         const int32_t* dsts = A.loc_dst + sb;
 #pragma unroll
@@ -490,18 +491,31 @@ __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R, X> A) {
           }
         }
       }
-      if constexpr ((WHAT & W_E) != 0 && (WHAT & W_HQ) != 0) {
-        double e = (double)e_acc, q = (double)q_acc;
-        block_sum2(e, q, red);
-        if (tid == 0) {
-          A.blk_red[2 * b] = e;
-          A.blk_red[2 * b + 1] = q;
+      if constexpr (RED) {
+        // one FP64 partial per wave and column into LDS; thread 0 adds them
+        // in a fixed order after one barrier (the block's closing barrier
+        // keeps red from being overwritten before it has read it)
+        const int lane = tid & 63, w = tid >> 6;
+        if constexpr ((WHAT & W_E) != 0) {
+          const double e = wave_sum((double)e_acc);
+          if (lane == 63) red[2 * w] = e;
         }
-      } else if constexpr (RED) {
-        // one of the two sums; its readers never look at the other column
-        constexpr int col = (WHAT & W_E) != 0 ? 0 : 1;
-        const double r = block_sum((double)(col == 0 ? e_acc : q_acc), red);
-        if (tid == 0) A.blk_red[2 * b + col] = r;
+        if constexpr ((WHAT & W_HQ) != 0) {
+          const double q = wave_sum((double)q_acc);
+          if (lane == 63) red[2 * w + 1] = q;
+        }
+        __syncthreads();
+        if (tid == 0) {
+#pragma unroll
+          for (int col = 0; col < 2; ++col) {
+            if (col == 0 && (WHAT & W_E) == 0) continue;
+            if (col == 1 && (WHAT & W_HQ) == 0) continue;
+            double r = 0.0;
+#pragma unroll
+            for (int ww = 0; ww < NTH / 64; ++ww) r += red[2 * ww + col];
+            A.blk_red[(size_t)RED_STRIDE * b + col] = r;
+          }
+        }
       }
       __syncthreads();
       });
@@ -539,7 +553,7 @@ __global__ __launch_bounds__(NTH) void finish_kernel(FinishArgs<R> A) {
     if (A.active != nullptr && A.active[i] == 0) return;
     const long long b0 = A.inst_blk_off[i];
     double eq[2];
-    flat_column_sums<2, NTH>(A.blk_red + 2 * b0, 2 * (A.inst_blk_off[i + 1] - b0), red, eq);
+    flat_column_sums<2, NTH>(A.blk_red + RED_STRIDE * b0, RED_STRIDE * (A.inst_blk_off[i + 1] - b0), red, eq);
     if (threadIdx.x == 0) {
       if (A.out_e != nullptr) A.out_e[i] = A.accumulate ? A.out_e[i] + eq[0] : eq[0];
       if (A.out_q != nullptr) A.out_q[i] = A.accumulate ? A.out_q[i] + eq[1] : eq[1];

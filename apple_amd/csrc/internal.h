@@ -14,6 +14,7 @@ namespace apl {
 constexpr int NTH = 256;       // threads per element / vector block
 constexpr int MAX_CH = 6;      // scatter channels per incidence (grad + hdiag)
 constexpr int N_DOTS = 16;     // per vertex-block partial dot products
+constexpr int RED_STRIDE = 2;  // per element block: energy, quadratic form
 
 enum Dtype : int { DT_F32 = 0, DT_F64 = 1 };
 
@@ -48,7 +49,7 @@ struct DevLayout {
   int64_t* inst_vtx_off = nullptr;
   // workspace
   void* partial = nullptr;   // n_slots x MAX_CH, vertex-major (one partial sum per local slot)
-  double* blk_red = nullptr; // 2 * nblocks
+  double* blk_red = nullptr; // RED_STRIDE * nblocks
   double* vblk_dots = nullptr;  // N_DOTS * nvblk
 };
 

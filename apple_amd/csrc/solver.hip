@@ -104,7 +104,7 @@ __device__ inline void energy_share(const double* blk_red, const int64_t* inst_b
   const long long per = (nb + nvb - 1) / nvb;
   const long long lo = b0 + (vb - vb0) * per;
   const long long hi = lo + per < b0 + nb ? lo + per : b0 + nb;
-  for (long long b = lo + threadIdx.x; b < hi; b += NTH) acc += blk_red[2 * b];
+  for (long long b = lo + threadIdx.x; b < hi; b += NTH) acc += blk_red[RED_STRIDE * b];
 }
 
 // Persistent loop of a vector kernel over its vertex blocks blockIdx.x +
@@ -419,7 +419,7 @@ __global__ __launch_bounds__(SCAL_TH) void pncg_alpha_kernel(StatePtrs st, Param
   if (dir) {
     const long long b0 = inst_blk_off[i];
     double eq[2];
-    flat_column_sums<2, SCAL_TH>(blk_red + 2 * b0, 2 * (inst_blk_off[i + 1] - b0), red, eq);
+    flat_column_sums<2, SCAL_TH>(blk_red + RED_STRIDE * b0, RED_STRIDE * (inst_blk_off[i + 1] - b0), red, eq);
     if (threadIdx.x == 0) {
       const double pHp = eq[1];
       st.pHp[i] = pHp;
