@@ -259,9 +259,18 @@ __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R, X> A) {
   const int n_cand = k0 < nb_x ? (nb_x - k0 + K - 1) / K : 0;
   // the active-block list (solves only) sits after the contribution buffer
   int* act_list = reinterpret_cast<int*>(contrib + (SCATTER ? PK::NCHP * CS : 0));
+  // Passes without a scatter phase have registers to spare: each thread
+  // loads the next block's staging inputs (its slot tid) while the current
+  // block finishes, so that block's staging waits on no global load.  Not in
+  // the scatter passes nor with FP64 displacements: there the registers
+  // cost a wave per SIMD and the gain disappears (measured).
+  constexpr bool PREFETCH = !SCATTER && sizeof(X) == sizeof(R);
+  int pf_b = -1;
+  X pf_x[3] = {X(0), X(0), X(0)};
+  R pf_p[3] = {R(0), R(0), R(0)};
   for_active<NTH>(
       n_cand, A.active != nullptr, act_list, [&](int j) { return xcd_block(k0 + j * K); },
-      [&](int bb) { return A.active[A.blk_inst[bb]] != 0; }, [&](const int b) {
+      [&](int bb) { return A.active[A.blk_inst[bb]] != 0; }, [&](const int b, const int bn) {
       const int inst = A.blk_inst[b];
       const int t0 = A.blk_tet_off[b];
       const int nt = A.blk_tet_off[b + 1] - t0;
@@ -270,6 +279,16 @@ __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R, X> A) {
       const X alpha = A.alpha != nullptr ? (X)A.alpha[inst] : X(0);
       const int lg = (4 * nloc <= SPT * NTH) ? 2 : ((2 * nloc <= SPT * NTH) ? 1 : 0);
       const int tps = 1 << lg;
+      // destinations of this thread's slot partials, loaded early
+      int dst_pre[SPT];
+      if constexpr (SCATTER) {
+#pragma unroll
+        for (int k = 0; k < SPT; ++k) {
+          const int item = tid + k * NTH;
+          const int s = item >> lg;
+          dst_pre[k] = (s < nloc && (item & (tps - 1)) == 0) ? at(A.loc_dst + sb, (unsigned)s) : 0;
+        }
+      }
 
       // this thread's first tet record, loaded before the staging barrier so the
       // two global-load latencies overlap
@@ -280,13 +299,25 @@ __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R, X> A) {
       // stage the block's vertex window (u = x + alpha p for a trial point)
       const int32_t* gids = A.loc_gid + sb;
       for (int s = tid; s < nloc; s += NTH) {
-        const unsigned g3 = 3u * (unsigned)at(gids, (unsigned)s);
-        X ux = at(A.x, g3), uy = at(A.x, g3 + 1), uz = at(A.x, g3 + 2);
+        X ux, uy, uz;
         R px = R(0), py = R(0), pz = R(0);
-        if (NEEDV || A.alpha != nullptr) {
-          px = at(A.p, g3);
-          py = at(A.p, g3 + 1);
-          pz = at(A.p, g3 + 2);
+        if (PREFETCH && s == tid && pf_b == b) {
+          ux = pf_x[0];
+          uy = pf_x[1];
+          uz = pf_x[2];
+          px = pf_p[0];
+          py = pf_p[1];
+          pz = pf_p[2];
+        } else {
+          const unsigned g3 = 3u * (unsigned)at(gids, (unsigned)s);
+          ux = at(A.x, g3);
+          uy = at(A.x, g3 + 1);
+          uz = at(A.x, g3 + 2);
+          if (NEEDV || A.alpha != nullptr) {
+            px = at(A.p, g3);
+            py = at(A.p, g3 + 1);
+            pz = at(A.p, g3 + 2);
+          }
         }
         if (A.alpha != nullptr) {
           ux += alpha * (X)px;
@@ -464,6 +495,24 @@ __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R, X> A) {
         }
       }
 
+      if constexpr (PREFETCH) {
+        if (bn >= 0) {
+          const int sb_n = A.blk_loc_off[bn];
+          const int nloc_n = A.blk_loc_off[bn + 1] - sb_n;
+          pf_b = bn;
+          if (tid < nloc_n) {
+            const unsigned g3 = 3u * (unsigned)at(A.loc_gid + sb_n, (unsigned)tid);
+            pf_x[0] = at(A.x, g3);
+            pf_x[1] = at(A.x, g3 + 1);
+            pf_x[2] = at(A.x, g3 + 2);
+            if (NEEDV || A.alpha != nullptr) {
+              pf_p[0] = at(A.p, g3);
+              pf_p[1] = at(A.p, g3 + 1);
+              pf_p[2] = at(A.p, g3 + 2);
+            }
+          }
+        }
+      }
       if constexpr (SCATTER) {
         // combine the lanes of a slot, then one partial per slot, at the
         // slot's place in the vertex-major order
@@ -479,13 +528,12 @@ __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R, X> A) {
 #pragma unroll
             for (int ch = 0; ch < NCH; ++ch) acc[k][ch] += dpp<R, DPP_QUAD_XOR2>(acc[k][ch]);
         }
-        const int32_t* dsts = A.loc_dst + sb;
 #pragma unroll
         for (int k = 0; k < SPT; ++k) {
           const int item = tid + k * NTH;
           const int s = item >> lg;
           if (s < nloc && (item & (tps - 1)) == 0) {
-            const unsigned j = (unsigned)at(dsts, (unsigned)s) * NCH;
+            const unsigned j = (unsigned)dst_pre[k] * NCH;
 #pragma unroll
             for (int ch = 0; ch < NCH; ++ch) at(A.partial, j + ch) = acc[k][ch];
           }

@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace apl {
 
 // Workgroup-uniform loop over items item_of(j), j = 0 .. n_cand-1.  When
@@ -12,8 +14,9 @@ namespace apl {
 // for all of them in one parallel step and only the active ones are visited,
 // in candidate order, so a pass over a batch whose instances have mostly
 // converged pays two loads instead of a chain of loads per skipped item.
-// With more candidates each one is tested as it comes.  body(item) is called
-// by every thread of the workgroup (it may synchronise).  list: NT ints of
+// With more candidates each one is tested as it comes.  body(item) or
+// body(item, next_item) is called by every thread of the workgroup (it may
+// synchronise).  list: NT ints of
 // LDS (only touched when filter is set).
 template <int NT, class ItemOf, class IsActive, class Body>
 __device__ inline void for_active(int n_cand, bool filter, int* list, ItemOf item_of, IsActive is_active,
@@ -42,7 +45,15 @@ __device__ inline void for_active(int n_cand, bool filter, int* list, ItemOf ite
   for (int q = 0; q < n; ++q) {
     const int it = listed ? list[q] : item_of(q);
     if (filter && !listed && !is_active(it)) continue;
-    body(it);
+    if constexpr (std::is_invocable_v<Body, int, int>) {
+      // body(item, next): next is the item visited after this one, or -1
+      // when it is unknown without testing (unlisted filtered loops) or there
+      // is none; lets the body prefetch for the next item
+      const int next = q + 1 < n ? (listed ? list[q + 1] : (filter ? -1 : item_of(q + 1))) : -1;
+      body(it, next);
+    } else {
+      body(it);
+    }
   }
 }
 
