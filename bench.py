@@ -639,8 +639,13 @@ def bench_combined(dev, n_meshes=64, lo=16, hi=48, seed=7, adj_rtol32=1e-6, adj_
         log(f"[bench] combined {dt}: {res[str(dt).split('.')[-1] + '_s']} s")
     b32, x32, st32, cs32, dmu32, dlam32 = runs[torch.float32]
     b64, x64, st64, cs64, dmu64, dlam64 = runs[torch.float64]
+    # the same FP64 batch solved to the FP32 run's tolerance (rtol 1e-6): how
+    # far an rtol-1e-6 equilibrium may sit from the tight one whatever the
+    # precision (ill-conditioned instances)
+    x64l, st64l, *_ = forward_adjoint(b64, PncgConfig(rtol=1e-6, max_steps=50000), adj_rtol64, adjoint=False)
+    x64l = x64l.double()
     diag = np.sqrt(3.0)
-    e_err, g_err, x_err = [], [], []
+    e_err, g_err, x_err, xl_err = [], [], [], []
     for i in range(n_meshes):
         sv = slice(int(b32.vtx_off[i]), int(b32.vtx_off[i + 1]))
         stt = slice(int(b32.tet_off[i]), int(b32.tet_off[i + 1]))
@@ -649,7 +654,8 @@ def bench_combined(dev, n_meshes=64, lo=16, hi=48, seed=7, adj_rtol32=1e-6, adj_
         gg64 = torch.cat([dmu64[stt], dlam64[stt]])
         g_err.append(float(torch.linalg.norm(gg32 - gg64) / torch.linalg.norm(gg64)))
         x_err.append(float((x32[sv] - x64[sv]).abs().max()) / diag)
-    e_err, g_err, x_err = map(np.asarray, (e_err, g_err, x_err))
+        xl_err.append(float((x64l[sv] - x64[sv]).abs().max()) / diag)
+    e_err, g_err, x_err, xl_err = map(np.asarray, (e_err, g_err, x_err, xl_err))
     ok = np.array([s.success for s in st32])
     res.update({
         "workload": f"{n_meshes} cubes, n ~ U{{{lo}..{hi}}} (seed {seed}), {b32.n_tets} tets, SNH, per-instance mu/nu "
@@ -665,11 +671,17 @@ def bench_combined(dev, n_meshes=64, lo=16, hi=48, seed=7, adj_rtol32=1e-6, adj_
         "within_energy_1e-5": int((e_err <= 1e-5).sum()),
         "within_grad_1e-4": int((g_err <= 1e-4).sum()),
         "within_pos_1e-5_diag": int((x_err <= 1e-5).sum()),
+        "fp64_at_rtol_1e-6": {
+            "pos_err_over_diag_max": float(xl_err.max()),
+            "within_pos_1e-5_diag": int((xl_err <= 1e-5).sum()),
+            "pos_err_over_diag_median": float(np.median(xl_err)),
+        },
         "per_instance": [{"n": sizes[i], "fp32": st32[i].result.name, "e": float(e_err[i]), "g": float(g_err[i]),
-                          "x": float(x_err[i])} for i in range(n_meshes)],
+                          "x": float(x_err[i]), "x_fp64_rtol_1e-6": float(xl_err[i])} for i in range(n_meshes)],
     })
     log(f"[bench] combined: energy rel err max {e_err.max():.2e}, dL/dq rel err max {g_err.max():.2e}, "
-        f"positions max {x_err.max():.2e} of the bbox diagonal")
+        f"positions max {x_err.max():.2e} of the bbox diagonal ({int((x_err <= 1e-5).sum())}/{n_meshes} within 1e-5); "
+        f"FP64 at the same rtol: max {xl_err.max():.2e} ({int((xl_err <= 1e-5).sum())}/{n_meshes} within 1e-5)")
     return res
 
 
