@@ -431,9 +431,11 @@ __host__ __device__ inline Svd<R> svd_rv(const M3<R>& G) {
 // sweeps (hardware-approximate rotations) find the eigenvectors to about
 // FP32 accuracy, the FP64 matrix is rotated into that basis (nearly
 // diagonal), and FP64 sweeps -- whose rotations are tiny, so quadratic
-// convergence finishes them -- complete it.
+// convergence finishes them -- complete it.  Three FP32 sweeps give the
+// same results as four on random and nearly repeated singular values up to
+// |G| ~ 10 (host harness); two do not.
 #ifndef REFINE_F32_SWEEPS
-#define REFINE_F32_SWEEPS 4
+#define REFINE_F32_SWEEPS 3
 #endif
 #ifndef REFINE_F64_SWEEPS
 #define REFINE_F64_SWEEPS 1

@@ -150,7 +150,7 @@ def bench_solve(prob, material, dev, rtol=1e-6):
     x = prob["x0"].clone()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    st = solver.solve(x, prob["f"], mask, PncgConfig(rtol=rtol, max_steps=20000))[0]
+    st = solver.solve(x, prob["f"], mask, PncgConfig(rtol=rtol, max_steps=60000))[0]
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     log(f"[bench] PNCG {material}: {st.result.name} in {st.n_steps} steps, {dt:.3f} s")
