@@ -89,6 +89,11 @@ static int create_common(int dtype, int64_t n_tets, int64_t n_verts, const int32
     set_error("dtype must be APL_F32 or APL_F64");
     return -1;
   }
+  // 32-bit byte offsets in the kernels: at most 2^27 tets and vertices
+  if (n_tets < 0 || n_verts < 0 || n_tets >= (int64_t(1) << 27) || n_verts >= (int64_t(1) << 27)) {
+    set_error("mesh too large or negative size (at most 2^27 tets and vertices)");
+    return -1;
+  }
   if (n_tets > 0 && cells == nullptr) {
     set_error("cells must be non-NULL");
     return -1;

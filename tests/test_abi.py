@@ -1,7 +1,8 @@
 """CPU checks of the C ABI: the library loads and exports every declared symbol.
 
-No compute call is made (there is no GPU here); creation-time validation
-errors are covered by the GPU tests.
+No compute call is made (there is no GPU here); validation that precedes
+any device call (mesh size limits) is checked here, the rest of the
+creation-time validation by the GPU tests.
 """
 
 import ctypes
@@ -45,3 +46,22 @@ def test_setup_mode_is_validated():
 
     with pytest.raises(ValueError, match="setup"):
         fem.StableNeoHookean.from_pyvista(cube(2), setup="gpu")
+
+
+@pytest.mark.parametrize("n_tets,n_verts", [(1 << 27, 8), (4, 1 << 27), (-1, 8)])
+def test_mesh_size_limits_rejected_before_any_device_call(n_tets, n_verts):
+    """At most 2^27 tets and vertices (32-bit byte offsets in the kernels):
+    rejected with a clear error before the cells are read or a device is
+    touched."""
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    L = _lib.lib()
+    cells = (ctypes.c_int32 * 4)(0, 1, 2, 3)
+    dummy = (ctypes.c_double * 9)()  # never read: the size check comes first
+    p = lambda a: ctypes.cast(a, ctypes.c_void_p)
+    out = ctypes.c_void_p()
+    for create in (L.apl_mesh_create, L.apl_mesh_create_from_points):
+        rc = create(_lib.APL_F32, n_tets, n_verts, p(cells), p(dummy), p(dummy), 1, None, None, 1, 0,
+                    ctypes.byref(out))
+        assert rc == -1 and out.value is None
+        assert b"2^27" in L.apl_last_error()
