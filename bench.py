@@ -187,6 +187,7 @@ def main():
                     help="central-difference tets of the differentiation check in the full run (--diff-only: 64)")
     ap.add_argument("--no-combined", action="store_true")
     ap.add_argument("--no-inverse", action="store_true")
+    ap.add_argument("--config1-only", action="store_true", help="only the 10^3 cube solved end to end, CPU and GPU")
     ap.add_argument("--inverse-only", action="store_true", help="only the muscle-activation inverse loop")
     ap.add_argument("--batched-meshes", type=int, default=256)
     ap.add_argument("--cpu-threads", type=int, default=16, help="threads of the multi-threaded CPU baseline")
@@ -218,6 +219,9 @@ def main():
         return
     if args.inverse_only:
         print(json.dumps(bench_inverse(dev)), flush=True)
+        return
+    if args.config1_only:
+        print(json.dumps(bench_config1(dev, args.cpu_threads)), flush=True)
         return
     log(f"[bench] building {n}^3 x 5 cube (SNH, FP32)")
     prob = make_cube_problem(n, "snh", dtype, dev, chunks=args.chunks)
@@ -377,6 +381,7 @@ def main():
 
     # ---------------------------------------------------------------- CPU baseline
     if not (args.quick or args.no_cpu):
+        out["config1_solve"] = bench_config1(dev, args.cpu_threads)
         log(f"[bench] CPU baseline (C restatement, {args.cpu_threads} threads and 1 thread)")
         out["cpu_baseline"] = cpu_baseline(prob, target_s=12.0, threads=args.cpu_threads)
         out["cpu_baseline_1thread"] = cpu_baseline(prob, target_s=12.0, threads=1)
@@ -683,6 +688,57 @@ def bench_combined(dev, n_meshes=64, lo=16, hi=48, seed=7, adj_rtol32=1e-6, adj_
         f"positions max {x_err.max():.2e} of the bbox diagonal ({int((x_err <= 1e-5).sum())}/{n_meshes} within 1e-5); "
         f"FP64 at the same rtol: max {xl_err.max():.2e} ({int((xl_err <= 1e-5).sum())}/{n_meshes} within 1e-5)")
     return res
+
+
+def bench_config1(dev, cpu_threads):
+    """The CPU configuration solved end to end: 10^3 x 5 cube, SNH mu = 1e4,
+    lambda = 1e5, rho = 1e3, lumped gravity, z = 0 face fixed, u0 ~ N(0, 0.01)
+    on the free DOFs (seed 0), PNCG to 1e-6.  CPU: the C restatement's PNCG
+    (FP64, separate passes with scatter) on 1 and cpu_threads threads; GPU:
+    the device PNCG in FP32 and FP64 (mesh and graph set up beforehand)."""
+    import torch
+
+    from apple_amd import fem
+    from apple_amd.mesh import Region, cube
+    from apple_amd.solver import DeviceSolver, PncgConfig, mask_bits
+    from oracle import c_oracle
+
+    m = cube(10)
+    m.cell_data["mu"] = np.full(m.n_cells, 1e4)
+    m.cell_data["lambda"] = np.full(m.n_cells, 1e5)
+    region = Region.from_mesh(m)
+    f = np.zeros((m.n_points, 3))
+    np.add.at(f[:, 2], m.cells.reshape(-1), np.repeat(-9.8 * 1e3 * region.dV / 4.0, 4))
+    fixed = np.zeros((m.n_points, 3), bool)
+    fixed[m.points[:, 2] == 0.0] = True
+    x0 = 0.01 * np.random.default_rng(0).standard_normal((m.n_points, 3))
+    x0[fixed] = 0.0
+    out = {"workload": "10^3 x 5 cube (5000 tets), SNH mu 1e4 lambda 1e5, gravity rho 1e3, z = 0 fixed, "
+                       "u0 ~ N(0, 0.01) seed 0, PNCG to 1e-6 relative gradient"}
+    cp = c_oracle.SnhProblem(m.cells, region.dhdX, region.dV, m.cell_data["mu"], m.cell_data["lambda"])
+    for threads in (1, cpu_threads):
+        used = c_oracle.set_threads(threads)
+        t0 = time.perf_counter()
+        _, steps, status, _ = cp.pncg(x0, f, ~fixed.reshape(-1), rtol=1e-6, max_steps=20000)
+        dt = time.perf_counter() - t0
+        out[f"cpu_{used}_threads"] = {"s": round(dt, 4), "steps": int(steps), "status": int(status)}
+    for dtype in (torch.float32, torch.float64):
+        pot = fem.StableNeoHookean.from_region(region, dtype=dtype, device=dev)
+        solver = DeviceSolver(pot)
+        t = lambda a: torch.as_tensor(a, dtype=dtype, device=dev).contiguous()
+        mask = mask_bits(fixed, dev)
+        xw = t(x0)
+        solver.solve(xw, t(f), mask, PncgConfig(rtol=1e-6, max_steps=16))  # graph capture
+        x = t(x0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = solver.solve(x, t(f), mask, PncgConfig(rtol=1e-6, max_steps=20000))[0]
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        out[f"gpu_{str(dtype).split('.')[-1]}"] = {"s": round(dt, 4), "steps": st.n_steps, "status": st.result.name}
+    log(f"[bench] config 1 end to end: CPU 1 thread {out['cpu_1_threads']['s']} s, GPU FP32 "
+        f"{out['gpu_float32']['s']} s, FP64 {out['gpu_float64']['s']} s")
+    return out
 
 
 def bench_setup(dev, n):
