@@ -247,6 +247,8 @@ def main():
 
     if args.profile_pass:
         bench_pass(pot, u, what_a, 20)
+        aprob = make_cube_problem(n, "arap", dtype, dev)  # ARAP pass A: its VALU count against the ALU roof
+        bench_pass(aprob["pot"], aprob["x0"], what_a, 20)
         a = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device=dev)  # 256 MiB
         b = torch.empty_like(a)
         for _ in range(3):
