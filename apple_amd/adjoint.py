@@ -122,5 +122,6 @@ def equilibrium(
 
 
 def last_stats():
-    """(forward stats, adjoint stats) of the most recent call on this thread."""
+    """(forward stats, adjoint stats) of the most recent forward / backward
+    (process-wide: the last call of any thread)."""
     return getattr(_Equilibrium, "last_forward_stats", None), getattr(_Equilibrium, "last_adjoint_stats", None)

@@ -92,6 +92,8 @@ def gravity_from_potential(potential, density, g=(0.0, 0.0, -9.8), **kwargs) -> 
     setup): f_v = rho g sum_{t ni v} dV_t / 4 with the device's dV (which
     includes the tets' Fraction), read back once."""
     mesh = potential.mesh
+    if getattr(potential, "cells_host", None) is None:
+        raise ValueError("the potential has no host cells (build it with from_region / from_pyvista)")
     _, dV = mesh.region()
     dV = dV.double().cpu().numpy()
     cells = potential.cells_host
