@@ -1,5 +1,8 @@
 """Shared test inputs: seeded cube meshes, materials, displacement fields."""
 
+import json
+import os
+
 import numpy as np
 
 import oracle.fem as of
@@ -35,3 +38,21 @@ def rel(a, b):
     b = np.asarray(b, np.float64)
     nb = np.linalg.norm(b)
     return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
+
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def patch_test():
+    """The reference's known-answer patch test as data (golden/patch_test.json):
+    points (5,3), cells (4,4), fixed mask / values (5,3) and the expected answer."""
+    with open(os.path.join(GOLDEN, "patch_test.json")) as f:
+        d = json.load(f)
+    points = np.asarray(d["points"], np.float64)
+    cells = np.asarray(d["cells"], np.int32)
+    mask = np.zeros(points.shape, bool)
+    values = np.zeros(points.shape)
+    for v, val in zip(d["fixed_vertices"], d["fixed_values"]):
+        mask[v] = True
+        values[v] = val
+    return points, cells, mask, values, d

@@ -20,13 +20,11 @@ import oracle.pncg as op
 from apple_amd import batch, fem, forward, potential
 from apple_amd.mesh import Mesh, Region, cube
 from apple_amd.solver import DeviceSolver, PncgConfig, Result, mask_bits
+from helpers import patch_test
 
 pytestmark = pytest.mark.gpu
 
-PATCH_POINTS = np.array(
-    [[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0], [0.25, 0.25, 0.25]]
-)
-PATCH_CELLS = np.array([[0, 1, 2, 4], [0, 1, 4, 3], [0, 4, 2, 3], [4, 1, 2, 3]], np.int32)
+PATCH_POINTS, PATCH_CELLS, PATCH_MASK, PATCH_VALUES, PATCH = patch_test()
 
 
 @pytest.mark.parametrize("dtype,atol", [(torch.float64, 1e-8), (torch.float32, 2e-6)])
@@ -34,14 +32,13 @@ PATCH_CELLS = np.array([[0, 1, 2, 4], [0, 1, 4, 3], [0, 4, 2, 3], [4, 1, 2, 3]],
 def test_patch_test_known_answer(dtype, atol, cls):
     print(f"[solver] patch {cls.__name__} {dtype}", flush=True)
     mesh = Mesh(PATCH_POINTS.copy(), PATCH_CELLS.copy())
-    mesh.cell_data["mu"] = np.ones(4)
-    mesh.cell_data["lambda"] = np.ones(4)
+    n = PATCH_CELLS.shape[0]
+    mesh.cell_data["mu"] = np.full(n, PATCH["mu"])
+    mesh.cell_data["lambda"] = np.ones(n)
     builder = forward.ModelBuilder()
     builder.add_vertices(mesh)
-    fixed_mask = np.zeros((5, 3), bool)
-    fixed_value = np.zeros((5, 3))
-    fixed_mask[:4] = True
-    fixed_value[3] = [0.2, -0.1, 0.15]
+    fixed_mask = PATCH_MASK.copy()
+    fixed_value = PATCH_VALUES.copy()
     mesh.point_data["FixedMask"] = fixed_mask
     mesh.point_data["FixedValue"] = fixed_value
     builder.add_fixed(mesh)
@@ -51,12 +48,13 @@ def test_patch_test_known_answer(dtype, atol, cls):
     e0 = float(fwd.problem.fun(fwd.state))
     sol = fwd.step()
     e1 = float(fwd.problem.fun(fwd.state))
-    assert sol.result.name == "PRIMARY_SUCCESS"
-    assert model.n_free == 3
+    exp = PATCH["expected"]
+    assert sol.result.name == exp["result"]
+    assert model.n_free == exp["n_free"]
     assert np.isfinite(e0) and np.isfinite(e1) and e1 < e0
     u = fwd.state.u.double().cpu().numpy()
-    np.testing.assert_allclose(u[:4], fixed_value[:4], atol=1e-7)
-    np.testing.assert_allclose(u[4], [0.05, -0.025, 0.0375], atol=atol)
+    np.testing.assert_allclose(u[fixed_mask[:, 0]], fixed_value[fixed_mask[:, 0]], atol=1e-7)
+    np.testing.assert_allclose(u[exp["interior_vertex"]], exp["interior_displacement"], atol=max(atol, exp["atol"]))
 
 
 def _gravity_case(n, material, seed=0, noise_scale=0.1):

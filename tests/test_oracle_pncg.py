@@ -15,39 +15,34 @@ import pytest
 import oracle.fem as of
 import oracle.pncg as op
 from apple_amd.mesh import cube
+from helpers import patch_test
 
-PATCH_POINTS = np.array(
-    [[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0], [0.25, 0.25, 0.25]]
-)
-# the VTK connectivity of test_static_simulation.py:24-47
-PATCH_CELLS = np.array([[0, 1, 2, 4], [0, 1, 4, 3], [0, 4, 2, 3], [4, 1, 2, 3]])
+PATCH_POINTS, PATCH_CELLS, PATCH_MASK, PATCH_VALUES, PATCH = patch_test()
 
 
 def _patch_problem(material):
     dhdX, dV = of.region(PATCH_POINTS, PATCH_CELLS)
     assert np.all(dV > 0)
-    mu = np.ones(4)
-    lam = np.ones(4) if material == of.SNH else None
+    n = PATCH_CELLS.shape[0]
+    mu = np.full(n, PATCH["mu"])
+    lam = np.ones(n) if material == of.SNH else None
     pot = of.FemPotential(material, PATCH_CELLS, dhdX, dV, mu, lam)
-    mask = np.zeros((5, 3), bool)
-    mask[:4] = True
-    val = np.zeros((5, 3))
-    val[3] = [0.2, -0.1, 0.15]
-    dof = of.DofMap(mask, val)
+    dof = of.DofMap(PATCH_MASK, PATCH_VALUES)
     return op.Problem(of.Model([pot]), dof)
 
 
 @pytest.mark.parametrize("material", [of.ARAP, of.SNH])
 def test_patch_test_known_answer(material):
     prob = _patch_problem(material)
-    assert prob.dof.n_free == 3
-    x0 = np.zeros(3)
+    exp = PATCH["expected"]
+    assert prob.dof.n_free == exp["n_free"]
+    x0 = np.zeros(exp["n_free"])
     f0 = prob.fun(x0)
     sol = prob.solve(x0, rtol=1e-9, max_steps=1500)
-    assert sol.result == op.Result.PRIMARY_SUCCESS
+    assert sol.result.name == exp["result"]
     u = prob.dof.to_full(sol.x)
-    np.testing.assert_allclose(u[:4], prob.dof.to_full(np.zeros(3))[:4])
-    np.testing.assert_allclose(u[4], [0.05, -0.025, 0.0375], atol=1e-8)
+    np.testing.assert_allclose(u[PATCH_MASK[:, 0]], PATCH_VALUES[PATCH_MASK[:, 0]])
+    np.testing.assert_allclose(u[exp["interior_vertex"]], exp["interior_displacement"], atol=exp["atol"])
     assert np.isfinite(f0) and sol.f < f0
 
 
