@@ -115,9 +115,33 @@ class Batch:
             self._solver = DeviceSolver(self.potential)
         return self._solver
 
-    def solve(self, x: torch.Tensor | None = None, config: PncgConfig | None = None) -> tuple[torch.Tensor, list[SolveStats]]:
+    def solve(self, x: torch.Tensor | None = None, config: PncgConfig | None = None, *,
+              refine_fp64: bool = False) -> tuple[torch.Tensor, list[SolveStats]]:
+        """PNCG over every instance.  refine_fp64 (FP32 batches): instances
+        that end in STAGNATION or MAX_STEPS_REACHED -- the FP32 rounding floor
+        of soft, heavily loaded instances -- are solved again in FP64 from
+        their FP32 result, to the FP32 run's absolute target rtol * |g_0|
+        (atol), and their positions and stats replaced.  Instances that met
+        rtol on their FP32 gradient are kept as they are."""
+        config = config or PncgConfig()
         x = self.x0.clone() if x is None else x
         stats = self.solver().solve(x, self.f_ext, self.mask, config)
+        if refine_fp64 and self.dtype == torch.float32:
+            from .solver import Result
+
+            redo = [i for i, st in enumerate(stats) if st.result in (Result.STAGNATION, Result.MAX_STEPS_REACHED)]
+            if redo:
+                parts = self.split(x)
+                sub = [dataclasses.replace(self.instances[i], x0=parts[i].double().cpu().numpy()) for i in redo]
+                b64 = Batch(sub, self.material, dtype=torch.float64, device=self.device)
+                atol = config.rtol * min(stats[i].grad_norm_first for i in redo)
+                cfg64 = dataclasses.replace(config, atol=max(config.atol, atol), wolfe_eps=None)
+                x64, st64 = b64.solve(config=cfg64)
+                self.refine_stats = st64  # the FP64 re-solves as run (their own first gradient)
+                for k, i in enumerate(redo):
+                    parts[i].copy_(b64.split(x64)[k].to(self.dtype))
+                    stats[i] = dataclasses.replace(st64[k], grad_norm_first=stats[i].grad_norm_first)
+                self.refined = redo
         return x, stats
 
     def split(self, x: torch.Tensor) -> list[torch.Tensor]:

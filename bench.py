@@ -651,8 +651,22 @@ def bench_combined(dev, n_meshes=64, lo=16, hi=48, seed=7, adj_rtol32=1e-6, adj_
     # precision (ill-conditioned instances)
     x64l, st64l, *_ = forward_adjoint(b64, PncgConfig(rtol=1e-6, max_steps=50000), adj_rtol64, adjoint=False)
     x64l = x64l.double()
+    # FP32 forward with the FP64 re-solve of the instances that stagnate
+    b32r = batch.Batch(insts, _lib.APL_SNH, dtype=torch.float32, device=dev)
+    cfg32 = PncgConfig(rtol=1e-6, max_steps=50000)
+    b32r.solve(config=PncgConfig(rtol=1e-6, max_steps=16))  # graph capture
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b32r.solve(config=cfg32)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    xr, _ = b32r.solve(config=cfg32, refine_fp64=True)
+    torch.cuda.synchronize()
+    t_ref = time.perf_counter() - t0
+    xr = xr.double()
     diag = np.sqrt(3.0)
-    e_err, g_err, x_err, xl_err = [], [], [], []
+    e_err, g_err, x_err, xl_err, xr_err = [], [], [], [], []
     for i in range(n_meshes):
         sv = slice(int(b32.vtx_off[i]), int(b32.vtx_off[i + 1]))
         stt = slice(int(b32.tet_off[i]), int(b32.tet_off[i + 1]))
@@ -662,7 +676,8 @@ def bench_combined(dev, n_meshes=64, lo=16, hi=48, seed=7, adj_rtol32=1e-6, adj_
         g_err.append(float(torch.linalg.norm(gg32 - gg64) / torch.linalg.norm(gg64)))
         x_err.append(float((x32[sv] - x64[sv]).abs().max()) / diag)
         xl_err.append(float((x64l[sv] - x64[sv]).abs().max()) / diag)
-    e_err, g_err, x_err, xl_err = map(np.asarray, (e_err, g_err, x_err, xl_err))
+        xr_err.append(float((xr[sv] - x64[sv]).abs().max()) / diag)
+    e_err, g_err, x_err, xl_err, xr_err = map(np.asarray, (e_err, g_err, x_err, xl_err, xr_err))
     ok = np.array([s.success for s in st32])
     res.update({
         "workload": f"{n_meshes} cubes, n ~ U{{{lo}..{hi}}} (seed {seed}), {b32.n_tets} tets, SNH, per-instance mu/nu "
@@ -678,6 +693,18 @@ def bench_combined(dev, n_meshes=64, lo=16, hi=48, seed=7, adj_rtol32=1e-6, adj_
         "within_energy_1e-5": int((e_err <= 1e-5).sum()),
         "within_grad_1e-4": int((g_err <= 1e-4).sum()),
         "within_pos_1e-5_diag": int((x_err <= 1e-5).sum()),
+        "fp32_forward_refine_fp64": {
+            "forward_s": round(t_plain, 3),
+            "forward_refined_s": round(t_ref, 3),
+            "refined_instances": len(getattr(b32r, "refined", [])),
+            "refine_runs": [{"n": sizes[i], "steps": r.n_steps, "result": r.result.name,
+                             "fp64_grad_at_start_over_g0": r.grad_norm_first / max(st32[i].grad_norm_first, 1e-300),
+                             "fp64_grad_end_over_g0": r.grad_norm / max(st32[i].grad_norm_first, 1e-300),
+                             "pos_err_over_diag": float(xr_err[i])}
+                            for i, r in zip(getattr(b32r, "refined", []), getattr(b32r, "refine_stats", []))],
+            "pos_err_over_diag_max": float(xr_err.max()),
+            "within_pos_1e-5_diag": int((xr_err <= 1e-5).sum()),
+        },
         "fp64_at_rtol_1e-6": {
             "pos_err_over_diag_max": float(xl_err.max()),
             "within_pos_1e-5_diag": int((xl_err <= 1e-5).sum()),
@@ -688,7 +715,9 @@ def bench_combined(dev, n_meshes=64, lo=16, hi=48, seed=7, adj_rtol32=1e-6, adj_
     })
     log(f"[bench] combined: energy rel err max {e_err.max():.2e}, dL/dq rel err max {g_err.max():.2e}, "
         f"positions max {x_err.max():.2e} of the bbox diagonal ({int((x_err <= 1e-5).sum())}/{n_meshes} within 1e-5); "
-        f"FP64 at the same rtol: max {xl_err.max():.2e} ({int((xl_err <= 1e-5).sum())}/{n_meshes} within 1e-5)")
+        f"FP64 at the same rtol: max {xl_err.max():.2e} ({int((xl_err <= 1e-5).sum())}/{n_meshes} within 1e-5); "
+        f"FP32 + FP64 re-solve of {len(getattr(b32r, 'refined', []))} stagnated: max {xr_err.max():.2e} "
+        f"({int((xr_err <= 1e-5).sum())}/{n_meshes}), forward {t_plain:.2f} s -> {t_ref:.2f} s")
     return res
 
 
