@@ -116,33 +116,86 @@ class Batch:
         return self._solver
 
     def solve(self, x: torch.Tensor | None = None, config: PncgConfig | None = None, *,
-              refine_fp64: bool = False) -> tuple[torch.Tensor, list[SolveStats]]:
-        """PNCG over every instance.  refine_fp64 (FP32 batches): instances
-        that end in STAGNATION or MAX_STEPS_REACHED -- the FP32 rounding floor
-        of soft, heavily loaded instances -- are solved again in FP64 from
-        their FP32 result, to the FP32 run's absolute target rtol * |g_0|
-        (atol), and their positions and stats replaced.  Instances that met
-        rtol on their FP32 gradient are kept as they are."""
+              refine_fp64: bool | str = False) -> tuple[torch.Tensor, list[SolveStats]]:
+        """PNCG over every instance.  For FP32 batches, refine_fp64 re-solves in
+        FP64, from their FP32 result and to the FP32 run's absolute target
+        rtol * |g_0| (atol), the instances whose FP32 answer is not trusted:
+
+        * True / "stagnated": those ending in STAGNATION or MAX_STEPS_REACHED
+          (the FP32 rounding floor of soft, heavily loaded instances);
+        * "checked": in addition every instance whose free-DOF gradient,
+          evaluated once in FP64 at its FP32 solution, exceeds that target
+          (an FP32 gradient can meet rtol while the true one does not).
+
+        "stagnated" re-solves a batch of just those instances; "checked"
+        builds an FP64 copy of the whole batch on first use (kept for later
+        calls) and re-solves on it with the other instances held fixed.
+        Positions and stats of re-solved instances are replaced (stats keep
+        the FP32 run's first gradient norm); ``self.refined`` lists them."""
+        from .solver import Result
+
         config = config or PncgConfig()
         x = self.x0.clone() if x is None else x
         stats = self.solver().solve(x, self.f_ext, self.mask, config)
-        if refine_fp64 and self.dtype == torch.float32:
-            from .solver import Result
-
-            redo = [i for i, st in enumerate(stats) if st.result in (Result.STAGNATION, Result.MAX_STEPS_REACHED)]
-            if redo:
-                parts = self.split(x)
-                sub = [dataclasses.replace(self.instances[i], x0=parts[i].double().cpu().numpy()) for i in redo]
-                b64 = Batch(sub, self.material, dtype=torch.float64, device=self.device)
-                atol = config.rtol * min(stats[i].grad_norm_first for i in redo)
-                cfg64 = dataclasses.replace(config, atol=max(config.atol, atol), wolfe_eps=None)
-                x64, st64 = b64.solve(config=cfg64)
-                self.refine_stats = st64  # the FP64 re-solves as run (their own first gradient)
-                for k, i in enumerate(redo):
-                    parts[i].copy_(b64.split(x64)[k].to(self.dtype))
-                    stats[i] = dataclasses.replace(st64[k], grad_norm_first=stats[i].grad_norm_first)
-                self.refined = redo
+        self.refined = []
+        if not refine_fp64 or self.dtype != torch.float32:
+            return x, stats
+        if refine_fp64 not in (True, "stagnated", "checked"):
+            raise ValueError("refine_fp64 must be False, True / 'stagnated' or 'checked'")
+        redo = {i for i, st in enumerate(stats) if st.result in (Result.STAGNATION, Result.MAX_STEPS_REACHED)}
+        if refine_fp64 == "checked":
+            b64 = self._fp64_copy()
+            x64 = x.double()
+            g = torch.zeros_like(x64)
+            if b64.activation is not None:
+                b64.mesh.set_activation(b64.activation)
+            b64.mesh.eval(b64.material, b64.mu, b64.lam, x64, grad=g)
+            g -= b64.f_ext
+            free = torch.stack([(b64.mask >> c) & 1 == 0 for c in range(3)], dim=1)
+            g = torch.where(free, g, torch.zeros_like(g))
+            for i in range(self.n_inst):
+                sl = slice(int(self.vtx_off[i]), int(self.vtx_off[i + 1]))
+                gn = float(torch.linalg.norm(g[sl]))
+                if gn > config.rtol * stats[i].grad_norm_first + config.atol:
+                    redo.add(i)
+        redo = sorted(redo)
+        if redo and refine_fp64 != "checked":
+            # a batch of just these instances, from their FP32 positions
+            parts = self.split(x)
+            sub = [dataclasses.replace(self.instances[i], x0=parts[i].double().cpu().numpy()) for i in redo]
+            bs = Batch(sub, self.material, dtype=torch.float64, device=self.device)
+            atol = config.rtol * min(stats[i].grad_norm_first for i in redo)
+            cfg64 = dataclasses.replace(config, atol=max(config.atol, atol), wolfe_eps=None)
+            xs, sts = bs.solve(config=cfg64)
+            for k, i in enumerate(redo):
+                parts[i].copy_(bs.split(xs)[k].to(self.dtype))
+                stats[i] = dataclasses.replace(sts[k], grad_norm_first=stats[i].grad_norm_first)
+            self.refine_stats = sts
+        elif redo:
+            # on the kept FP64 copy; the others are held fixed (all DOFs) and converge at once
+            keep = torch.ones(self.n_verts, dtype=torch.bool, device=self.device)
+            for i in redo:
+                keep[int(self.vtx_off[i]):int(self.vtx_off[i + 1])] = False
+            mask = torch.where(keep, torch.full_like(b64.mask, 7), b64.mask)
+            atol = config.rtol * min(stats[i].grad_norm_first for i in redo)
+            cfg64 = dataclasses.replace(config, atol=max(config.atol, atol), wolfe_eps=None)
+            st64 = b64.solver().solve(x64, b64.f_ext, mask, cfg64)
+            parts, parts64 = self.split(x), b64.split(x64)
+            for i in redo:
+                parts[i].copy_(parts64[i].to(self.dtype))
+                stats[i] = dataclasses.replace(st64[i], grad_norm_first=stats[i].grad_norm_first)
+            self.refine_stats = [st64[i] for i in redo]
+        self.refined = redo
         return x, stats
+
+    def _fp64_copy(self) -> "Batch":
+        if getattr(self, "_b64", None) is None:
+            self._b64 = Batch(self.instances, self.material, dtype=torch.float64, device=self.device)
+        return self._b64
+
+    @property
+    def n_verts(self) -> int:
+        return int(self.vtx_off[-1])
 
     def split(self, x: torch.Tensor) -> list[torch.Tensor]:
         return [x[self.vtx_off[i] : self.vtx_off[i + 1]] for i in range(self.n_inst)]

@@ -665,8 +665,22 @@ def bench_combined(dev, n_meshes=64, lo=16, hi=48, seed=7, adj_rtol32=1e-6, adj_
     torch.cuda.synchronize()
     t_ref = time.perf_counter() - t0
     xr = xr.double()
+    n_ref = len(b32r.refined)
+    ref_runs_stats = list(zip(b32r.refined, getattr(b32r, "refine_stats", [])))
+    # "checked": an FP64 gradient check of every FP32 solution (first call
+    # builds the FP64 copy of the batch), then the FP64 re-solve
+    t0 = time.perf_counter()
+    xc, _ = b32r.solve(config=cfg32, refine_fp64="checked")
+    torch.cuda.synchronize()
+    t_chk_first = time.perf_counter() - t0
+    n_chk = len(b32r.refined)
+    t0 = time.perf_counter()
+    b32r.solve(config=cfg32, refine_fp64="checked")
+    torch.cuda.synchronize()
+    t_chk = time.perf_counter() - t0
+    xc = xc.double()
     diag = np.sqrt(3.0)
-    e_err, g_err, x_err, xl_err, xr_err = [], [], [], [], []
+    e_err, g_err, x_err, xl_err, xr_err, xc_err = [], [], [], [], [], []
     for i in range(n_meshes):
         sv = slice(int(b32.vtx_off[i]), int(b32.vtx_off[i + 1]))
         stt = slice(int(b32.tet_off[i]), int(b32.tet_off[i + 1]))
@@ -677,7 +691,8 @@ def bench_combined(dev, n_meshes=64, lo=16, hi=48, seed=7, adj_rtol32=1e-6, adj_
         x_err.append(float((x32[sv] - x64[sv]).abs().max()) / diag)
         xl_err.append(float((x64l[sv] - x64[sv]).abs().max()) / diag)
         xr_err.append(float((xr[sv] - x64[sv]).abs().max()) / diag)
-    e_err, g_err, x_err, xl_err, xr_err = map(np.asarray, (e_err, g_err, x_err, xl_err, xr_err))
+        xc_err.append(float((xc[sv] - x64[sv]).abs().max()) / diag)
+    e_err, g_err, x_err, xl_err, xr_err, xc_err = map(np.asarray, (e_err, g_err, x_err, xl_err, xr_err, xc_err))
     ok = np.array([s.success for s in st32])
     res.update({
         "workload": f"{n_meshes} cubes, n ~ U{{{lo}..{hi}}} (seed {seed}), {b32.n_tets} tets, SNH, per-instance mu/nu "
@@ -696,14 +711,21 @@ def bench_combined(dev, n_meshes=64, lo=16, hi=48, seed=7, adj_rtol32=1e-6, adj_
         "fp32_forward_refine_fp64": {
             "forward_s": round(t_plain, 3),
             "forward_refined_s": round(t_ref, 3),
-            "refined_instances": len(getattr(b32r, "refined", [])),
+            "refined_instances": n_ref,
             "refine_runs": [{"n": sizes[i], "steps": r.n_steps, "result": r.result.name,
                              "fp64_grad_at_start_over_g0": r.grad_norm_first / max(st32[i].grad_norm_first, 1e-300),
                              "fp64_grad_end_over_g0": r.grad_norm / max(st32[i].grad_norm_first, 1e-300),
                              "pos_err_over_diag": float(xr_err[i])}
-                            for i, r in zip(getattr(b32r, "refined", []), getattr(b32r, "refine_stats", []))],
+                            for i, r in ref_runs_stats],
             "pos_err_over_diag_max": float(xr_err.max()),
             "within_pos_1e-5_diag": int((xr_err <= 1e-5).sum()),
+        },
+        "fp32_forward_checked_fp64": {
+            "forward_checked_first_s": round(t_chk_first, 3),
+            "forward_checked_s": round(t_chk, 3),
+            "refined_instances": n_chk,
+            "pos_err_over_diag_max": float(xc_err.max()),
+            "within_pos_1e-5_diag": int((xc_err <= 1e-5).sum()),
         },
         "fp64_at_rtol_1e-6": {
             "pos_err_over_diag_max": float(xl_err.max()),
@@ -716,8 +738,10 @@ def bench_combined(dev, n_meshes=64, lo=16, hi=48, seed=7, adj_rtol32=1e-6, adj_
     log(f"[bench] combined: energy rel err max {e_err.max():.2e}, dL/dq rel err max {g_err.max():.2e}, "
         f"positions max {x_err.max():.2e} of the bbox diagonal ({int((x_err <= 1e-5).sum())}/{n_meshes} within 1e-5); "
         f"FP64 at the same rtol: max {xl_err.max():.2e} ({int((xl_err <= 1e-5).sum())}/{n_meshes} within 1e-5); "
-        f"FP32 + FP64 re-solve of {len(getattr(b32r, 'refined', []))} stagnated: max {xr_err.max():.2e} "
-        f"({int((xr_err <= 1e-5).sum())}/{n_meshes}), forward {t_plain:.2f} s -> {t_ref:.2f} s")
+        f"FP32 + FP64 re-solve of {n_ref} stagnated: max {xr_err.max():.2e} "
+        f"({int((xr_err <= 1e-5).sum())}/{n_meshes}), forward {t_plain:.2f} s -> {t_ref:.2f} s; checked: {n_chk} "
+        f"re-solved, max {xc_err.max():.2e} ({int((xc_err <= 1e-5).sum())}/{n_meshes}), {t_chk_first:.2f} s first, "
+        f"{t_chk:.2f} s")
     return res
 
 
