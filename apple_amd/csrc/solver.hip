@@ -53,6 +53,38 @@ struct StatePtrs {
 };
 constexpr int N_SD = 12, N_SI = 11;
 
+// One instance's state in registers.  The scalar kernels load it whole (all
+// loads independent, issued together and overlapped with the table sums),
+// decide in registers and store it back whole: the state arrays may alias as
+// far as the compiler knows, so reading and writing them field by field
+// chains one memory round trip per access.
+struct InstState {
+  double f, gd, alpha, alpha_commit, beta, gnorm0, gnorm, pmean, f_first, pHp, gbest, f_mark;
+  int32_t status, act_eval, act_dir, commit, n_steps, n_evals, ls_step, n_backtracks, phase, restarts, best_step;
+};
+
+__device__ inline InstState load_state(const StatePtrs& st, int i) {
+  InstState s;
+  s.f = st.f[i], s.gd = st.gd[i], s.alpha = st.alpha[i], s.alpha_commit = st.alpha_commit[i];
+  s.beta = st.beta[i], s.gnorm0 = st.gnorm0[i], s.gnorm = st.gnorm[i], s.pmean = st.pmean[i];
+  s.f_first = st.f_first[i], s.pHp = st.pHp[i], s.gbest = st.gbest[i], s.f_mark = st.f_mark[i];
+  s.status = st.status[i], s.act_eval = st.act_eval[i], s.act_dir = st.act_dir[i], s.commit = st.commit[i];
+  s.n_steps = st.n_steps[i], s.n_evals = st.n_evals[i], s.ls_step = st.ls_step[i];
+  s.n_backtracks = st.n_backtracks[i], s.phase = st.phase[i], s.restarts = st.restarts[i];
+  s.best_step = st.best_step[i];
+  return s;
+}
+
+__device__ inline void store_state(const StatePtrs& st, int i, const InstState& s) {
+  st.f[i] = s.f, st.gd[i] = s.gd, st.alpha[i] = s.alpha, st.alpha_commit[i] = s.alpha_commit;
+  st.beta[i] = s.beta, st.gnorm0[i] = s.gnorm0, st.gnorm[i] = s.gnorm, st.pmean[i] = s.pmean;
+  st.f_first[i] = s.f_first, st.pHp[i] = s.pHp, st.gbest[i] = s.gbest, st.f_mark[i] = s.f_mark;
+  st.status[i] = s.status, st.act_eval[i] = s.act_eval, st.act_dir[i] = s.act_dir, st.commit[i] = s.commit;
+  st.n_steps[i] = s.n_steps, st.n_evals[i] = s.n_evals, st.ls_step[i] = s.ls_step;
+  st.n_backtracks[i] = s.n_backtracks, st.phase[i] = s.phase, st.restarts[i] = s.restarts;
+  st.best_step[i] = s.best_step;
+}
+
 struct Params {
   double rtol, atol, overstep, c1, wolfe_delta, wolfe_eps, beta_max;
   int max_steps, max_backtracking, stall_steps;
@@ -264,21 +296,25 @@ __global__ __launch_bounds__(SCAL_TH) void pncg_scalar_kernel(StatePtrs st, Para
                                                           const int64_t* inst_vblk_off) {
   __shared__ double sm[(SCAL_TH / 64 + 1) * N_DOTS];
   const int i = blockIdx.x;
-  if (st.act_eval[i] == 0) return;
+  // the active flag, the table bounds and the state are independent: all are
+  // loaded before the first wait (an inactive instance sums an empty table)
+  const bool act = st.act_eval[i] != 0;
+  const long long vb0 = inst_vblk_off[i], vb1 = inst_vblk_off[i + 1];
+  InstState s;
+  if (threadIdx.x == 0) s = load_state(st, i);
   double tot[N_DOTS];
-  const long long vb0 = inst_vblk_off[i];
-  flat_column_sums<N_DOTS, SCAL_TH>(vblk_dots + vb0 * N_DOTS, (inst_vblk_off[i + 1] - vb0) * N_DOTS, sm, tot);
-  if (threadIdx.x != 0) return;
+  flat_column_sums<N_DOTS, SCAL_TH>(vblk_dots + vb0 * N_DOTS, act ? (vb1 - vb0) * N_DOTS : 0, sm, tot);
+  if (threadIdx.x != 0 || !act) return;
   const double fE = tot[D_E];
   const double ft = fE + tot[D_EXT];
   const double gn = sqrt(tot[D_G2]);
-  const int phase = st.phase[i];
-  st.n_evals[i] += 1;
+  const int phase = s.phase;
+  s.n_evals += 1;
   bool accept;
   if (phase != 1) {
     accept = true;
   } else {
-    const double f0 = st.f[i], a = st.alpha[i], gd = st.gd[i];
+    const double f0 = s.f, a = s.alpha, gd = s.gd;
     const bool fin = isfinite(ft);
     const bool armijo = fin && ft <= f0 + P.c1 * a * gd;
     const bool wolfe = P.wolfe_delta > 0.0 && fin && ft <= f0 + P.wolfe_eps * fabs(f0) &&
@@ -286,43 +322,42 @@ __global__ __launch_bounds__(SCAL_TH) void pncg_scalar_kernel(StatePtrs st, Para
     accept = armijo || wolfe;
   }
   if (accept) {
-    st.alpha_commit[i] = phase == 1 ? st.alpha[i] : 0.0;
-    st.f[i] = ft;
-    st.gnorm[i] = gn;
-    st.commit[i] = 1;
+    s.alpha_commit = phase == 1 ? s.alpha : 0.0;
+    s.f = ft;
+    s.gnorm = gn;
+    s.commit = 1;
     if (phase == 0) {
-      st.gnorm0[i] = gn;
-      st.f_first[i] = ft;
+      s.gnorm0 = gn;
+      s.f_first = ft;
     }
     if (phase == 1) {
-      st.n_steps[i] += 1;
-      st.restarts[i] = 0;
+      s.n_steps += 1;
+      s.restarts = 0;
     }
     // progress marker: the last step that lowered the best gradient norm by
     // 10% or the energy, since the previous marker, by more than the energy
     // noise the approximate-Wolfe test tolerates (10 wolfe_eps relative)
-    const bool g_prog = gn < 0.9 * st.gbest[i];
+    const bool g_prog = gn < 0.9 * s.gbest;
     const double f_tol = fmax(10.0 * P.wolfe_eps, 1e-12) * fabs(ft);
-    const bool f_prog = !(st.f_mark[i] - ft <= f_tol);
-    if (g_prog) st.gbest[i] = gn;
+    const bool f_prog = !(s.f_mark - ft <= f_tol);
+    if (g_prog) s.gbest = gn;
     if (g_prog || f_prog) {
-      st.best_step[i] = st.n_steps[i];
-      st.f_mark[i] = ft;
+      s.best_step = s.n_steps;
+      s.f_mark = ft;
     }
     int status = APL_RUNNING;
     if (!isfinite(gn) || !isfinite(ft))
       status = APL_NON_FINITE;
-    else if (gn <= P.rtol * st.gnorm0[i] || gn <= P.atol)
+    else if (gn <= P.rtol * s.gnorm0 || gn <= P.atol)
       status = APL_PRIMARY_SUCCESS;
-    else if (st.n_steps[i] >= P.max_steps)
+    else if (s.n_steps >= P.max_steps)
       status = APL_MAX_STEPS_REACHED;
-    else if (P.stall_steps > 0 && st.n_steps[i] - st.best_step[i] >= P.stall_steps &&
-             st.gbest[i] <= 1e3 * P.rtol * st.gnorm0[i])
+    else if (P.stall_steps > 0 && s.n_steps - s.best_step >= P.stall_steps && s.gbest <= 1e3 * P.rtol * s.gnorm0)
       status = APL_STAGNATION;  // within 3 decades of the target: the rounding floor of the gradient
-    st.status[i] = status;
+    s.status = status;
     if (status == APL_RUNNING) {
       const double pm = tot[D_HCNT] > 0.0 ? tot[D_HSUM] / tot[D_HCNT] : 1.0;
-      st.pmean[i] = pm;
+      s.pmean = pm;
       const double gPg = tot[D_GPG] + tot[D_GPG0] / pm;
       double beta = 0.0;
       if (phase == 1) {
@@ -343,36 +378,37 @@ __global__ __launch_bounds__(SCAL_TH) void pncg_scalar_kernel(StatePtrs st, Para
         beta = 0.0;
         gd = -gPg;
       }
-      st.beta[i] = beta;
-      st.gd[i] = gd;
-      st.act_dir[i] = 1;
+      s.beta = beta;
+      s.gd = gd;
+      s.act_dir = 1;
     } else {
-      st.act_dir[i] = 0;
-      st.act_eval[i] = 0;
+      s.act_dir = 0;
+      s.act_eval = 0;
     }
-    st.ls_step[i] = 0;
-    st.phase[i] = 1;
+    s.ls_step = 0;
+    s.phase = 1;
   } else {
-    st.commit[i] = 0;
-    st.act_dir[i] = 0;
-    st.ls_step[i] += 1;
-    st.n_backtracks[i] += 1;
-    const double a = st.alpha[i] * 0.5;
-    if (st.ls_step[i] > P.max_backtracking || !(a > 0.0)) {
-      if (st.restarts[i] >= 1) {
-        st.status[i] = APL_STAGNATION;
-        st.act_eval[i] = 0;
+    s.commit = 0;
+    s.act_dir = 0;
+    s.ls_step += 1;
+    s.n_backtracks += 1;
+    const double a = s.alpha * 0.5;
+    if (s.ls_step > P.max_backtracking || !(a > 0.0)) {
+      if (s.restarts >= 1) {
+        s.status = APL_STAGNATION;
+        s.act_eval = 0;
       } else {
         // restart from x with steepest descent: re-evaluate at alpha = 0
-        st.restarts[i] += 1;
-        st.phase[i] = 2;
-        st.alpha[i] = 0.0;
-        st.ls_step[i] = 0;
+        s.restarts += 1;
+        s.phase = 2;
+        s.alpha = 0.0;
+        s.ls_step = 0;
       }
     } else {
-      st.alpha[i] = a;
+      s.alpha = a;
     }
   }
+  store_state(st, i, s);
 }
 
 // D: commit and new direction
@@ -415,15 +451,17 @@ __global__ __launch_bounds__(SCAL_TH) void pncg_alpha_kernel(StatePtrs st, Param
                                                          const int64_t* inst_blk_off) {
   __shared__ double red[2 * (SCAL_TH / 64 + 1)];
   const int i = blockIdx.x;
+  // all loads issued before the first wait, as in pncg_scalar_kernel
   const bool dir = st.act_dir[i] != 0;
+  const long long b0 = inst_blk_off[i], b1 = inst_blk_off[i + 1];
+  const double gd = threadIdx.x == 0 ? st.gd[i] : 0.0;
+  double eq[2];
+  flat_column_sums<2, SCAL_TH>(blk_red + RED_STRIDE * b0, dir ? RED_STRIDE * (b1 - b0) : 0, red, eq);
   if (dir) {
-    const long long b0 = inst_blk_off[i];
-    double eq[2];
-    flat_column_sums<2, SCAL_TH>(blk_red + RED_STRIDE * b0, RED_STRIDE * (inst_blk_off[i + 1] - b0), red, eq);
     if (threadIdx.x == 0) {
       const double pHp = eq[1];
       st.pHp[i] = pHp;
-      double a = -st.gd[i] / pHp;
+      double a = -gd / pHp;
       if (isnan(a)) a = 0.0;
       else if (a == -INFINITY) a = 0.0;
       else if (a == INFINITY) a = 1.0;
@@ -554,54 +592,57 @@ __global__ __launch_bounds__(SCAL_TH) void cg_scalar_kernel(StatePtrs st, int st
                                                         const double* vblk_dots, const int64_t* inst_vblk_off) {
   __shared__ double sm[(SCAL_TH / 64 + 1) * N_DOTS];
   const int i = blockIdx.x;
-  if (stage >= 2 && st.act_eval[i] == 0) return;
+  const bool act = stage < 2 || st.act_eval[i] != 0;  // loaded with the rest, as in pncg_scalar_kernel
+  const long long vb0 = inst_vblk_off[i], vb1 = inst_vblk_off[i + 1];
+  InstState s;
+  if (threadIdx.x == 0) s = load_state(st, i);
   double tot[N_DOTS];
-  const long long vb0 = inst_vblk_off[i];
-  flat_column_sums<N_DOTS, SCAL_TH>(vblk_dots + vb0 * N_DOTS, (inst_vblk_off[i + 1] - vb0) * N_DOTS, sm, tot);
-  if (threadIdx.x != 0) return;
+  flat_column_sums<N_DOTS, SCAL_TH>(vblk_dots + vb0 * N_DOTS, act ? (vb1 - vb0) * N_DOTS : 0, sm, tot);
+  if (threadIdx.x != 0 || !act) return;
   if (stage == 0) {
-    st.pmean[i] = tot[D_HCNT] > 0.0 ? tot[D_HSUM] / tot[D_HCNT] : 1.0;
+    s.pmean = tot[D_HCNT] > 0.0 ? tot[D_HSUM] / tot[D_HCNT] : 1.0;
     const double bn = sqrt(tot[D_G2]);
-    st.gnorm0[i] = bn;
-    st.gnorm[i] = bn;
-    st.n_steps[i] = 0;
-    st.f[i] = bn > 0.0 ? 1.0 : 0.0;
-    st.status[i] = bn > 0.0 ? APL_RUNNING : APL_PRIMARY_SUCCESS;
-    st.act_eval[i] = bn > 0.0 ? 1 : 0;
-    st.alpha[i] = 0.0;
-    st.beta[i] = 0.0;
+    s.gnorm0 = bn;
+    s.gnorm = bn;
+    s.n_steps = 0;
+    s.f = bn > 0.0 ? 1.0 : 0.0;
+    s.status = bn > 0.0 ? APL_RUNNING : APL_PRIMARY_SUCCESS;
+    s.act_eval = bn > 0.0 ? 1 : 0;
+    s.alpha = 0.0;
+    s.beta = 0.0;
   } else if (stage == 1) {
-    st.gd[i] = tot[D_GPG];
+    s.gd = tot[D_GPG];
   } else if (stage == 2) {
     const double dq = tot[D_GTP];
-    st.pHp[i] = dq;
+    s.pHp = dq;
     if (!(dq > 0.0) || !isfinite(dq)) {
-      st.status[i] = APL_NON_FINITE;
-      st.act_eval[i] = 0;
-      st.alpha[i] = 0.0;
+      s.status = APL_NON_FINITE;
+      s.act_eval = 0;
+      s.alpha = 0.0;
     } else {
-      st.alpha[i] = st.gd[i] / dq;
+      s.alpha = s.gd / dq;
     }
   } else {
     const double rz = tot[D_GPG];
     const double rn = sqrt(tot[D_G2]);
-    st.gnorm[i] = rn;
-    st.n_steps[i] += 1;
-    st.f[i] = st.gnorm0[i] > 0.0 ? rn / st.gnorm0[i] : 0.0;
-    if (rn <= rtol * st.gnorm0[i]) {
-      st.status[i] = APL_PRIMARY_SUCCESS;
-      st.act_eval[i] = 0;
+    s.gnorm = rn;
+    s.n_steps += 1;
+    s.f = s.gnorm0 > 0.0 ? rn / s.gnorm0 : 0.0;
+    if (rn <= rtol * s.gnorm0) {
+      s.status = APL_PRIMARY_SUCCESS;
+      s.act_eval = 0;
     } else if (!isfinite(rn)) {
-      st.status[i] = APL_NON_FINITE;
-      st.act_eval[i] = 0;
-    } else if (st.n_steps[i] >= max_iters) {
-      st.status[i] = APL_MAX_STEPS_REACHED;
-      st.act_eval[i] = 0;
+      s.status = APL_NON_FINITE;
+      s.act_eval = 0;
+    } else if (s.n_steps >= max_iters) {
+      s.status = APL_MAX_STEPS_REACHED;
+      s.act_eval = 0;
     } else {
-      st.beta[i] = rz / st.gd[i];
-      st.gd[i] = rz;
+      s.beta = rz / s.gd;
+      s.gd = rz;
     }
   }
+  store_state(st, i, s);
 }
 
 }  // namespace apl
