@@ -39,6 +39,7 @@ import re
 import xml.etree.ElementTree as ET
 import zlib
 from collections.abc import Iterator
+from xml.sax.saxutils import quoteattr
 from typing import Any
 
 import numpy as np
@@ -160,13 +161,13 @@ def save_vtu(path: str | os.PathLike, mesh: Mesh, *, encoding: str = "binary", c
     appended: list[bytes] = []
     app_off = 0
 
-    def data_array(a: np.ndarray, name: str | None) -> str:
+    def data_array(a, name: str | None) -> str:
         nonlocal app_off
-        a = np.asarray(a)
+        a = a.detach().cpu().numpy() if hasattr(a, "detach") else np.asarray(a)  # torch tensors on any device
         n = a.shape[0] if a.ndim else 1
         ncomp = int(np.prod(a.shape[1:])) if a.ndim > 1 else 1
         vtype, a = _vtk_type(a.reshape(n, ncomp) if a.ndim > 1 else a.reshape(n))
-        attrs = f'type="{vtype}"' + (f' Name="{name}"' if name is not None else "")
+        attrs = f'type="{vtype}"' + (f" Name={quoteattr(name)}" if name is not None else "")
         attrs += f' NumberOfComponents="{ncomp}"'
         if encoding == "ascii":
             if a.dtype.kind == "f":
@@ -186,7 +187,7 @@ def save_vtu(path: str | os.PathLike, mesh: Mesh, *, encoding: str = "binary", c
         return f'<DataArray {attrs} format="appended" offset="{off}"/>'
 
     def section(tag: str, data: dict[str, Any]) -> str:
-        inner = "".join(data_array(np.asarray(v), str(k)) for k, v in data.items())
+        inner = "".join(data_array(v, str(k)) for k, v in data.items())
         return f"<{tag}>{inner}</{tag}>"
 
     T = cells.shape[0]

@@ -209,6 +209,17 @@ def test_reference_attribute_names_survive(tmp_path):
     assert {"FixedMask", "FixedValue", "GlobalPointId"} <= set(r.point_data)
 
 
+def test_torch_fields_and_awkward_names(tmp_path):
+    import torch
+
+    m = cube(1)
+    m.point_data['a "quoted" <name> & more'] = torch.arange(3 * m.n_points, dtype=torch.float32).reshape(-1, 3)
+    vtu.save_vtu(tmp_path / "t.vtu", m)
+    r = vtu.load_vtu(tmp_path / "t.vtu")
+    np.testing.assert_array_equal(r.point_data['a "quoted" <name> & more'],
+                                  np.arange(3 * m.n_points, dtype=np.float32).reshape(-1, 3))
+
+
 def test_with_solution_picks_rows_by_global_point_id():
     m = cube(1)
     m.point_data[GLOBAL_POINT_ID.vtk] = np.arange(m.n_points)[::-1] + 3  # mesh added after 3 other points
