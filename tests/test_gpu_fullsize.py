@@ -1,0 +1,132 @@
+"""Element passes at the benchmark's full size: the 64^3 x 5 cube (1,310,720
+tets, 274,625 vertices), lognormal mu (median 1e4, sigma 0.5), lambda = 10 mu,
+a seeded noisy displacement -- the workload the pass-A throughput is quoted
+on.
+
+* SNH in FP32 against the C restatement of the oracle in FP64 (OpenMP, the
+  same arithmetic as the numpy oracle, checked against it in
+  tests/test_oracle_c.py) on the same float32-rounded inputs: energy 1e-5
+  relative, gradient, Hessian diagonal and quadratic form 1e-4 relative
+  (norm-wise) -- the stated tolerances of the FP32 path.
+* Size-independent properties, SNH and ARAP, in FP64 on the device:
+  hess_prod is linear in the direction (1e-12); hess_quad is quadratic in
+  it (1e-12) and, being the sum of per-tet forms clamped at zero
+  (_base.py:379-380), never below v . hess_prod(v).  ARAP's element
+  Hessian is PSD by its lambda clamp (func/_misc.py:31-43), so no clamp
+  fires and the two are equal (1e-10); SNH's is not (the -mu (J - 1) term
+  is indefinite away from rest), so for a random v they differ.  ARAP in
+  FP32 against the FP64 device pass (itself held to 1e-10 of the numpy
+  oracle at small sizes) at the FP32 tolerances.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from apple_amd import fem
+from helpers import rel, snh_case
+
+pytestmark = pytest.mark.gpu
+
+N = 64
+
+
+@pytest.fixture(scope="module")
+def case():
+    print(f"[fullsize] building the {N}^3 x 5 cube and its region", flush=True)
+    m, region, u, v = snh_case(N, seed=7, noise=0.01)
+    return m, region, u, v
+
+
+def _pass(pot, u, v):
+    V = pot.n_points
+    e = torch.zeros(1, dtype=torch.float64, device="cuda")
+    g = torch.zeros(V, 3, dtype=pot.dtype, device="cuda")
+    h = torch.zeros_like(g)
+    pot.fused(u, energy=e, grad=g, hess_diag=h)
+    hq = torch.zeros(1, dtype=torch.float64, device="cuda")
+    pot.hess_quad(u, v, hq)
+    torch.cuda.synchronize()
+    return e.item(), g.double().cpu().numpy(), h.double().cpu().numpy(), hq.item()
+
+
+def test_snh_fp32_matches_c_oracle_at_full_size(case):
+    from oracle import c_oracle
+
+    m, region, u64, v64 = case
+    print(f"[fullsize] SNH FP32 vs C oracle, {m.n_cells} tets", flush=True)
+    u_r = u64.astype(np.float32).astype(np.float64)
+    v_r = v64.astype(np.float32).astype(np.float64)
+    mu = m.cell_data["mu"].astype(np.float32).astype(np.float64)
+    lam = m.cell_data["lambda"].astype(np.float32).astype(np.float64)
+    pot = fem.StableNeoHookean.from_region(region, dtype=torch.float32, device="cuda")
+    u = torch.as_tensor(u_r, dtype=torch.float32, device="cuda")
+    v = torch.as_tensor(v_r, dtype=torch.float32, device="cuda")
+    e, g, h, hq = _pass(pot, u, v)
+
+    c_oracle.set_threads(16)
+    cp = c_oracle.SnhProblem(region.cells, region.dhdX, region.dV, mu, lam)
+    e_ref = cp.fun(u_r)
+    g_ref = np.zeros_like(u_r)
+    h_ref = np.zeros_like(u_r)
+    cp.pass_a(np.ascontiguousarray(u_r), g_ref, h_ref)
+    hq_ref = cp.hess_quad(u_r, v_r)
+    print(f"[fullsize] energy rel {abs(e - e_ref) / abs(e_ref):.2e}, grad rel {rel(g, g_ref):.2e}, "
+          f"hdiag rel {rel(h, h_ref):.2e}, hq rel {abs(hq - hq_ref) / abs(hq_ref):.2e}", flush=True)
+    assert abs(e - e_ref) <= 1e-5 * abs(e_ref)
+    assert rel(g, g_ref) <= 1e-4
+    assert rel(h, h_ref) <= 1e-4
+    assert abs(hq - hq_ref) <= 1e-4 * abs(hq_ref)
+
+
+@pytest.mark.parametrize("material", ["snh", "arap"])
+def test_fp64_hessian_properties_at_full_size(case, material):
+    m, region, u64, v64 = case
+    print(f"[fullsize] {material} FP64 linearity and v.Hv", flush=True)
+    cls = fem.StableNeoHookean if material == "snh" else fem.Arap
+    pot = cls.from_region(region, dtype=torch.float64, device="cuda")
+    rng = np.random.default_rng(11)
+    u = torch.as_tensor(u64, device="cuda")
+    v1 = torch.as_tensor(v64, device="cuda")
+    v2 = torch.as_tensor(rng.standard_normal(v64.shape), device="cuda")
+    a, b = 0.75, -1.25
+
+    def hp(v):
+        out = torch.zeros_like(v)
+        pot.hess_prod(u, v, out)
+        return out
+
+    lhs = hp(a * v1 + b * v2)
+    rhs = a * hp(v1) + b * hp(v2)
+    torch.cuda.synchronize()
+    assert rel(lhs.cpu().numpy(), rhs.cpu().numpy()) <= 1e-12
+
+    def hq(v):
+        out = torch.zeros(1, dtype=torch.float64, device="cuda")
+        pot.hess_quad(u, v, out)
+        return out.item()
+
+    q1 = hq(v1)
+    assert abs(hq(2.5 * v1) - 6.25 * q1) <= 1e-12 * abs(q1)
+    vhv = float((v1 * hp(v1)).sum())
+    print(f"[fullsize] {material}: hess_quad {q1:.6e}, v.Hv {vhv:.6e}", flush=True)
+    assert q1 >= vhv - 1e-10 * abs(vhv)
+    if material == "arap":
+        assert abs(q1 - vhv) <= 1e-10 * abs(vhv)
+
+
+def test_arap_fp32_matches_fp64_device_pass_at_full_size(case):
+    m, region, u64, v64 = case
+    print("[fullsize] ARAP FP32 vs FP64 device pass", flush=True)
+    u_r = u64.astype(np.float32).astype(np.float64)
+    v_r = v64.astype(np.float32).astype(np.float64)
+    out = {}
+    for dt in (torch.float32, torch.float64):
+        pot = fem.Arap.from_region(region, dtype=dt, device="cuda")
+        out[dt] = _pass(pot, torch.as_tensor(u_r, dtype=dt, device="cuda"), torch.as_tensor(v_r, dtype=dt, device="cuda"))
+        del pot
+    (e32, g32, h32, hq32), (e64, g64, h64, hq64) = out[torch.float32], out[torch.float64]
+    assert abs(e32 - e64) <= 1e-5 * abs(e64)
+    assert rel(g32, g64) <= 1e-4
+    assert rel(h32, h64) <= 1e-4
+    assert abs(hq32 - hq64) <= 1e-4 * abs(hq64)
