@@ -16,7 +16,7 @@ on.
   fires and the two are equal (1e-10); SNH's is not (the -mu (J - 1) term
   is indefinite away from rest), so for a random v they differ.  ARAP in
   FP32 against the FP64 device pass (itself held to 1e-10 of the numpy
-  oracle at small sizes) at the FP32 tolerances.
+  oracle at small sizes) at the FP32 tolerances; the same for the muscle law.
 """
 
 import numpy as np
@@ -130,3 +130,37 @@ def test_arap_fp32_matches_fp64_device_pass_at_full_size(case):
     assert rel(g32, g64) <= 1e-4
     assert rel(h32, h64) <= 1e-4
     assert abs(hq32 - hq64) <= 1e-4 * abs(hq64)
+
+
+def test_muscle_fp32_matches_fp64_device_pass_at_full_size(case):
+    """The muscle law at full size with a random activation (0.15 standard
+    normal per component of sym(A - I)): FP32 vs the FP64 device pass, which is
+    held to 1e-10 of the numpy oracle at small sizes, including the (T, 6)
+    activation derivatives."""
+    m, region, u64, v64 = case
+    print("[fullsize] muscle FP32 vs FP64 device pass", flush=True)
+    rng = np.random.default_rng(107)
+    region.cell_data["activation"] = 0.15 * rng.standard_normal((m.n_cells, 6))
+    u_r = u64.astype(np.float32).astype(np.float64)
+    v_r = v64.astype(np.float32).astype(np.float64)
+    try:
+        out = {}
+        for dt in (torch.float32, torch.float64):
+            pot = fem.StableNeoHookeanMuscle.from_region(region, dtype=dt, device="cuda")
+            u = torch.as_tensor(u_r, dtype=dt, device="cuda")
+            v = torch.as_tensor(v_r, dtype=dt, device="cuda")
+            res = _pass(pot, u, v)
+            mix = pot.mixed_derivative_prod(u, v)
+            torch.cuda.synchronize()
+            out[dt] = res + ({k: t.double().cpu().numpy() for k, t in mix.items()},)
+            del pot
+    finally:
+        del region.cell_data["activation"]
+    (e32, g32, h32, hq32, mx32), (e64, g64, h64, hq64, mx64) = out[torch.float32], out[torch.float64]
+    assert abs(e32 - e64) <= 1e-5 * abs(e64)
+    assert rel(g32, g64) <= 1e-4
+    assert rel(h32, h64) <= 1e-4
+    assert abs(hq32 - hq64) <= 1e-4 * abs(hq64)
+    assert set(mx64) == {"mu", "lambda", "activation"}
+    for k in mx64:
+        assert rel(mx32[k], mx64[k]) <= 1e-4, k
