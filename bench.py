@@ -232,6 +232,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16, help="threads of the multi-threaded CPU baseline")
     ap.add_argument("--profile-pass", action="store_true",
                     help="only run pass-A launches and a reference copy (for rocprofv3 counter runs)")
+    ap.add_argument("--profile-hess-prod", action="store_true",
+                    help="only 20 pass-C (hess_prod, SNH) launches and 3 calibration copies, for rocprofv3")
     ap.add_argument("--profile-solve", type=int, default=0, metavar="STEPS",
                     help="only run a PNCG solve of STEPS iterations (for rocprofv3 kernel traces)")
     args = ap.parse_args()
@@ -294,6 +296,15 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         log("[bench] profile-pass done")
+        return
+    if args.profile_hess_prod:
+        bench_pass(pot, u, _lib.APL_HESS_PROD, 20)
+        a = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device=dev)  # 256 MiB copies: counter calibration
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("[bench] profile-hess-prod done")
         return
     if args.profile_solve:
         from apple_amd.solver import DeviceSolver, PncgConfig, mask_bits
