@@ -5,7 +5,8 @@ liblaf/apple reads its inputs and writes its results as ``.vtu`` files
 through pyvista (``melon.load_unstructured_grid`` / ``melon.save`` in
 exp/2026/03/11/test-inverse/src/10-gen.py:14-17 and 20-forward.py:21-60),
 keyed by the attribute names of ``common/attr_name.py:4-44`` (``mu``,
-``lambda``, ``Activation``, ``FixedMask``, ``GlobalPointId``, ...).  This
+``lambda``, ``FixedMask``, ``FixedValue``, ``GlobalPointId``; the muscle
+activation under the enum value ``activation``, warp/fem/utils/_material.py:16).  This
 module reads and writes the same files without VTK: ``load_vtu`` returns an
 ``apple_amd.mesh.Mesh`` (points, tetra cells, point and cell data), which
 ``from_pyvista`` / ``from_mesh`` of every potential accept, and ``save_vtu``

@@ -11,7 +11,7 @@ import pytest
 import torch
 
 from apple_amd import fem, forward, vtu
-from apple_amd.common import DISPLACEMENT, FIXED_MASK, FIXED_VALUE, LAMBDA, MU
+from apple_amd.common import ACTIVATION, DISPLACEMENT, FIXED_MASK, FIXED_VALUE, LAMBDA, MU
 from apple_amd.mesh import Mesh
 
 from helpers import patch_test
@@ -19,8 +19,10 @@ from helpers import patch_test
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("cls", [fem.Arap, fem.StableNeoHookean])
+@pytest.mark.parametrize("cls", [fem.Arap, fem.StableNeoHookean, fem.StableNeoHookeanMuscle])
 def test_patch_test_through_vtu_files(tmp_path, cls):
+    # the muscle law with zero activation is SNH (bit for bit), so the same
+    # known answer holds; it also reads its activation from the file
     print(f"[vtu] patch {cls.__name__} through .vtu files", flush=True)
     points, cells, mask, values, ref = patch_test()
     src = Mesh(points, cells)
@@ -28,6 +30,8 @@ def test_patch_test_through_vtu_files(tmp_path, cls):
     src.point_data[FIXED_VALUE.vtk] = values
     src.cell_data[MU.vtk] = np.full(cells.shape[0], ref["mu"])
     src.cell_data[LAMBDA.vtk] = np.ones(cells.shape[0])
+    if cls is fem.StableNeoHookeanMuscle:
+        src.cell_data[ACTIVATION.value] = np.zeros((cells.shape[0], 6))  # utils/_material.py:16
     vtu.save_vtu(tmp_path / "input.vtu", src)  # binary, zlib
 
     mesh = vtu.load_vtu(tmp_path / "input.vtu")

@@ -139,7 +139,7 @@ def _attributed_cube(n=3, seed=0):
     m.point_data[GLOBAL_POINT_ID.vtk] = np.arange(m.n_points, dtype=np.int64)
     m.cell_data[MU.vtk] = rng.lognormal(np.log(1e4), 0.5, m.n_cells)
     m.cell_data[LAMBDA.vtk] = (10.0 * m.cell_data[MU.vtk]).astype(np.float32)
-    m.cell_data[ACTIVATION.vtk] = rng.standard_normal((m.n_cells, 6))
+    m.cell_data[ACTIVATION.value] = rng.standard_normal((m.n_cells, 6))  # keyed by the enum value (utils/_material.py:16)
     m.cell_data["tag"] = rng.integers(-5, 5, m.n_cells, dtype=np.int16)
     return m
 
@@ -199,13 +199,17 @@ def test_rejects_other_cells_and_bad_files(tmp_path):
 
 
 def test_reference_attribute_names_survive(tmp_path):
-    # the names a liblaf/apple input file carries (common/attr_name.py:39-44)
-    assert [a.vtk for a in (MU, LAMBDA, ACTIVATION, FIXED_MASK, FIXED_VALUE, GLOBAL_POINT_ID, DISPLACEMENT)] == [
-        "mu", "lambda", "Activation", "FixedMask", "FixedValue", "GlobalPointId", "Displacement"]
+    # the names a liblaf/apple input file carries: .vtk names (common/attr_name.py:39-44)
+    # for mu, lambda and the point fields (utils/_material.py:27-31,
+    # forward/dof_map/_builder.py:36-39), the enum value for the activation
+    # (utils/_material.py:16)
+    assert [a.vtk for a in (MU, LAMBDA, FIXED_MASK, FIXED_VALUE, GLOBAL_POINT_ID, DISPLACEMENT)] == [
+        "mu", "lambda", "FixedMask", "FixedValue", "GlobalPointId", "Displacement"]
+    assert ACTIVATION.value == "activation"
     m = _attributed_cube()
     vtu.save_vtu(tmp_path / "m.vtu", m)
     r = vtu.load_vtu(tmp_path / "m.vtu")
-    assert {"mu", "lambda", "Activation", "tag"} <= set(r.cell_data)
+    assert {"mu", "lambda", "activation", "tag"} <= set(r.cell_data)
     assert {"FixedMask", "FixedValue", "GlobalPointId"} <= set(r.point_data)
 
 
