@@ -131,6 +131,18 @@ def test_appended_base64_known_layout(tmp_path):
     assert m.cell_data["mu"].tolist() == [1.0] and m.cell_data["nu"].tolist() == [1.0]
 
 
+def test_appended_base64_compressed_known_layout(tmp_path):
+    # the zlib payload of test_zlib_known_payload (table and stream, each
+    # base64-encoded) as the first array of a base64 appended section, then
+    # the same again: the second array's offset is the first one's length
+    z = "AQAAAACAAAAEAAAADwAAAA==" + "eAEBBAD7/wcAAAAAIAAI"
+    arrs = ('<DataArray type="Int32" Name="mu" format="appended" offset="0"/>'
+            f'<DataArray type="Int32" Name="tag" format="appended" offset="{len(z)}"/>')
+    tail = b'<AppendedData encoding="base64">_' + (z + z).encode() + b"\n</AppendedData>"
+    m = vtu.load_vtu(_write(tmp_path, "ac.vtu", _one_tet(arrs, "UInt32", "vtkZLibDataCompressor", tail=tail)))
+    assert m.cell_data["mu"].tolist() == [7] and m.cell_data["tag"].tolist() == [7]
+
+
 def _attributed_cube(n=3, seed=0):
     m = cube(n)
     rng = np.random.default_rng(seed)
