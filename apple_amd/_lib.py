@@ -100,6 +100,8 @@ def _declare(lib):
     lib.apl_elem_eval.argtypes = [
         c_vp, ctypes.c_int, c_vp, c_vp, c_vp, c_vp, ctypes.c_uint, c_dp, c_vp, c_vp, c_vp, c_dp, c_vp, c_vp, c_vp,
     ]
+    lib.apl_elem_eval_static.argtypes = lib.apl_elem_eval.argtypes
+    lib.apl_static_scratch_bytes.restype = ctypes.c_size_t
     lib.apl_mesh_set_activation.argtypes = [c_vp, c_vp]
     lib.apl_elem_mixed_activation.argtypes = [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.c_int, c_vp]
     lib.apl_ext_force_eval.argtypes = [c_vp, c_i64, c_vp, c_vp, c_vp, c_dp, c_vp, ctypes.c_int, c_vp]

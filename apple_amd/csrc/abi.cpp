@@ -281,9 +281,19 @@ int apl_mesh_info(const apl_mesh* mesh, apl_mesh_info_t* info) {
   return 0;
 }
 
-int apl_elem_eval(apl_mesh* m, int material, const void* mu, const void* lam, const void* u, const void* p,
-                  unsigned what, double* energy, void* grad, void* hess_diag, void* hess_prod, double* hess_quad,
-                  void* dmu, void* dlam, void* stream) {
+}  // extern "C"
+
+namespace apl {
+
+size_t eval_scratch_bytes(const apl_mesh* m) {
+  const size_t part = (size_t)MAX_CH * (size_t)std::max<int64_t>(m->L.n_slots, 1) * real_size(m);
+  const size_t red = RED_STRIDE * (size_t)std::max<int64_t>(m->L.nblocks, 1) * sizeof(double);
+  return ((part + 255) / 256) * 256 + red;
+}
+
+int elem_eval_impl(apl_mesh* m, int material, const void* mu, const void* lam, const void* u, const void* p,
+                   unsigned what, double* energy, void* grad, void* hess_diag, void* hess_prod, double* hess_quad,
+                   void* dmu, void* dlam, hipStream_t s, const Workspace* ws) {
   if (m == nullptr) {
     set_error("mesh is NULL");
     return -1;
@@ -312,33 +322,51 @@ int apl_elem_eval(apl_mesh* m, int material, const void* mu, const void* lam, co
   }
   APL_CHECK_HIP(hipSetDevice(m->device));
   (void)hipGetLastError();  // drop errors left by unrelated earlier HIP calls
-  hipStream_t s = (hipStream_t)stream;
   const unsigned egh = w & (W_E | W_G | W_H);
   if (egh) {
     // instantiated pass variants: E, G, H, E|G|H; other subsets run as separate passes
-    std::vector<unsigned> passes;
+    unsigned passes[3];
+    int np = 0;
     if (egh == (W_E | W_G | W_H) || egh == W_E || egh == W_G || egh == W_H)
-      passes.push_back(egh);
+      passes[np++] = egh;
     else
       for (unsigned b : {W_E, W_G, W_H})
-        if (egh & b) passes.push_back(b);
-    for (unsigned pw : passes) {
-      if (launch_elem(m, material, pw, mu, lam, u, nullptr, nullptr, nullptr, grad, hess_diag, acc, 0, s)) return -1;
-      if (launch_finish(m, pw, grad, hess_diag, (pw & W_E) ? energy : nullptr, nullptr, acc, nullptr, s)) return -1;
+        if (egh & b) passes[np++] = b;
+    for (int i = 0; i < np; ++i) {
+      const unsigned pw = passes[i];
+      if (launch_elem(m, material, pw, mu, lam, u, nullptr, nullptr, nullptr, grad, hess_diag, acc, 0, s, 0, nullptr,
+                      ws))
+        return -1;
+      if (launch_finish(m, pw, grad, hess_diag, (pw & W_E) ? energy : nullptr, nullptr, acc, nullptr, s, ws))
+        return -1;
     }
   }
   if (w & W_HP) {
-    if (launch_elem(m, material, W_HP, mu, lam, u, p, nullptr, nullptr, hess_prod, nullptr, acc, 0, s)) return -1;
-    if (launch_finish(m, W_HP, hess_prod, nullptr, nullptr, nullptr, acc, nullptr, s)) return -1;
+    if (launch_elem(m, material, W_HP, mu, lam, u, p, nullptr, nullptr, hess_prod, nullptr, acc, 0, s, 0, nullptr, ws))
+      return -1;
+    if (launch_finish(m, W_HP, hess_prod, nullptr, nullptr, nullptr, acc, nullptr, s, ws)) return -1;
   }
   if (w & W_HQ) {
-    if (launch_elem(m, material, W_HQ, mu, lam, u, p, nullptr, nullptr, nullptr, nullptr, acc, 0, s)) return -1;
-    if (launch_finish(m, W_HQ, nullptr, nullptr, nullptr, hess_quad, acc, nullptr, s)) return -1;
+    if (launch_elem(m, material, W_HQ, mu, lam, u, p, nullptr, nullptr, nullptr, nullptr, acc, 0, s, 0, nullptr, ws))
+      return -1;
+    if (launch_finish(m, W_HQ, nullptr, nullptr, nullptr, hess_quad, acc, nullptr, s, ws)) return -1;
   }
   if (w & W_MIX) {
-    if (launch_elem(m, material, W_MIX, mu, lam, u, p, nullptr, nullptr, dmu, dlam, acc, 0, s)) return -1;
+    if (launch_elem(m, material, W_MIX, mu, lam, u, p, nullptr, nullptr, dmu, dlam, acc, 0, s, 0, nullptr, ws))
+      return -1;
   }
   return 0;
+}
+
+}  // namespace apl
+
+extern "C" {
+
+int apl_elem_eval(apl_mesh* m, int material, const void* mu, const void* lam, const void* u, const void* p,
+                  unsigned what, double* energy, void* grad, void* hess_diag, void* hess_prod, double* hess_quad,
+                  void* dmu, void* dlam, void* stream) {
+  return elem_eval_impl(m, material, mu, lam, u, p, what, energy, grad, hess_diag, hess_prod, hess_quad, dmu, dlam,
+                        (hipStream_t)stream, nullptr);
 }
 
 int apl_mesh_set_activation(apl_mesh* mesh, const void* activation) {

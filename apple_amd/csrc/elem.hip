@@ -33,6 +33,9 @@
 #include "persist.h"
 #include "vsum.h"
 
+#include <algorithm>
+#include <mutex>
+#include <string>
 #include <type_traits>
 
 namespace apl {
@@ -649,7 +652,7 @@ __global__ __launch_bounds__(NTH) void finish_kernel(FinishArgs<R> A) {
 template <class R, class X>
 static ElemArgs<R, X> make_args(apl_mesh* m, const void* mu, const void* lam, const void* x, const void* p,
                                 const double* alpha, const int32_t* active, void* out_a, void* out_b,
-                                int accumulate, void* out_c) {
+                                int accumulate, void* out_c, const Workspace* ws) {
   ElemArgs<R, X> A;
   A.dminv = (const R*)m->d.dminv;
   A.dV = (const R*)m->d.dV;
@@ -671,8 +674,8 @@ static ElemArgs<R, X> make_args(apl_mesh* m, const void* mu, const void* lam, co
   A.out_a = (R*)out_a;
   A.out_b = (R*)out_b;
   A.out_c = (R*)out_c;
-  A.partial = (R*)m->d.partial;
-  A.blk_red = m->d.blk_red;
+  A.partial = (R*)(ws != nullptr ? ws->partial : m->d.partial);
+  A.blk_red = ws != nullptr ? ws->blk_red : m->d.blk_red;
   A.T = (int)m->L.T;
   A.nblocks = (int)m->L.nblocks;
   A.nloc_max = m->L.nloc_max;
@@ -726,7 +729,8 @@ static int grid_for(apl_mesh* m, bool filtered) {
 template <int MAT, unsigned WHAT, class R, class X>
 static int launch_one(apl_mesh* m, const void* mu, const void* lam, const void* x, const void* p,
                       const double* alpha, const int32_t* active, void* out_a, void* out_b,
-                      int accumulate, int grid_override, hipStream_t s, void* out_c = nullptr) {
+                      int accumulate, int grid_override, hipStream_t s, void* out_c = nullptr,
+                      const Workspace* ws = nullptr) {
   if (m->L.nblocks == 0) return 0;
   const bool filtered = active != nullptr;
   const size_t smem = smem_bytes<MAT, WHAT, R, X>(m, filtered);
@@ -735,7 +739,7 @@ static int launch_one(apl_mesh* m, const void* mu, const void* lam, const void* 
     return -1;
   }
   const int grid = grid_override > 0 ? ((grid_override + 7) / 8) * 8 : grid_for<MAT, WHAT, R, X>(m, filtered);
-  ElemArgs<R, X> A = make_args<R, X>(m, mu, lam, x, p, alpha, active, out_a, out_b, accumulate, out_c);
+  ElemArgs<R, X> A = make_args<R, X>(m, mu, lam, x, p, alpha, active, out_a, out_b, accumulate, out_c, ws);
   constexpr bool SCATTER = (WHAT & (W_G | W_H | W_HP)) != 0;
   const int spt = SCATTER ? spt_for(m) : 1;
   if (spt == 1)
@@ -761,7 +765,8 @@ static int launch_one(apl_mesh* m, const void* mu, const void* lam, const void* 
 template <int MAT, class R>
 static int dispatch_what(apl_mesh* m, unsigned what, const void* mu, const void* lam, const void* x,
                          const void* p, const double* alpha, const int32_t* active, void* out_a,
-                         void* out_b, int accumulate, int grid_override, int x_f64, hipStream_t s, void* out_c) {
+                         void* out_b, int accumulate, int grid_override, int x_f64, hipStream_t s, void* out_c,
+                         const Workspace* ws) {
   if constexpr (MAT == MAT_MUSCLE) {
     // activation derivatives, alone or with the mu / lambda ones
     if (what == W_MIXA)
@@ -779,7 +784,7 @@ static int dispatch_what(apl_mesh* m, unsigned what, const void* mu, const void*
     return -1;
   }
 #define APL_CASE(W) \
-  if (what == (W)) return launch_one<MAT, (W), R, R>(m, mu, lam, x, p, alpha, active, out_a, out_b, accumulate, grid_override, s);
+  if (what == (W)) return launch_one<MAT, (W), R, R>(m, mu, lam, x, p, alpha, active, out_a, out_b, accumulate, grid_override, s, nullptr, ws);
   APL_WHAT_LIST(APL_CASE)
 #undef APL_CASE
   set_error("unsupported combination of outputs for one element pass");
@@ -788,12 +793,12 @@ static int dispatch_what(apl_mesh* m, unsigned what, const void* mu, const void*
 
 int launch_elem(apl_mesh* m, int material, unsigned what, const void* mu, const void* lam, const void* x,
                 const void* p, const double* alpha, const int32_t* active, void* out_a, void* out_b,
-                int accumulate, int grid_override, hipStream_t s, int x_f64, void* out_c) {
+                int accumulate, int grid_override, hipStream_t s, int x_f64, void* out_c, const Workspace* ws) {
   auto go = [&](auto mat_c) -> int {
     constexpr int MAT = decltype(mat_c)::value;
     if (m->dtype == DT_F32)
-      return dispatch_what<MAT, float>(m, what, mu, lam, x, p, alpha, active, out_a, out_b, accumulate, grid_override, x_f64, s, out_c);
-    return dispatch_what<MAT, double>(m, what, mu, lam, x, p, alpha, active, out_a, out_b, accumulate, grid_override, 0, s, out_c);
+      return dispatch_what<MAT, float>(m, what, mu, lam, x, p, alpha, active, out_a, out_b, accumulate, grid_override, x_f64, s, out_c, ws);
+    return dispatch_what<MAT, double>(m, what, mu, lam, x, p, alpha, active, out_a, out_b, accumulate, grid_override, 0, s, out_c, ws);
   };
   if (material == MAT_SNH) return go(std::integral_constant<int, MAT_SNH>{});
   if (material == MAT_ARAP) return go(std::integral_constant<int, MAT_ARAP>{});
@@ -830,7 +835,7 @@ int elem_grid(apl_mesh* m, int material, unsigned what, int x_f64, bool filtered
 }
 
 int launch_finish(apl_mesh* m, unsigned what, void* out_a, void* out_b, double* out_e, double* out_q,
-                  int accumulate, const int32_t* active, hipStream_t s) {
+                  int accumulate, const int32_t* active, hipStream_t s, const Workspace* ws) {
   const bool g = what & W_G, h = what & W_H, hp = what & W_HP;
   const int nch = (g ? 3 : 0) + (h ? 3 : 0) + (hp ? 3 : 0);
   const bool inst = out_e != nullptr || out_q != nullptr;
@@ -844,11 +849,11 @@ int launch_finish(apl_mesh* m, unsigned what, void* out_a, void* out_b, double* 
     A.vblk_off = m->d.vblk_off;
     A.vblk_inst = m->d.vblk_inst;
     A.vtx_slot_off = m->d.vtx_slot_off;
-    A.partial = (const R*)m->d.partial;
+    A.partial = (const R*)(ws != nullptr ? ws->partial : m->d.partial);
     A.active = active;
     A.out_a = (R*)out_a;
     A.out_b = (R*)out_b;
-    A.blk_red = m->d.blk_red;
+    A.blk_red = ws != nullptr ? ws->blk_red : m->d.blk_red;
     A.inst_blk_off = m->d.inst_blk_off;
     A.out_e = out_e;
     A.out_q = out_q;
@@ -872,3 +877,74 @@ int launch_finish(apl_mesh* m, unsigned what, void* out_a, void* out_b, double* 
 }
 
 }  // namespace apl
+
+// ------------------------------------------------------------ allocation-free entry point
+//
+// apl_elem_eval_static: apl_elem_eval with the element passes' scratch (slot
+// partials and block sums) in a file-scope device array instead of the
+// mesh's workspace.  Each GPU the library runs on holds its own copy of the
+// array until the process ends; its device address is looked up once per
+// device (hipGetSymbolAddress, with that device current) and only that
+// address is handed to kernels.  Every call on one GPU shares the array, so
+// calls that overlap on one GPU race.  Nothing in the array needs to start
+// at zero: a pass writes every partial and block sum it later reads.
+
+namespace apl {
+
+constexpr size_t STATIC_SCRATCH_BYTES = size_t(32) << 20;  // 32 MiB per GPU
+static __device__ __align__(256) unsigned char scratch[STATIC_SCRATCH_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[MAX_DEVICES] = {};
+
+// device address of this GPU's scratch array (m->device is current)
+static int static_scratch(int device, void** out) {
+  if (device < 0 || device >= MAX_DEVICES) {
+    set_error("device index out of range for the static scratch table");
+    return -1;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device] == nullptr) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+      set_error(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+      (void)hipGetLastError();
+      return -1;
+    }
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return 0;
+}
+
+}  // namespace apl
+
+extern "C" size_t apl_static_scratch_bytes(void) { return apl::STATIC_SCRATCH_BYTES; }
+
+extern "C" int apl_elem_eval_static(apl_mesh* m, int material, const void* mu, const void* lam, const void* u,
+                                    const void* p, unsigned what, double* energy, void* grad, void* hess_diag,
+                                    void* hess_prod, double* hess_quad, void* dmu, void* dlam, void* stream) {
+  using namespace apl;
+  if (m == nullptr) {
+    set_error("mesh is NULL");
+    return -1;
+  }
+  const size_t need = eval_scratch_bytes(m);
+  if (need > STATIC_SCRATCH_BYTES) {
+    set_error("mesh needs " + std::to_string(need) + " bytes of element-pass scratch; apl_elem_eval_static holds " +
+              std::to_string(STATIC_SCRATCH_BYTES) + " (use apl_elem_eval)");
+    return -1;
+  }
+  APL_CHECK_HIP(hipSetDevice(m->device));
+  (void)hipGetLastError();  // drop errors left by unrelated earlier HIP calls
+  void* base = nullptr;
+  if (static_scratch(m->device, &base)) return -1;
+  const size_t part = (size_t)MAX_CH * (size_t)std::max<int64_t>(m->L.n_slots, 1) * real_size(m);
+  Workspace ws;
+  ws.partial = base;
+  ws.blk_red = reinterpret_cast<double*>(static_cast<unsigned char*>(base) + ((part + 255) / 256) * 256);
+  return elem_eval_impl(m, material, mu, lam, u, p, what, energy, grad, hess_diag, hess_prod, hess_quad, dmu, dlam,
+                        (hipStream_t)stream, &ws);
+}

@@ -82,20 +82,36 @@ inline void drop_hip_error(hipError_t) { (void)hipGetLastError(); }
 
 inline size_t real_size(const apl_mesh* m) { return m->dtype == DT_F64 ? 8 : 4; }
 
+// Scratch of the element passes: the vertex-major slot partials and the
+// per-block energy / quadratic-form sums.  By default the mesh's own
+// workspace (allocated at creation); the allocation-free entry point passes
+// a static device array instead.
+struct Workspace {
+  void* partial = nullptr;
+  double* blk_red = nullptr;
+};
+// bytes of scratch one apl_elem_eval call may use on this mesh (partials of
+// up to MAX_CH channels, then the block sums at a 256-byte aligned offset)
+size_t eval_scratch_bytes(const apl_mesh* m);
+// apl_elem_eval against workspace ws (nullptr: the mesh's)
+int elem_eval_impl(apl_mesh* m, int material, const void* mu, const void* lam, const void* u, const void* p,
+                   unsigned what, double* energy, void* grad, void* hess_diag, void* hess_prod, double* hess_quad,
+                   void* dmu, void* dlam, hipStream_t s, const Workspace* ws);
+
 // element pass: per-slot partial sums into the partial workspace (scatter
 // passes), per-block energy/quadratic-form sums, or per-tet outputs (mixed);
 // x_f64: x holds FP64 displacements while the mesh is FP32 (solver passes)
 int launch_elem(apl_mesh* m, int material, unsigned what, const void* mu, const void* lam,
                 const void* x, const void* p, const double* alpha, const int32_t* active,
                 void* out_a, void* out_b, int accumulate, int grid_override, hipStream_t s, int x_f64 = 0,
-                void* out_c = nullptr);
+                void* out_c = nullptr, const Workspace* ws = nullptr);
 // vertex sums of the slot partials into the (V,3) outputs
 // Finish pass of an element pass in one launch: vertex sums of the slot
 // partials into out_a / out_b (what: the scatter channels of the pass) and,
 // when out_e / out_q are given, per-instance sums of the block energies /
 // quadratic forms.
 int launch_finish(apl_mesh* m, unsigned what, void* out_a, void* out_b, double* out_e, double* out_q,
-                  int accumulate, const int32_t* active, hipStream_t s);
+                  int accumulate, const int32_t* active, hipStream_t s, const Workspace* ws = nullptr);
 int elem_grid(apl_mesh* m, int material, unsigned what, int x_f64 = 0, bool filtered = false);
 // device setup (setup.hip): the element data (inverse rest shapes, dV *
 // fraction) of a created mesh from host cells / points / fraction, and the

@@ -15,6 +15,7 @@
 #ifndef APPLE_AMD_H
 #define APPLE_AMD_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -119,6 +120,19 @@ int apl_mesh_set_activation(apl_mesh* mesh, const void* activation);
 int apl_elem_eval(apl_mesh* mesh, int material, const void* mu, const void* lam, const void* u, const void* p,
                   unsigned what, double* energy, void* grad, void* hess_diag, void* hess_prod, double* hess_quad,
                   void* dmu, void* dlam, void* stream);
+
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  As apl_elem_eval, but the element passes' scratch
+ * (slot partials and block sums) lives in a static per-GPU device array of
+ * apl_static_scratch_bytes() (32 MiB) instead of the mesh's workspace; the
+ * call allocates nothing.  Meshes whose scratch exceeds the array are
+ * rejected with an error (64^3 x 5 tets fits in FP32 and FP64).  Every call
+ * on one GPU shares the array: calls that overlap on one GPU race. */
+int apl_elem_eval_static(apl_mesh* mesh, int material, const void* mu, const void* lam, const void* u,
+                         const void* p, unsigned what, double* energy, void* grad, void* hess_diag, void* hess_prod,
+                         double* hess_quad, void* dmu, void* dlam, void* stream);
+size_t apl_static_scratch_bytes(void);
 
 /* Mixed derivatives of the muscle law for the activation adjoint
  * (mixed_derivative_prod of exp/2026/01/28/smas/src/
