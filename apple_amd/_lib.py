@@ -15,7 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libapple_amd.so")
 
 APL_F32, APL_F64 = 0, 1
-APL_SNH, APL_ARAP, APL_SNH_MUSCLE = 0, 1, 2
+APL_SNH, APL_ARAP, APL_SNH_MUSCLE, APL_MODEL = 0, 1, 2, 3
 APL_ENERGY, APL_GRAD, APL_HESS_DIAG, APL_HESS_PROD, APL_HESS_QUAD, APL_MIXED = 1, 2, 4, 8, 16, 32
 APL_ACCUMULATE = 1 << 16
 
@@ -82,6 +82,23 @@ class PncgStats(ctypes.Structure):
     ]
 
 
+class PncgState(ctypes.Structure):
+    _fields_ = [
+        ("status", c_i32),
+        ("n_steps", c_i32),
+        ("n_evals", c_i32),
+        ("ls_steps", c_i32),
+        ("ok", c_i32),
+        ("alpha", ctypes.c_double),
+        ("f0", ctypes.c_double),
+        ("f_alpha", ctypes.c_double),
+        ("grad_norm", ctypes.c_double),
+        ("grad_norm_first", ctypes.c_double),
+        ("hess_diag_mean", ctypes.c_double),
+        ("beta", ctypes.c_double),
+    ]
+
+
 def _declare(lib):
     lib.apl_last_error.restype = ctypes.c_char_p
     lib.apl_version.restype = ctypes.c_int
@@ -92,6 +109,10 @@ def _declare(lib):
     lib.apl_mesh_create_from_points.argtypes = [
         ctypes.c_int, c_i64, c_i64, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, ctypes.c_int, ctypes.c_int,
         ctypes.POINTER(c_vp),
+    ]
+    lib.apl_model_create.argtypes = [
+        ctypes.c_int, c_i64, c_i64, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, ctypes.c_int, c_vp, c_vp, ctypes.c_int,
+        ctypes.c_int, ctypes.POINTER(c_vp),
     ]
     lib.apl_mesh_region.argtypes = [c_vp, c_vp, c_vp, c_vp]
     lib.apl_mesh_destroy.argtypes = [c_vp]
@@ -115,10 +136,14 @@ def _declare(lib):
         lib.apl_pncg_solve.argtypes = [
             c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.POINTER(PncgParams), ctypes.POINTER(PncgStats), c_vp,
         ]
+        lib.apl_pncg_begin.argtypes = [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.POINTER(PncgParams), c_vp]
+        lib.apl_pncg_advance.argtypes = [c_vp, c_i32, c_i64, c_vp, ctypes.POINTER(PncgState), c_vp]
+        lib.apl_pncg_end.argtypes = [c_vp, ctypes.POINTER(PncgStats), c_vp]
         lib.apl_cg_solve.argtypes = [
             c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.c_double, ctypes.c_int, ctypes.POINTER(PncgStats),
             c_vp,
         ]
+        lib.apl_cg_solve_normal.argtypes = lib.apl_cg_solve.argtypes
     return lib
 
 

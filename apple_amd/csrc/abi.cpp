@@ -40,7 +40,8 @@ static void free_mesh(apl_mesh* m) {
   void* ptrs[] = {d.dminv,        d.dV,           d.corner,      d.blk_tet_off,  d.blk_loc_off,
                   d.blk_inst,     d.inst_blk_off, d.loc_gid,     d.blk_co_off,   d.chunk_off,
                   d.vtx_slot_off, d.loc_dst,      d.vblk_off,    d.vblk_inst,    d.inst_vblk_off,
-                  d.inst_vtx_off, d.partial,      d.blk_red,     d.vblk_dots};
+                  d.inst_vtx_off, d.partial,      d.blk_red,     d.vblk_dots,    d.law_blk[0],
+                  d.law_blk[1],   d.law_blk[2]};
   for (void* p : ptrs)
     if (p) drop_hip_error(hipFree(p));
   delete m;
@@ -65,6 +66,20 @@ static unsigned to_internal(unsigned what) {
 
 using namespace apl;
 
+namespace apl {
+
+bool material_ok(const apl_mesh* m, int material) {
+  if (m->model) return material == APL_MODEL;
+  return material == APL_SNH || material == APL_ARAP || material == APL_SNH_MUSCLE;
+}
+
+bool needs_lambda(const apl_mesh* m, int material) {
+  if (material == APL_MODEL) return m->law_nblk[MAT_SNH] + m->law_nblk[MAT_MUSCLE] > 0;
+  return material != APL_ARAP;
+}
+
+}  // namespace apl
+
 extern "C" {
 
 const char* apl_last_error(void) { return g_err.c_str(); }
@@ -79,7 +94,8 @@ namespace apl {
 // and volumes): validation, block partition, layout upload, workspaces.
 static int create_common(int dtype, int64_t n_tets, int64_t n_verts, const int32_t* cells, int64_t n_inst,
                          const int64_t* inst_tet_off, const int64_t* inst_vtx_off, int chunks, int device,
-                         apl_mesh** out) {
+                         apl_mesh** out, int n_seg = 0, const int32_t* seg_law = nullptr,
+                         const int64_t* seg_tet_off = nullptr) {
   if (out == nullptr) {
     set_error("out is NULL");
     return -1;
@@ -92,6 +108,14 @@ static int create_common(int dtype, int64_t n_tets, int64_t n_verts, const int32
   // 32-bit byte offsets in the kernels: at most 2^27 tets and vertices
   if (n_tets < 0 || n_verts < 0 || n_tets >= (int64_t(1) << 27) || n_verts >= (int64_t(1) << 27)) {
     set_error("mesh too large or negative size (at most 2^27 tets and vertices)");
+    return -1;
+  }
+  // the element kernels address the SoA inverse rest shapes (9 rows of T)
+  // by a 32-bit byte offset from each block's start: 8 T + 8192 values of
+  // the mesh dtype must stay below 2^32 bytes (2^26 - 1024 FP64 tets)
+  const uint64_t rs_bytes = dtype == APL_F64 ? 8 : 4;
+  if ((8 * (uint64_t)n_tets + 8192) * rs_bytes >= (uint64_t(1) << 32)) {
+    set_error("mesh too large for 32-bit element offsets: at most 2^26 - 1024 FP64 tets (2^27 - 1024 FP32)");
     return -1;
   }
   if (n_tets > 0 && cells == nullptr) {
@@ -121,11 +145,11 @@ static int create_common(int dtype, int64_t n_tets, int64_t n_verts, const int32
   int ncu = 0;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
     m->n_cu = ncu;
-  // LDS budget (64 KiB per block): FP64 blocks hold at most 512 slots and one chunk
-  const int nloc_cap = dtype == APL_F64 ? 512 : 1024;
+  // LDS budget of the element workgroups: FP64 blocks hold at most 478 slots and one chunk
+  const int nloc_cap = nloc_limit(dtype);
   if (chunks < 1 || dtype == APL_F64) chunks = 1;
   std::string err = build_layout(cells, n_tets, n_verts, n_inst, inst_tet_off, inst_vtx_off, NTH, chunks, nloc_cap,
-                                 m->L);
+                                 m->L, seg_tet_off != nullptr ? n_inst * (int64_t)n_seg : 0, seg_tet_off);
   if (!err.empty()) {
     set_error(err);
     delete m;
@@ -134,6 +158,12 @@ static int create_common(int dtype, int64_t n_tets, int64_t n_verts, const int32
   Layout& L = m->L;
   DevLayout& d = m->d;
   const size_t rs = dtype == APL_F64 ? 8 : 4;
+  // the element kernels address the slot partials by 32-bit byte offsets
+  if ((uint64_t)L.n_slots * MAX_CH * rs >= (uint64_t(1) << 32)) {
+    set_error("too many block-local vertex slots for 32-bit partial offsets (reorder the mesh for locality)");
+    delete m;
+    return -1;
+  }
   int rc = 0;
   rc |= alloc_bytes(m, &d.dminv, 9 * (size_t)n_tets * rs);
   rc |= alloc_bytes(m, &d.dV, (size_t)n_tets * rs);
@@ -155,6 +185,17 @@ static int create_common(int dtype, int64_t n_tets, int64_t n_verts, const int32
   rc |= alloc_bytes(m, (void**)&d.blk_red, RED_STRIDE * (size_t)std::max<int64_t>(L.nblocks, 1) * sizeof(double));
   rc |= alloc_bytes(m, (void**)&d.vblk_dots,
                     (size_t)N_DOTS * std::max<size_t>(L.vblk_inst.size(), 1) * sizeof(double));
+  if (seg_tet_off != nullptr) {
+    // model mesh: the blocks of each law, ascending (segment s of every
+    // instance evaluates seg_law[s])
+    m->model = 1;
+    std::vector<int32_t> lists[N_MAT];
+    for (int64_t b = 0; b < L.nblocks; ++b) lists[seg_law[L.seg_of_blk[b] % n_seg]].push_back((int32_t)b);
+    for (int law = 0; law < N_MAT; ++law) {
+      m->law_nblk[law] = (int64_t)lists[law].size();
+      rc |= upload(m, &d.law_blk[law], lists[law].data(), lists[law].size());
+    }
+  }
   if (rc) {
     free_mesh(m);
     return -1;
@@ -170,18 +211,21 @@ static int create_common(int dtype, int64_t n_tets, int64_t n_verts, const int32
 
 }  // namespace apl
 
-extern "C" {
+namespace apl {
 
-int apl_mesh_create(int dtype, int64_t n_tets, int64_t n_verts, const int32_t* cells, const double* dminv,
-                    const double* dv, int64_t n_inst, const int64_t* inst_tet_off, const int64_t* inst_vtx_off,
-                    int chunks, int device, apl_mesh** out) {
+static int mesh_create_impl(int dtype, int64_t n_tets, int64_t n_verts, const int32_t* cells, const double* dminv,
+                            const double* dv, int64_t n_inst, const int64_t* inst_tet_off,
+                            const int64_t* inst_vtx_off, int chunks, int device, apl_mesh** out, int n_seg,
+                            const int32_t* seg_law, const int64_t* seg_tet_off) {
   if (n_tets > 0 && (dminv == nullptr || dv == nullptr)) {
     set_error("cells, dminv and dv must be non-NULL");
     if (out != nullptr) *out = nullptr;
     return -1;
   }
   apl_mesh* m = nullptr;
-  if (create_common(dtype, n_tets, n_verts, cells, n_inst, inst_tet_off, inst_vtx_off, chunks, device, &m)) return -1;
+  if (create_common(dtype, n_tets, n_verts, cells, n_inst, inst_tet_off, inst_vtx_off, chunks, device, &m, n_seg,
+                    seg_law, seg_tet_off))
+    return -1;
   // element data, SoA, in the mesh precision
   const size_t rs = dtype == APL_F64 ? 8 : 4;
   std::vector<char> buf(9 * (size_t)n_tets * rs);
@@ -211,6 +255,17 @@ int apl_mesh_create(int dtype, int64_t n_tets, int64_t n_verts, const int32_t* c
   return 0;
 }
 
+}  // namespace apl
+
+extern "C" {
+
+int apl_mesh_create(int dtype, int64_t n_tets, int64_t n_verts, const int32_t* cells, const double* dminv,
+                    const double* dv, int64_t n_inst, const int64_t* inst_tet_off, const int64_t* inst_vtx_off,
+                    int chunks, int device, apl_mesh** out) {
+  return mesh_create_impl(dtype, n_tets, n_verts, cells, dminv, dv, n_inst, inst_tet_off, inst_vtx_off, chunks,
+                          device, out, 0, nullptr, nullptr);
+}
+
 int apl_mesh_create_from_points(int dtype, int64_t n_tets, int64_t n_verts, const int32_t* cells,
                                 const double* points, const double* fraction, int64_t n_inst,
                                 const int64_t* inst_tet_off, const int64_t* inst_vtx_off, int chunks, int device,
@@ -228,6 +283,36 @@ int apl_mesh_create_from_points(int dtype, int64_t n_tets, int64_t n_verts, cons
   }
   *out = m;
   return 0;
+}
+
+int apl_model_create(int dtype, int64_t n_tets, int64_t n_verts, const int32_t* cells, const double* dminv,
+                     const double* dv, int64_t n_inst, const int64_t* inst_tet_off, const int64_t* inst_vtx_off,
+                     int n_seg, const int32_t* seg_law, const int64_t* seg_tet_off, int chunks, int device,
+                     apl_mesh** out) {
+  if (out != nullptr) *out = nullptr;
+  if (n_seg < 1 || n_seg > 1024 || seg_law == nullptr || seg_tet_off == nullptr) {
+    set_error("a model needs 1..1024 segments with their laws and tet offsets");
+    return -1;
+  }
+  for (int k = 0; k < n_seg; ++k)
+    if (seg_law[k] != APL_SNH && seg_law[k] != APL_ARAP && seg_law[k] != APL_SNH_MUSCLE) {
+      set_error("segment laws must be APL_SNH, APL_ARAP or APL_SNH_MUSCLE");
+      return -1;
+    }
+  const int64_t ni = n_inst <= 1 || inst_tet_off == nullptr ? 1 : n_inst;
+  int64_t one_t[2] = {0, n_tets};
+  const int64_t* ito = ni == 1 ? one_t : inst_tet_off;
+  for (int64_t i = 0; i < ni; ++i)
+    if (seg_tet_off[i * n_seg] != ito[i]) {
+      set_error("segment 0 of every instance must start at the instance's first tet");
+      return -1;
+    }
+  if (seg_tet_off[ni * n_seg] != n_tets) {
+    set_error("segment offsets must end at n_tets");
+    return -1;
+  }
+  return mesh_create_impl(dtype, n_tets, n_verts, cells, dminv, dv, n_inst, inst_tet_off, inst_vtx_off, chunks,
+                          device, out, n_seg, seg_law, seg_tet_off);
 }
 
 int apl_mesh_region(apl_mesh* mesh, void* dminv, void* dv, void* stream) {
@@ -298,14 +383,15 @@ int elem_eval_impl(apl_mesh* m, int material, const void* mu, const void* lam, c
     set_error("mesh is NULL");
     return -1;
   }
-  if (material != APL_SNH && material != APL_ARAP && material != APL_SNH_MUSCLE) {
-    set_error("material must be APL_SNH, APL_ARAP or APL_SNH_MUSCLE");
+  if (!material_ok(m, material)) {
+    set_error(m->model ? "a model mesh takes material APL_MODEL"
+                       : "material must be APL_SNH, APL_ARAP or APL_SNH_MUSCLE");
     return -1;
   }
   const unsigned w = to_internal(what);
   const int acc = (what & APL_ACCUMULATE) ? 1 : 0;
   if (w == 0) return 0;
-  if (m->L.T > 0 && (mu == nullptr || u == nullptr || (material != APL_ARAP && lam == nullptr))) {
+  if (m->L.T > 0 && (mu == nullptr || u == nullptr || (needs_lambda(m, material) && lam == nullptr))) {
     set_error("mu, lam (SNH, SNH_MUSCLE) and u must be non-NULL");
     return -1;
   }
@@ -316,7 +402,7 @@ int elem_eval_impl(apl_mesh* m, int material, const void* mu, const void* lam, c
   const bool has_v = m->L.V > 0, has_t = m->L.T > 0;
   if (((w & W_E) && !energy) || ((w & W_G) && has_v && !grad) || ((w & W_H) && has_v && !hess_diag) ||
       ((w & W_HP) && has_v && !hess_prod) || ((w & W_HQ) && !hess_quad) ||
-      ((w & W_MIX) && has_t && (!dmu || (material != APL_ARAP && !dlam)))) {
+      ((w & W_MIX) && has_t && (!dmu || (needs_lambda(m, material) && !dlam)))) {
     set_error("a requested output pointer is NULL");
     return -1;
   }
@@ -393,8 +479,9 @@ int apl_elem_mixed_activation(apl_mesh* m, const void* mu, const void* lam, cons
   APL_CHECK_HIP(hipSetDevice(m->device));
   (void)hipGetLastError();  // drop errors left by unrelated earlier HIP calls
   const unsigned w = dmu != nullptr ? (W_MIX | W_MIXA) : W_MIXA;
-  return launch_elem(m, MAT_MUSCLE, w, mu, lam, u, p, nullptr, nullptr, dmu, dlam, accumulate ? 1 : 0, 0,
-                     (hipStream_t)stream, 0, dact);
+  // a model mesh: dact of its muscle-law tets, dmu / dlam of every tet
+  return launch_elem(m, m->model ? MAT_MODEL : MAT_MUSCLE, w, mu, lam, u, p, nullptr, nullptr, dmu, dlam,
+                     accumulate ? 1 : 0, 0, (hipStream_t)stream, 0, dact);
 }
 
 int apl_bench_elem_pass(apl_mesh* m, int material, unsigned what, const void* mu, const void* lam, const void* u,

@@ -41,6 +41,7 @@
 namespace apl {
 
 constexpr int XCD_CHUNK = 16;  // consecutive blocks per XCD chunk
+constexpr size_t ELEM_LDS_MAX = 65536;  // LDS one element workgroup may use
 
 template <class R, class X>
 struct ElemArgs {
@@ -66,8 +67,9 @@ struct ElemArgs {
   R* out_c;  // (T, 6) activation derivatives of the mixed pass
   R* partial;  // vertex-major partial sums (scatter passes)
   double* blk_red;
+  const int32_t* blk_list;  // model meshes: this law's blocks (else NULL: every block)
   int T;
-  int nblocks;
+  int nblocks;  // blocks this launch visits (the list's length with a list)
   int nloc_max;
   int chunks;
   int accumulate;
@@ -158,7 +160,8 @@ struct TetRec {
 };
 
 // Resources of one block's tet data.  dminv is SoA (row kk at kk * T), so its
-// base moves with the block and the row offset stays below 2^32 bytes.
+// base moves with the block and the row offset kk * T * sizeof(R) stays
+// below 2^32 bytes (mesh creation limits T accordingly).
 struct BlockRes {
   rsrc_t dm, dv, mu, lam, act, cr;
   unsigned row, t0b;  // row stride and block start, in bytes of R
@@ -255,7 +258,10 @@ __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R, X> A) {
   const int my_chunks = xcd < nchunk ? (nchunk - 1 - xcd) / 8 + 1 : 0;
   const bool owns_last = my_chunks > 0 && (nchunk - 1) % 8 == xcd;
   const int nb_x = my_chunks * C - (owns_last ? nchunk * C - nb : 0);
-  auto xcd_block = [&](int j) { return ((j / C) * 8 + xcd) * C + j % C; };
+  auto xcd_block = [&](int j) {
+    const int k = ((j / C) * 8 + xcd) * C + j % C;
+    return A.blk_list != nullptr ? A.blk_list[k] : k;
+  };
 
   // This workgroup's blocks are xcd_block(k0 + j K), j < n_cand; in a solve
   // only those of still-active instances.
@@ -677,6 +683,7 @@ static ElemArgs<R, X> make_args(apl_mesh* m, const void* mu, const void* lam, co
   A.partial = (R*)(ws != nullptr ? ws->partial : m->d.partial);
   A.blk_red = ws != nullptr ? ws->blk_red : m->d.blk_red;
   A.T = (int)m->L.T;
+  A.blk_list = nullptr;
   A.nblocks = (int)m->L.nblocks;
   A.nloc_max = m->L.nloc_max;
   A.chunks = m->L.chunks;
@@ -684,15 +691,48 @@ static ElemArgs<R, X> make_args(apl_mesh* m, const void* mu, const void* lam, co
   return A;
 }
 
-template <int MAT, unsigned WHAT, class R, class X>
-static size_t smem_bytes(const apl_mesh* m, bool filtered) {
+// LDS of one element workgroup: wave partials, the vertex window (u in X,
+// v in R when the pass needs it, 4 lanes per slot), the packed contribution
+// buffer of the scatter passes and, in solves, the active-block list.
+template <unsigned WHAT, class R, class X>
+constexpr size_t smem_bytes_n(int nloc_max, bool filtered) {
   constexpr bool NEEDV = (WHAT & (W_HP | W_HQ | W_MIX | W_MIXA)) != 0;
   constexpr int NCH = ((WHAT & W_G) ? 3 : 0) + ((WHAT & W_H) ? 3 : 0) + ((WHAT & W_HP) ? 3 : 0);
-  size_t n = 8 * sizeof(double) + 4 * (size_t)m->L.nloc_max * (sizeof(X) + (NEEDV ? sizeof(R) : 0));
+  size_t n = 8 * sizeof(double) + 4 * (size_t)nloc_max * (sizeof(X) + (NEEDV ? sizeof(R) : 0));
   n += sizeof(R) * (size_t)(NCH > 0 ? Pack<NCH>::NCHP : 0) * 4 * NTH;
   if (filtered) n += sizeof(int) * NTH;  // active-block list
   return n;
 }
+
+template <int MAT, unsigned WHAT, class R, class X>
+static size_t smem_bytes(const apl_mesh* m, bool filtered) {
+  return smem_bytes_n<WHAT, R, X>(m->L.nloc_max, filtered);
+}
+
+// The largest block window (distinct vertices per block) every pass variant
+// of a mesh of this dtype can stage within ELEM_LDS_MAX, filtered launches
+// included: 1024 for FP32 meshes (also with the solver's FP64 displacements),
+// 478 for FP64 meshes (pass A: 6 FP64 channels x 1024 positions + 32 B per slot).
+template <class R, class X>
+static bool fits(int n) {
+  return smem_bytes_n<W_E | W_G | W_H, R, X>(n, true) <= ELEM_LDS_MAX &&
+         smem_bytes_n<W_HP, R, R>(n, true) <= ELEM_LDS_MAX && smem_bytes_n<W_HQ, R, X>(n, true) <= ELEM_LDS_MAX &&
+         smem_bytes_n<W_MIX | W_MIXA, R, R>(n, true) <= ELEM_LDS_MAX && smem_bytes_n<W_G, R, R>(n, true) <= ELEM_LDS_MAX &&
+         smem_bytes_n<W_H, R, R>(n, true) <= ELEM_LDS_MAX;
+}
+
+int nloc_limit(int dtype) {
+  int n = 1024;
+  if (dtype == DT_F64) {
+    while (n > 4 && !fits<double, double>(n)) --n;
+  } else {
+    while (n > 4 && !(fits<float, float>(n) && fits<float, double>(n))) --n;
+  }
+  return n;
+}
+
+// blocks one launch of law MAT visits (model meshes: the law's own blocks)
+inline int64_t launch_blocks(const apl_mesh* m, int mat) { return m->model ? m->law_nblk[mat] : m->L.nblocks; }
 
 template <int MAT, unsigned WHAT, class R, class X, int SPT>
 static int grid_for_spt(apl_mesh* m, bool filtered) {
@@ -705,7 +745,7 @@ static int grid_for_spt(apl_mesh* m, bool filtered) {
       occ < 1)
     occ = 1;
   long long g = (long long)occ * m->n_cu * m->grid_pct / 100;
-  long long nb8 = ((m->L.nblocks + 7) / 8) * 8;
+  long long nb8 = ((launch_blocks(m, MAT) + 7) / 8) * 8;
   if (g > nb8) g = nb8;
   g = ((g + 7) / 8) * 8;
   if (g < 8) g = 8;
@@ -731,15 +771,19 @@ static int launch_one(apl_mesh* m, const void* mu, const void* lam, const void* 
                       const double* alpha, const int32_t* active, void* out_a, void* out_b,
                       int accumulate, int grid_override, hipStream_t s, void* out_c = nullptr,
                       const Workspace* ws = nullptr) {
-  if (m->L.nblocks == 0) return 0;
+  if (launch_blocks(m, MAT) == 0) return 0;
   const bool filtered = active != nullptr;
   const size_t smem = smem_bytes<MAT, WHAT, R, X>(m, filtered);
-  if (smem > 65536) {
-    set_error("element block needs more than 64 KiB of LDS; rebuild the mesh with a smaller nloc_cap");
+  if (smem > ELEM_LDS_MAX) {
+    set_error("element block needs more than 64 KiB of LDS (block window above the mesh's nloc cap)");
     return -1;
   }
   const int grid = grid_override > 0 ? ((grid_override + 7) / 8) * 8 : grid_for<MAT, WHAT, R, X>(m, filtered);
   ElemArgs<R, X> A = make_args<R, X>(m, mu, lam, x, p, alpha, active, out_a, out_b, accumulate, out_c, ws);
+  if (m->model) {
+    A.blk_list = m->d.law_blk[MAT];
+    A.nblocks = (int)m->law_nblk[MAT];
+  }
   constexpr bool SCATTER = (WHAT & (W_G | W_H | W_HP)) != 0;
   const int spt = SCATTER ? spt_for(m) : 1;
   if (spt == 1)
@@ -800,10 +844,28 @@ int launch_elem(apl_mesh* m, int material, unsigned what, const void* mu, const 
       return dispatch_what<MAT, float>(m, what, mu, lam, x, p, alpha, active, out_a, out_b, accumulate, grid_override, x_f64, s, out_c, ws);
     return dispatch_what<MAT, double>(m, what, mu, lam, x, p, alpha, active, out_a, out_b, accumulate, grid_override, 0, s, out_c, ws);
   };
+  if (material == MAT_MODEL) {
+    // a model mesh: every law present, each over its own blocks (the muscle
+    // law's activation derivatives only exist for that law)
+    if (!m->model) {
+      set_error("APL_MODEL needs a mesh created by apl_model_create");
+      return -1;
+    }
+    for (int law = 0; law < N_MAT; ++law) {
+      if (m->law_nblk[law] == 0) continue;
+      unsigned w = what;
+      if (law != MAT_MUSCLE) w &= ~W_MIXA;
+      if (w == 0) continue;
+      if (launch_elem(m, law, w, mu, lam, x, p, alpha, active, out_a, out_b, accumulate, grid_override, s, x_f64,
+                      out_c, ws))
+        return -1;
+    }
+    return 0;
+  }
   if (material == MAT_SNH) return go(std::integral_constant<int, MAT_SNH>{});
   if (material == MAT_ARAP) return go(std::integral_constant<int, MAT_ARAP>{});
   if (material == MAT_MUSCLE) {
-    if (m->activation == nullptr && m->L.T > 0) {
+    if (m->activation == nullptr && launch_blocks(m, MAT_MUSCLE) > 0) {
       set_error("the muscle law needs the activation: call apl_mesh_set_activation first");
       return -1;
     }
@@ -829,6 +891,12 @@ static int grid_mat(apl_mesh* m, unsigned what, int x_f64, bool filtered) {
 }
 
 int elem_grid(apl_mesh* m, int material, unsigned what, int x_f64, bool filtered) {
+  if (material == MAT_MODEL) {
+    int g = 0;
+    for (int law = 0; law < N_MAT; ++law)
+      if (m->law_nblk[law] > 0) g = std::max(g, elem_grid(m, law, what, x_f64, filtered));
+    return g;
+  }
   if (material == MAT_MUSCLE) return grid_mat<MAT_MUSCLE>(m, what, x_f64, filtered);
   return material == MAT_SNH ? grid_mat<MAT_SNH>(m, what, x_f64, filtered)
                              : grid_mat<MAT_ARAP>(m, what, x_f64, filtered);

@@ -8,6 +8,7 @@
 #include <string>
 
 #include "layout.h"
+#include "tetmath.h"
 
 namespace apl {
 
@@ -15,6 +16,8 @@ constexpr int NTH = 256;       // threads per element / vector block
 constexpr int MAX_CH = 6;      // scatter channels per incidence (grad + hdiag)
 constexpr int N_DOTS = 16;     // per vertex-block partial dot products
 constexpr int RED_STRIDE = 2;  // per element block: energy, quadratic form
+// material id of a model mesh's launches: every law over its own blocks
+constexpr int MAT_MODEL = 3;
 
 enum Dtype : int { DT_F32 = 0, DT_F64 = 1 };
 
@@ -47,6 +50,7 @@ struct DevLayout {
   int32_t* vblk_inst = nullptr;
   int64_t* inst_vblk_off = nullptr;
   int64_t* inst_vtx_off = nullptr;
+  int32_t* law_blk[N_MAT] = {};  // model meshes: the blocks of each material law, ascending
   // workspace
   void* partial = nullptr;   // n_slots x MAX_CH, vertex-major (one partial sum per local slot)
   double* blk_red = nullptr; // RED_STRIDE * nblocks
@@ -66,6 +70,10 @@ struct apl_mesh {
   int refs = 1;                 // the caller's handle plus one per solver built on the mesh
   int grid_pct = 100;           // persistent element grid, percent of occupancy x CUs (autotuned)
   const void* activation = nullptr;  // caller-owned (T, 6) muscle activation, read at every muscle launch
+  // model meshes (apl_model_create): several potentials' tets over one
+  // vertex set, in segments of one material law each
+  int model = 0;
+  int64_t law_nblk[3] = {0, 0, 0};  // blocks per law (MAT_SNH, MAT_ARAP, MAT_MUSCLE)
 };
 
 namespace apl {
@@ -112,6 +120,13 @@ int launch_elem(apl_mesh* m, int material, unsigned what, const void* mu, const 
 // quadratic forms.
 int launch_finish(apl_mesh* m, unsigned what, void* out_a, void* out_b, double* out_e, double* out_q,
                   int accumulate, const int32_t* active, hipStream_t s, const Workspace* ws = nullptr);
+// material id valid for this mesh (APL_MODEL exactly for model meshes)
+bool material_ok(const apl_mesh* m, int material);
+// whether launches of this material read lambda
+bool needs_lambda(const apl_mesh* m, int material);
+// largest block window (distinct vertices per block) whose LDS every element
+// pass variant of a mesh of this dtype fits
+int nloc_limit(int dtype);
 int elem_grid(apl_mesh* m, int material, unsigned what, int x_f64 = 0, bool filtered = false);
 // device setup (setup.hip): the element data (inverse rest shapes, dV *
 // fraction) of a created mesh from host cells / points / fraction, and the

@@ -8,7 +8,7 @@ namespace apl {
 
 std::string build_layout(const int32_t* cells, int64_t T, int64_t V, int64_t n_inst,
                          const int64_t* inst_tet_off, const int64_t* inst_vtx_off, int nth,
-                         int chunks, int nloc_cap, Layout& L) {
+                         int chunks, int nloc_cap, Layout& L, int64_t n_seg_total, const int64_t* seg_off) {
   char msg[256];
   if (T < 0 || V < 0 || n_inst < 1) return "invalid mesh sizes";
   if (nth != 256) return "threads per block must be 256";
@@ -32,6 +32,17 @@ std::string build_layout(const int32_t* cells, int64_t T, int64_t V, int64_t n_i
         }
       }
   }
+  if (seg_off != nullptr) {
+    if (n_seg_total < n_inst || seg_off[0] != 0 || seg_off[n_seg_total] != T) return "segment offsets must span [0, T]";
+    for (int64_t k = 0; k < n_seg_total; ++k)
+      if (seg_off[k + 1] < seg_off[k]) return "segment offsets must be non-decreasing";
+    // every instance boundary must be a segment boundary
+    int64_t k = 0;
+    for (int64_t i = 0; i <= n_inst; ++i) {
+      while (k < n_seg_total && seg_off[k] < inst_tet_off[i]) ++k;
+      if (seg_off[k] != inst_tet_off[i]) return "instance boundaries must be segment boundaries";
+    }
+  }
   L = Layout();
   L.nth = nth;
   L.chunks = chunks;
@@ -45,12 +56,20 @@ std::string build_layout(const int32_t* cells, int64_t T, int64_t V, int64_t n_i
   std::vector<int64_t> stamp(V, -1);
   L.blk_tet_off.push_back(0);
   L.inst_blk_off.push_back(0);
+  int64_t seg = 0;  // segment of tet t (segments are consecutive)
   for (int64_t i = 0; i < n_inst; ++i) {
     int64_t t = inst_tet_off[i], t1 = inst_tet_off[i + 1];
     while (t < t1) {
       int64_t b = (int64_t)L.blk_inst.size();
       int nloc = 0, nt = 0;
-      while (t < t1 && nt < max_tets) {
+      // this block ends at the instance's or the segment's end
+      int64_t tend = t1;
+      if (seg_off != nullptr) {
+        while (seg_off[seg + 1] <= t) ++seg;
+        tend = std::min<int64_t>(tend, seg_off[seg + 1]);
+      }
+      L.seg_of_blk.push_back(seg_off != nullptr ? (int32_t)seg : 0);
+      while (t < tend && nt < max_tets) {
         int add = 0;
         for (int c = 0; c < 4; ++c) {
           int64_t v = cells[t * 4 + c];

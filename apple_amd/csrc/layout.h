@@ -47,11 +47,18 @@ struct Layout {
   std::vector<int32_t> vblk_off;      // nvblk+1
   std::vector<int32_t> vblk_inst;     // nvblk
   std::vector<int64_t> inst_vblk_off; // n_inst+1
+  // segments (a model of several potentials): blocks never straddle a
+  // segment; seg_of_blk[b] is the segment of block b (0 without segments)
+  std::vector<int32_t> seg_of_blk;    // nblocks
 };
 
-// Build the partition.  Returns an empty string on success, else an error.
+// Build the partition.  seg_off (n_seg_total + 1 tet offsets, or NULL): the
+// tets are split into consecutive segments that blocks must not straddle
+// (instance boundaries must be segment boundaries).  Returns an empty string
+// on success, else an error.
 std::string build_layout(const int32_t* cells, int64_t T, int64_t V, int64_t n_inst,
                          const int64_t* inst_tet_off, const int64_t* inst_vtx_off, int nth,
-                         int chunks, int nloc_cap, Layout& L);
+                         int chunks, int nloc_cap, Layout& L, int64_t n_seg_total = 0,
+                         const int64_t* seg_off = nullptr);
 
 }  // namespace apl

@@ -47,11 +47,11 @@ static_assert(D_E < N_DOTS, "dot table too small");
 static_assert(D_PP < N_DOTS, "dot slots");
 
 struct StatePtrs {
-  double *f, *gd, *alpha, *alpha_commit, *beta, *gnorm0, *gnorm, *pmean, *f_first, *pHp, *gbest, *f_mark;
+  double *f, *gd, *alpha, *alpha_commit, *beta, *gnorm0, *gnorm, *pmean, *f_first, *pHp, *gbest, *f_mark, *f0;
   int32_t *status, *act_eval, *act_dir, *commit, *n_steps, *n_evals, *ls_step, *n_backtracks, *phase, *restarts,
-      *best_step;
+      *best_step, *ls_last;
 };
-constexpr int N_SD = 12, N_SI = 11;
+constexpr int N_SD = 13, N_SI = 12;
 
 // One instance's state in registers.  The scalar kernels load it whole (all
 // loads independent, issued together and overlapped with the table sums),
@@ -59,8 +59,9 @@ constexpr int N_SD = 12, N_SI = 11;
 // far as the compiler knows, so reading and writing them field by field
 // chains one memory round trip per access.
 struct InstState {
-  double f, gd, alpha, alpha_commit, beta, gnorm0, gnorm, pmean, f_first, pHp, gbest, f_mark;
-  int32_t status, act_eval, act_dir, commit, n_steps, n_evals, ls_step, n_backtracks, phase, restarts, best_step;
+  double f, gd, alpha, alpha_commit, beta, gnorm0, gnorm, pmean, f_first, pHp, gbest, f_mark, f0;
+  int32_t status, act_eval, act_dir, commit, n_steps, n_evals, ls_step, n_backtracks, phase, restarts, best_step,
+      ls_last;
 };
 
 __device__ inline InstState load_state(const StatePtrs& st, int i) {
@@ -72,6 +73,7 @@ __device__ inline InstState load_state(const StatePtrs& st, int i) {
   s.n_steps = st.n_steps[i], s.n_evals = st.n_evals[i], s.ls_step = st.ls_step[i];
   s.n_backtracks = st.n_backtracks[i], s.phase = st.phase[i], s.restarts = st.restarts[i];
   s.best_step = st.best_step[i];
+  s.f0 = st.f0[i], s.ls_last = st.ls_last[i];
   return s;
 }
 
@@ -83,6 +85,7 @@ __device__ inline void store_state(const StatePtrs& st, int i, const InstState& 
   st.n_steps[i] = s.n_steps, st.n_evals[i] = s.n_evals, st.ls_step[i] = s.ls_step;
   st.n_backtracks[i] = s.n_backtracks, st.phase[i] = s.phase, st.restarts[i] = s.restarts;
   st.best_step[i] = s.best_step;
+  st.f0[i] = s.f0, st.ls_last[i] = s.ls_last;
 }
 
 struct Params {
@@ -197,6 +200,8 @@ __global__ void pncg_init_kernel(StatePtrs st, int n) {
   st.gbest[i] = INFINITY;
   st.f_mark[i] = INFINITY;
   st.best_step[i] = 0;
+  st.f0[i] = 0.0;
+  st.ls_last[i] = 0;
   st.status[i] = APL_RUNNING;
   st.act_eval[i] = 1;
   st.act_dir[i] = 0;
@@ -291,20 +296,8 @@ __global__ __launch_bounds__(NTH) void pncg_reduce_kernel(VecArgs<R> A) {
                 [&](int vb) { pncg_reduce_block<R>(A, vb, sm, win); });
 }
 
-// S: per-instance decision
-__global__ __launch_bounds__(SCAL_TH) void pncg_scalar_kernel(StatePtrs st, Params P, const double* vblk_dots,
-                                                          const int64_t* inst_vblk_off) {
-  __shared__ double sm[(SCAL_TH / 64 + 1) * N_DOTS];
-  const int i = blockIdx.x;
-  // the active flag, the table bounds and the state are independent: all are
-  // loaded before the first wait (an inactive instance sums an empty table)
-  const bool act = st.act_eval[i] != 0;
-  const long long vb0 = inst_vblk_off[i], vb1 = inst_vblk_off[i + 1];
-  InstState s;
-  if (threadIdx.x == 0) s = load_state(st, i);
-  double tot[N_DOTS];
-  flat_column_sums<N_DOTS, SCAL_TH>(vblk_dots + vb0 * N_DOTS, act ? (vb1 - vb0) * N_DOTS : 0, sm, tot);
-  if (threadIdx.x != 0 || !act) return;
+// S: per-instance decision from the instance's dot totals (one thread)
+__device__ inline void pncg_scalar_step(InstState& s, const Params& P, const double (&tot)[N_DOTS]) {
   const double fE = tot[D_E];
   const double ft = fE + tot[D_EXT];
   const double gn = sqrt(tot[D_G2]);
@@ -323,6 +316,8 @@ __global__ __launch_bounds__(SCAL_TH) void pncg_scalar_kernel(StatePtrs st, Para
   }
   if (accept) {
     s.alpha_commit = phase == 1 ? s.alpha : 0.0;
+    s.f0 = phase == 0 ? ft : s.f;  // the line search's start and end energies
+    s.ls_last = s.ls_step;
     s.f = ft;
     s.gnorm = gn;
     s.commit = 1;
@@ -408,6 +403,23 @@ __global__ __launch_bounds__(SCAL_TH) void pncg_scalar_kernel(StatePtrs st, Para
       s.alpha = a;
     }
   }
+}
+
+// S: per-instance decision, one workgroup per instance
+__global__ __launch_bounds__(SCAL_TH) void pncg_scalar_kernel(StatePtrs st, Params P, const double* vblk_dots,
+                                                          const int64_t* inst_vblk_off) {
+  __shared__ double sm[(SCAL_TH / 64 + 1) * N_DOTS];
+  const int i = blockIdx.x;
+  // the active flag, the table bounds and the state are independent: all are
+  // loaded before the first wait (an inactive instance sums an empty table)
+  const bool act = st.act_eval[i] != 0;
+  const long long vb0 = inst_vblk_off[i], vb1 = inst_vblk_off[i + 1];
+  InstState s;
+  if (threadIdx.x == 0) s = load_state(st, i);
+  double tot[N_DOTS];
+  flat_column_sums<N_DOTS, SCAL_TH>(vblk_dots + vb0 * N_DOTS, act ? (vb1 - vb0) * N_DOTS : 0, sm, tot);
+  if (threadIdx.x != 0 || !act) return;
+  pncg_scalar_step(s, P, tot);
   store_state(st, i, s);
 }
 
@@ -494,19 +506,25 @@ struct CgArgs {
   R* d;
   R* q;
   R* hd;
+  R* t;     // normal equations: H d (or the masked right-hand side while it is formed)
+  R* tout;  // CG_T / CG_MASKCOPY output
   double* vblk_dots;
   StatePtrs st;
   int nvblk;   // vertex blocks of the mesh
   int filter;  // visit only blocks of active instances (batches)
+  int normal;  // CG on the normal equations H^2 y = H b, Jacobi with diag(H)^2
 };
 
-enum CgStage : int { CG_DIAG = 0, CG_PRECOND = 1, CG_Q = 2, CG_UPDATE = 3, CG_DIR = 4 };
+// CG_T: tout = H v summed from the partials, fixed DOFs 0 (no dots);
+// CG_MASKCOPY: tout = rhs with fixed DOFs 0
+enum CgStage : int { CG_DIAG = 0, CG_PRECOND = 1, CG_Q = 2, CG_UPDATE = 3, CG_DIR = 4, CG_T = 5, CG_MASKCOPY = 6 };
 
 template <class R, int STAGE>
 __device__ inline void cg_vec_block(const CgArgs<R>& A, const int vb, double* sm, R* win) {
-  constexpr bool SUMS = STAGE == CG_DIAG || STAGE == CG_Q;
+  constexpr bool SUMS = STAGE == CG_DIAG || STAGE == CG_Q || STAGE == CG_T;
+  constexpr bool PRE = STAGE == CG_DIAG || STAGE == CG_PRECOND || STAGE == CG_MASKCOPY;  // before the status
   const int inst = A.vblk_inst[vb];
-  if (STAGE != CG_DIAG && STAGE != CG_PRECOND && A.st.act_eval[inst] == 0) return;
+  if (!PRE && A.st.act_eval[inst] == 0) return;
   const int v0 = A.vblk_off[vb], v1 = A.vblk_off[vb + 1];
   const int i0 = 3 * v0, i1 = 3 * v1;
   double dd[N_DOTS];
@@ -549,11 +567,15 @@ __device__ inline void cg_vec_block(const CgArgs<R>& A, const int vb, double* sm
             dd[D_HCNT] += 1.0;
           }
         }
+      } else if constexpr (STAGE == CG_T) {
+        A.tout[i] = fixed ? R(0) : s1[0];
       } else {
         const R qv = fixed ? R(0) : s1[0];
         A.q[i] = qv;
         dd[D_GTP] += (double)A.d[i] * (double)qv;
       }
+    } else if constexpr (STAGE == CG_MASKCOPY) {
+      A.tout[i] = fixed ? R(0) : A.rhs[i];
     } else if constexpr (STAGE == CG_PRECOND || STAGE == CG_UPDATE) {
       const R a = (R)A.st.alpha[inst];
       const R pm = (R)A.st.pmean[inst];
@@ -564,7 +586,9 @@ __device__ inline void cg_vec_block(const CgArgs<R>& A, const int vb, double* sm
         A.r[i] = rv;
       }
       const R h = A.hd[i];
-      const R zv = fixed ? R(0) : (h > R(0) ? rv / h : rv / pm);
+      // Jacobi: 1/diag(H), or 1/diag(H)^2 for the normal equations (diag(H^2) >= diag(H)^2)
+      const R hh = h > R(0) ? h : pm;
+      const R zv = fixed ? R(0) : (A.normal ? rv / (hh * hh) : rv / hh);
       A.z[i] = zv;
       if constexpr (STAGE == CG_PRECOND) A.d[i] = zv;
       dd[D_GPG] += (double)rv * (double)zv;
@@ -573,32 +597,25 @@ __device__ inline void cg_vec_block(const CgArgs<R>& A, const int vb, double* sm
       A.d[i] = A.z[i] + (R)A.st.beta[inst] * A.d[i];
     }
   }
-  if constexpr (STAGE != CG_DIR) block_dots(dd, sm, A.vblk_dots + (long long)vb * N_DOTS);
+  if constexpr (STAGE != CG_DIR && STAGE != CG_T && STAGE != CG_MASKCOPY)
+    block_dots(dd, sm, A.vblk_dots + (long long)vb * N_DOTS);
 }
 
 template <class R, int STAGE>
 __global__ __launch_bounds__(NTH) void cg_vec_kernel(CgArgs<R> A) {
   __shared__ double sm[(NTH / 64) * N_DOTS];
-  constexpr bool SUMS = STAGE == CG_DIAG || STAGE == CG_Q;
+  constexpr bool SUMS = STAGE == CG_DIAG || STAGE == CG_Q || STAGE == CG_T;
   __shared__ R win[SUMS ? VSUM_LDS_BYTES / sizeof(R) : 1];
   __shared__ int list[NTH];
-  constexpr bool ALL = STAGE == CG_DIAG || STAGE == CG_PRECOND;  // run before the status is known
+  constexpr bool ALL = STAGE == CG_DIAG || STAGE == CG_PRECOND || STAGE == CG_MASKCOPY;  // before the status
   vertex_blocks(A.nvblk, A.filter && !ALL, list, [&](int vb) { return A.st.act_eval[A.vblk_inst[vb]] != 0; },
                 [&](int vb) { cg_vec_block<R, STAGE>(A, vb, sm, win); });
 }
 
-// per-instance CG scalars; stage 0: after DIAG, 1: after PRECOND, 2: after Q, 3: after UPDATE
-__global__ __launch_bounds__(SCAL_TH) void cg_scalar_kernel(StatePtrs st, int stage, double rtol, int max_iters,
-                                                        const double* vblk_dots, const int64_t* inst_vblk_off) {
-  __shared__ double sm[(SCAL_TH / 64 + 1) * N_DOTS];
-  const int i = blockIdx.x;
-  const bool act = stage < 2 || st.act_eval[i] != 0;  // loaded with the rest, as in pncg_scalar_kernel
-  const long long vb0 = inst_vblk_off[i], vb1 = inst_vblk_off[i + 1];
-  InstState s;
-  if (threadIdx.x == 0) s = load_state(st, i);
-  double tot[N_DOTS];
-  flat_column_sums<N_DOTS, SCAL_TH>(vblk_dots + vb0 * N_DOTS, act ? (vb1 - vb0) * N_DOTS : 0, sm, tot);
-  if (threadIdx.x != 0 || !act) return;
+// per-instance CG scalars (one thread); stage 0: after DIAG, 1: after
+// PRECOND, 2: after Q, 3: after UPDATE
+__device__ inline void cg_scalar_step(InstState& s, int stage, double rtol, int max_iters,
+                                      const double (&tot)[N_DOTS]) {
   if (stage == 0) {
     s.pmean = tot[D_HCNT] > 0.0 ? tot[D_HSUM] / tot[D_HCNT] : 1.0;
     const double bn = sqrt(tot[D_G2]);
@@ -642,6 +659,21 @@ __global__ __launch_bounds__(SCAL_TH) void cg_scalar_kernel(StatePtrs st, int st
       s.gd = rz;
     }
   }
+}
+
+// the CG's scalar stages, one workgroup per instance
+__global__ __launch_bounds__(SCAL_TH) void cg_scalar_kernel(StatePtrs st, int stage, double rtol, int max_iters,
+                                                        const double* vblk_dots, const int64_t* inst_vblk_off) {
+  __shared__ double sm[(SCAL_TH / 64 + 1) * N_DOTS];
+  const int i = blockIdx.x;
+  const bool act = stage < 2 || st.act_eval[i] != 0;  // loaded with the rest, as in pncg_scalar_kernel
+  const long long vb0 = inst_vblk_off[i], vb1 = inst_vblk_off[i + 1];
+  InstState s;
+  if (threadIdx.x == 0) s = load_state(st, i);
+  double tot[N_DOTS];
+  flat_column_sums<N_DOTS, SCAL_TH>(vblk_dots + vb0 * N_DOTS, act ? (vb1 - vb0) * N_DOTS : 0, sm, tot);
+  if (threadIdx.x != 0 || !act) return;
+  cg_scalar_step(s, stage, rtol, max_iters, tot);
   store_state(st, i, s);
 }
 
@@ -662,7 +694,12 @@ struct apl_solver {
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
   std::vector<char> gkey;
+  hipGraphExec_t cg_gexec = nullptr;  // K PCG iterations
+  hipGraph_t cg_graph = nullptr;
+  std::vector<char> cg_gkey;
   int32_t* host_status = nullptr;
+  int32_t* host_steps = nullptr;
+  struct apl_solver_run* run = nullptr;  // the step-wise solve in progress
 };
 
 namespace apl {
@@ -751,10 +788,44 @@ static int fill(R* a, long long n, R v, hipStream_t st) {
   return 0;
 }
 
+// A solve in progress: what apl_pncg_begin bound, for the following
+// advance / end calls.
+struct PncgRun {
+  bool open = false;
+  const void* mu = nullptr;
+  const void* lam = nullptr;
+  void* x = nullptr;  // the caller's iterate (mesh dtype)
+  const void* fext = nullptr;
+  const uint8_t* mask = nullptr;
+  double* xs = nullptr;  // the FP64 iterate (the caller's x for FP64 meshes)
+  Params P{};
+  int K = 16, grid = 0;
+  bool use_graph = true;
+  long long done = 0, guard = 0;
+};
+
 template <class R>
-static int pncg_solve_t(apl_solver* s, const void* mu, const void* lam, void* x, const void* fext,
-                        const uint8_t* mask, const apl_pncg_params* prm, apl_pncg_stats* stats, hipStream_t st) {
+static int pncg_begin_t(apl_solver* s, const void* mu, const void* lam, void* x, const void* fext,
+                        const uint8_t* mask, const apl_pncg_params* prm, hipStream_t st);
+template <class R>
+static int pncg_advance_t(apl_solver* s, long long max_iters, int steps, hipStream_t st);
+template <class R>
+static int pncg_end_t(apl_solver* s, apl_pncg_stats* stats, hipStream_t st);
+
+}  // namespace apl
+
+struct apl_solver_run {
+  apl::PncgRun r;
+};
+
+namespace apl {
+
+template <class R>
+static int pncg_begin_t(apl_solver* s, const void* mu, const void* lam, void* x, const void* fext,
+                        const uint8_t* mask, const apl_pncg_params* prm, hipStream_t st) {
   apl_mesh* m = s->mesh;
+  PncgRun& run = s->run->r;
+  run.open = false;
   const int n = (int)m->L.n_inst;
   const Params P = to_params(prm);
   const int K = prm->iters_per_launch > 0 ? prm->iters_per_launch : 16;
@@ -815,25 +886,78 @@ static int pncg_solve_t(apl_solver* s, const void* mu, const void* lam, void* x,
       s->gkey = key;
     }
   }
-  const long long guard = ((long long)prm->max_steps + 2) * ((long long)prm->max_backtracking + 2) * 2 + 8;
-  long long done = 0;
+  run.mu = mu;
+  run.lam = lam;
+  run.x = x;
+  run.fext = fext;
+  run.mask = mask;
+  run.xs = xs;
+  run.P = P;
+  run.K = K;
+  run.grid = grid;
+  run.use_graph = use_graph;
+  run.done = 0;
+  run.guard = ((long long)prm->max_steps + 2) * ((long long)prm->max_backtracking + 2) * 2 + 8;
+  run.open = true;
+  return 0;
+}
+
+// Run device iterations (trial evaluations) until no instance is running,
+// the guard is reached, max_iters more have run (max_iters > 0) or, when
+// steps > 0, every running instance has taken that many more accepted steps
+// (one trial per launch, with a status read after each).
+template <class R>
+static int pncg_advance_t(apl_solver* s, long long max_iters, int steps, hipStream_t st) {
+  apl_mesh* m = s->mesh;
+  PncgRun& run = s->run->r;
+  const int n = (int)m->L.n_inst;
+  std::vector<int32_t> target;
+  if (steps > 0) {
+    APL_CHECK_HIP(hipMemcpyAsync(s->host_steps, s->st.n_steps, n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    APL_CHECK_HIP(hipStreamSynchronize(st));
+    target.assign(s->host_steps, s->host_steps + n);
+    for (int32_t& t : target) t += steps;
+  }
+  long long ran = 0;
   while (true) {
-    if (use_graph) {
+    // one trial per launch when counting steps; eager launches for a
+    // remainder shorter than the captured graph
+    const long long left = max_iters > 0 ? max_iters - ran : run.K;
+    const bool graph = run.use_graph && steps <= 0 && left >= run.K;
+    const int K = steps > 0 ? 1 : (graph ? run.K : (int)std::min<long long>(run.K, left));
+    if (graph) {
       APL_CHECK_HIP(hipGraphLaunch(s->gexec, st));
     } else {
       for (int it = 0; it < K; ++it)
-        if (pncg_iteration<R>(s, mu, lam, xs, fext, mask, P, grid, st)) return -1;
+        if (pncg_iteration<R>(s, run.mu, run.lam, run.xs, run.fext, run.mask, run.P, run.grid, st)) return -1;
     }
-    done += K;
+    run.done += K;
+    ran += K;
     APL_CHECK_HIP(hipMemcpyAsync(s->host_status, s->st.status, n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    if (steps > 0)
+      APL_CHECK_HIP(hipMemcpyAsync(s->host_steps, s->st.n_steps, n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
     APL_CHECK_HIP(hipStreamSynchronize(st));
-    bool running = false;
-    for (int i = 0; i < n; ++i) running = running || s->host_status[i] == APL_RUNNING;
-    if (!running || done > guard) break;
+    bool running = false, pending = false;
+    for (int i = 0; i < n; ++i) {
+      const bool r = s->host_status[i] == APL_RUNNING;
+      running = running || r;
+      if (steps > 0 && r && s->host_steps[i] < target[i]) pending = true;
+    }
+    if (!running || run.done > run.guard || (max_iters > 0 && ran >= max_iters) || (steps > 0 && !pending)) break;
   }
-  if (x_f64 && nx > 0) {
+  return 0;
+}
+
+// Copy the FP64 iterate into the caller's x (FP32 meshes) and read the stats.
+template <class R>
+static int pncg_end_t(apl_solver* s, apl_pncg_stats* stats, hipStream_t st) {
+  apl_mesh* m = s->mesh;
+  PncgRun& run = s->run->r;
+  const int n = (int)m->L.n_inst;
+  const long long nx = 3 * (long long)m->L.V;
+  if (sizeof(R) == 4 && nx > 0) {
     hipLaunchKernelGGL((convert_kernel<double, R>), dim3((unsigned)std::min<long long>((nx + NTH - 1) / NTH, 4096)),
-                       dim3(NTH), 0, st, (const double*)xs, (R*)x, nx);
+                       dim3(NTH), 0, st, (const double*)run.xs, (R*)run.x, nx);
     APL_CHECK_HIP(hipGetLastError());
   }
   if (stats != nullptr) {
@@ -860,6 +984,15 @@ static int pncg_solve_t(apl_solver* s, const void* mu, const void* lam, void* x,
   return 0;
 }
 
+template <class R>
+static int pncg_solve_t(apl_solver* s, const void* mu, const void* lam, void* x, const void* fext,
+                        const uint8_t* mask, const apl_pncg_params* prm, apl_pncg_stats* stats, hipStream_t st) {
+  if (pncg_begin_t<R>(s, mu, lam, x, fext, mask, prm, st)) return -1;
+  if (pncg_advance_t<R>(s, 0, 0, st)) return -1;
+  const int rc = pncg_end_t<R>(s, stats, st);
+  s->run->r.open = false;
+  return rc;
+}
 
 template <class R>
 static CgArgs<R> cg_args(apl_solver* s, const void* rhs, void* y, const uint8_t* mask) {
@@ -879,10 +1012,13 @@ static CgArgs<R> cg_args(apl_solver* s, const void* rhs, void* y, const uint8_t*
   A.d = base + 6 * nv;
   A.q = base + 7 * nv;
   A.hd = base + 8 * nv;
+  A.t = base;  // the PNCG's direction buffer is free during a CG solve
+  A.tout = base;
   A.vblk_dots = m->d.vblk_dots;
   A.st = s->st;
   A.nvblk = (int)s->mesh->L.vblk_inst.size();
   A.filter = s->mesh->L.n_inst > 1 ? 1 : 0;
+  A.normal = 0;
   return A;
 }
 
@@ -892,7 +1028,19 @@ static int cg_iteration(apl_solver* s, const void* mu, const void* lam, const vo
   apl_mesh* m = s->mesh;
   const unsigned nvb = (unsigned)m->L.vblk_inst.size();
   const unsigned ni = (unsigned)m->L.n_inst;
-  if (launch_elem(m, s->material, W_HP, mu, lam, x, A.d, nullptr, s->st.act_eval, A.q, nullptr, 0, 0, st)) return -1;
+  if (A.normal) {
+    // q = H (H d): t = H d with fixed DOFs zeroed, then the pass on t
+    if (launch_elem(m, s->material, W_HP, mu, lam, x, A.d, nullptr, s->st.act_eval, nullptr, nullptr, 0, 0, st))
+      return -1;
+    CgArgs<R> At = A;
+    At.tout = A.t;
+    if (nvb) hipLaunchKernelGGL((cg_vec_kernel<R, CG_T>), dim3(vec_grid(m)), dim3(NTH), 0, st, At);
+    if (launch_elem(m, s->material, W_HP, mu, lam, x, A.t, nullptr, s->st.act_eval, nullptr, nullptr, 0, 0, st))
+      return -1;
+  } else {
+    if (launch_elem(m, s->material, W_HP, mu, lam, x, A.d, nullptr, s->st.act_eval, nullptr, nullptr, 0, 0, st))
+      return -1;
+  }
   if (nvb) hipLaunchKernelGGL((cg_vec_kernel<R, CG_Q>), dim3(vec_grid(m)), dim3(NTH), 0, st, A);
   hipLaunchKernelGGL(cg_scalar_kernel, dim3(ni), dim3(SCAL_TH), 0, st, s->st, 2, rtol, max_iters, m->d.vblk_dots,
                      m->d.inst_vblk_off);
@@ -906,12 +1054,26 @@ static int cg_iteration(apl_solver* s, const void* mu, const void* lam, const vo
 
 template <class R>
 static int cg_solve_t(apl_solver* s, const void* mu, const void* lam, const void* x, const void* rhs, void* y,
-                      const uint8_t* mask, double rtol, int max_iters, apl_pncg_stats* stats, hipStream_t st) {
+                      const uint8_t* mask, double rtol, int max_iters, apl_pncg_stats* stats, hipStream_t st,
+                      int normal) {
   apl_mesh* m = s->mesh;
   const int n = (int)m->L.n_inst;
   const unsigned nvb = (unsigned)m->L.vblk_inst.size();
   CgArgs<R> A = cg_args<R>(s, rhs, y, mask);
   hipLaunchKernelGGL(pncg_init_kernel, dim3((n + 255) / 256), dim3(256), 0, st, s->st, n);
+  if (normal) {
+    // the normal equations' right-hand side H b (b with fixed DOFs zeroed),
+    // kept in the PNCG's g_cur buffer; the CG then runs on it
+    A.normal = 1;
+    R* b2 = (R*)s->vecs + 3 * (size_t)m->L.V;
+    CgArgs<R> Ab = A;
+    Ab.tout = A.t;
+    if (nvb) hipLaunchKernelGGL((cg_vec_kernel<R, CG_MASKCOPY>), dim3(vec_grid(m)), dim3(NTH), 0, st, Ab);
+    if (launch_elem(m, s->material, W_HP, mu, lam, x, A.t, nullptr, nullptr, nullptr, nullptr, 0, 0, st)) return -1;
+    Ab.tout = b2;
+    if (nvb) hipLaunchKernelGGL((cg_vec_kernel<R, CG_T>), dim3(vec_grid(m)), dim3(NTH), 0, st, Ab);
+    A.rhs = b2;
+  }
   // zero the diagonal buffer (vertices with no tet keep 0) and run the Hdiag pass at x
   if (fill<R>(A.hd, 3 * (long long)m->L.V, R(0), st)) return -1;
   if (launch_elem(m, s->material, W_H, mu, lam, x, nullptr, nullptr, nullptr, nullptr, A.hd, 0, 0, st)) return -1;
@@ -924,10 +1086,48 @@ static int cg_solve_t(apl_solver* s, const void* mu, const void* lam, const void
   APL_CHECK_HIP(hipGetLastError());
   (void)elem_grid(m, s->material, W_HP, 0, true);
   const int K = 16;
+  // K iterations captured once per set of pointers and parameters
+  {
+    std::vector<char> key(sizeof(void*) * 7 + sizeof(double) + 3 * sizeof(int));
+    char* k = key.data();
+    const void* ptrs[7] = {mu, lam, x, rhs, y, mask, m->activation};
+    std::memcpy(k, ptrs, sizeof ptrs);
+    k += sizeof ptrs;
+    std::memcpy(k, &rtol, sizeof rtol);
+    k += sizeof rtol;
+    std::memcpy(k, &max_iters, sizeof max_iters);
+    k += sizeof max_iters;
+    std::memcpy(k, &normal, sizeof normal);
+    k += sizeof normal;
+    std::memcpy(k, &m->grid_pct, sizeof m->grid_pct);
+    if (s->cg_gexec == nullptr || key != s->cg_gkey) {
+      if (s->cg_gexec) {
+        drop_hip_error(hipGraphExecDestroy(s->cg_gexec));
+        s->cg_gexec = nullptr;
+      }
+      if (s->cg_graph) {
+        drop_hip_error(hipGraphDestroy(s->cg_graph));
+        s->cg_graph = nullptr;
+      }
+      if (s->cap == nullptr) APL_CHECK_HIP(hipStreamCreateWithFlags(&s->cap, hipStreamNonBlocking));
+      APL_CHECK_HIP(hipStreamBeginCapture(s->cap, hipStreamCaptureModeThreadLocal));
+      int rc = 0;
+      for (int it = 0; it < K && rc == 0; ++it) rc = cg_iteration<R>(s, mu, lam, x, A, rtol, max_iters, s->cap);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(s->cap, &g);
+      if (rc != 0 || e != hipSuccess) {
+        if (g) drop_hip_error(hipGraphDestroy(g));
+        if (rc == 0) set_error(std::string("graph capture failed: ") + hipGetErrorString(e));
+        return -1;
+      }
+      s->cg_graph = g;
+      APL_CHECK_HIP(hipGraphInstantiate(&s->cg_gexec, g, nullptr, nullptr, 0));
+      s->cg_gkey = key;
+    }
+  }
   long long done = 0;
   while (true) {
-    for (int it = 0; it < K; ++it)
-      if (cg_iteration<R>(s, mu, lam, x, A, rtol, max_iters, st)) return -1;
+    APL_CHECK_HIP(hipGraphLaunch(s->cg_gexec, st));
     done += K;
     APL_CHECK_HIP(hipMemcpyAsync(s->host_status, s->st.status, n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
     APL_CHECK_HIP(hipStreamSynchronize(st));
@@ -967,14 +1167,16 @@ int apl_solver_create(apl_mesh* m, int material, apl_solver** out) {
     set_error("NULL argument");
     return -1;
   }
-  if (material != APL_SNH && material != APL_ARAP && material != APL_SNH_MUSCLE) {
-    set_error("material must be APL_SNH, APL_ARAP or APL_SNH_MUSCLE");
+  if (!material_ok(m, material)) {
+    set_error(m->model ? "a model mesh takes material APL_MODEL"
+                       : "material must be APL_SNH, APL_ARAP or APL_SNH_MUSCLE");
     return -1;
   }
   *out = nullptr;
   APL_CHECK_HIP(hipSetDevice(m->device));
   (void)hipGetLastError();  // drop errors left by unrelated earlier HIP calls
   apl_solver* s = new apl_solver();
+  s->run = new apl_solver_run();
   s->mesh = m;
   ++m->refs;
   s->device = m->device;
@@ -985,7 +1187,8 @@ int apl_solver_create(apl_mesh* m, int material, apl_solver** out) {
   if (hipMalloc(&s->vecs, vb) != hipSuccess || (xb && hipMalloc(&s->x64, xb) != hipSuccess) ||
       hipMalloc((void**)&s->sd, N_SD * (size_t)n * sizeof(double)) != hipSuccess ||
       hipMalloc((void**)&s->si, N_SI * (size_t)n * sizeof(int32_t)) != hipSuccess ||
-      hipHostMalloc((void**)&s->host_status, (size_t)n * sizeof(int32_t), 0) != hipSuccess) {
+      hipHostMalloc((void**)&s->host_status, (size_t)n * sizeof(int32_t), 0) != hipSuccess ||
+      hipHostMalloc((void**)&s->host_steps, (size_t)n * sizeof(int32_t), 0) != hipSuccess) {
     set_error("out of device memory creating the solver");
     apl_solver_destroy(s);
     return -1;
@@ -1008,6 +1211,7 @@ int apl_solver_create(apl_mesh* m, int material, apl_solver** out) {
   s->st.pHp = d + 9 * n;
   s->st.gbest = d + 10 * n;
   s->st.f_mark = d + 11 * n;
+  s->st.f0 = d + 12 * n;
   int32_t* ii = s->si;
   s->st.status = ii + 0 * n;
   s->st.act_eval = ii + 1 * n;
@@ -1020,6 +1224,7 @@ int apl_solver_create(apl_mesh* m, int material, apl_solver** out) {
   s->st.phase = ii + 8 * n;
   s->st.restarts = ii + 9 * n;
   s->st.best_step = ii + 10 * n;
+  s->st.ls_last = ii + 11 * n;
   *out = s;
   return 0;
 }
@@ -1029,12 +1234,16 @@ int apl_solver_destroy(apl_solver* s) {
   drop_hip_error(hipSetDevice(s->device));
   if (s->gexec) drop_hip_error(hipGraphExecDestroy(s->gexec));
   if (s->graph) drop_hip_error(hipGraphDestroy(s->graph));
+  if (s->cg_gexec) drop_hip_error(hipGraphExecDestroy(s->cg_gexec));
+  if (s->cg_graph) drop_hip_error(hipGraphDestroy(s->cg_graph));
   if (s->cap) drop_hip_error(hipStreamDestroy(s->cap));
   if (s->vecs) drop_hip_error(hipFree(s->vecs));
   if (s->x64) drop_hip_error(hipFree(s->x64));
   if (s->sd) drop_hip_error(hipFree(s->sd));
   if (s->si) drop_hip_error(hipFree(s->si));
   if (s->host_status) drop_hip_error(hipHostFree(s->host_status));
+  if (s->host_steps) drop_hip_error(hipHostFree(s->host_steps));
+  delete s->run;
   if (s->mesh) mesh_release(s->mesh);
   delete s;
   return 0;
@@ -1043,7 +1252,7 @@ int apl_solver_destroy(apl_solver* s) {
 int apl_pncg_solve(apl_solver* s, const void* mu, const void* lam, void* x, const void* f_ext,
                    const uint8_t* fixed_mask, const apl_pncg_params* params, apl_pncg_stats* stats, void* stream) {
   if (s == nullptr || params == nullptr || (s->mesh->L.V > 0 && x == nullptr) ||
-      (s->mesh->L.T > 0 && (mu == nullptr || (s->material != APL_ARAP && lam == nullptr)))) {
+      (s->mesh->L.T > 0 && (mu == nullptr || (needs_lambda(s->mesh, s->material) && lam == nullptr)))) {
     set_error("NULL argument");
     return -1;
   }
@@ -1058,10 +1267,89 @@ int apl_pncg_solve(apl_solver* s, const void* mu, const void* lam, void* x, cons
   return pncg_solve_t<double>(s, mu, lam, x, f_ext, fixed_mask, params, stats, (hipStream_t)stream);
 }
 
+int apl_pncg_begin(apl_solver* s, const void* mu, const void* lam, void* x, const void* f_ext,
+                   const uint8_t* fixed_mask, const apl_pncg_params* params, void* stream) {
+  if (s == nullptr || params == nullptr || (s->mesh->L.V > 0 && x == nullptr) ||
+      (s->mesh->L.T > 0 && (mu == nullptr || (needs_lambda(s->mesh, s->material) && lam == nullptr)))) {
+    set_error("NULL argument");
+    return -1;
+  }
+  if (params->max_steps < 0 || params->max_backtracking < 0 || !(params->rtol >= 0) || !(params->atol >= 0)) {
+    set_error("invalid PNCG parameters");
+    return -1;
+  }
+  APL_CHECK_HIP(hipSetDevice(s->mesh->device));
+  (void)hipGetLastError();  // drop errors left by unrelated earlier HIP calls
+  if (s->mesh->dtype == DT_F32) return pncg_begin_t<float>(s, mu, lam, x, f_ext, fixed_mask, params, (hipStream_t)stream);
+  return pncg_begin_t<double>(s, mu, lam, x, f_ext, fixed_mask, params, (hipStream_t)stream);
+}
+
+int apl_pncg_advance(apl_solver* s, int32_t steps, int64_t max_iters, void* x_out, apl_pncg_state* state,
+                     void* stream) {
+  if (s == nullptr || !s->run->r.open) {
+    set_error("no PNCG run in progress: call apl_pncg_begin first");
+    return -1;
+  }
+  APL_CHECK_HIP(hipSetDevice(s->mesh->device));
+  (void)hipGetLastError();  // drop errors left by unrelated earlier HIP calls
+  hipStream_t st = (hipStream_t)stream;
+  apl_mesh* m = s->mesh;
+  const bool f32 = m->dtype == DT_F32;
+  if ((f32 ? pncg_advance_t<float>(s, max_iters, steps, st) : pncg_advance_t<double>(s, max_iters, steps, st)))
+    return -1;
+  const long long nx = 3 * (long long)m->L.V;
+  if (x_out != nullptr && nx > 0) {
+    if (f32) {
+      hipLaunchKernelGGL((convert_kernel<double, float>), dim3((unsigned)std::min<long long>((nx + NTH - 1) / NTH, 4096)),
+                         dim3(NTH), 0, st, (const double*)s->run->r.xs, (float*)x_out, nx);
+      APL_CHECK_HIP(hipGetLastError());
+    } else if (x_out != s->run->r.xs) {
+      APL_CHECK_HIP(hipMemcpyAsync(x_out, s->run->r.xs, nx * sizeof(double), hipMemcpyDeviceToDevice, st));
+    }
+  }
+  if (state != nullptr) {
+    const int n = (int)m->L.n_inst;
+    std::vector<double> hd((size_t)N_SD * n);
+    std::vector<int32_t> hi((size_t)N_SI * n);
+    APL_CHECK_HIP(hipMemcpyAsync(hd.data(), s->sd, hd.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    APL_CHECK_HIP(hipMemcpyAsync(hi.data(), s->si, hi.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    APL_CHECK_HIP(hipStreamSynchronize(st));
+    for (int i = 0; i < n; ++i) {
+      apl_pncg_state& o = state[i];
+      o.status = hi[0 * n + i];
+      o.n_steps = hi[4 * n + i];
+      o.n_evals = hi[5 * n + i];
+      o.ls_steps = hi[11 * n + i];
+      o.ok = hi[9 * n + i] == 0 ? 1 : 0;
+      o.alpha = hd[3 * n + i];
+      o.f0 = hd[12 * n + i];
+      o.f_alpha = hd[0 * n + i];
+      o.grad_norm = hd[6 * n + i];
+      o.grad_norm_first = hd[5 * n + i];
+      o.hess_diag_mean = hd[7 * n + i];
+      o.beta = hd[4 * n + i];
+    }
+  }
+  return 0;
+}
+
+int apl_pncg_end(apl_solver* s, apl_pncg_stats* stats, void* stream) {
+  if (s == nullptr || !s->run->r.open) {
+    set_error("no PNCG run in progress: call apl_pncg_begin first");
+    return -1;
+  }
+  APL_CHECK_HIP(hipSetDevice(s->mesh->device));
+  (void)hipGetLastError();  // drop errors left by unrelated earlier HIP calls
+  hipStream_t st = (hipStream_t)stream;
+  const int rc = s->mesh->dtype == DT_F32 ? pncg_end_t<float>(s, stats, st) : pncg_end_t<double>(s, stats, st);
+  s->run->r.open = false;
+  return rc;
+}
+
 int apl_cg_solve(apl_solver* s, const void* mu, const void* lam, const void* x, const void* rhs, void* y,
                  const uint8_t* fixed_mask, double rtol, int max_iters, apl_pncg_stats* stats, void* stream) {
   if (s == nullptr || (s->mesh->L.V > 0 && (x == nullptr || rhs == nullptr || y == nullptr)) ||
-      (s->mesh->L.T > 0 && (mu == nullptr || (s->material != APL_ARAP && lam == nullptr)))) {
+      (s->mesh->L.T > 0 && (mu == nullptr || (needs_lambda(s->mesh, s->material) && lam == nullptr)))) {
     set_error("NULL argument");
     return -1;
   }
@@ -1072,8 +1360,26 @@ int apl_cg_solve(apl_solver* s, const void* mu, const void* lam, const void* x, 
   APL_CHECK_HIP(hipSetDevice(s->mesh->device));
   (void)hipGetLastError();  // drop errors left by unrelated earlier HIP calls
   if (s->mesh->dtype == DT_F32)
-    return cg_solve_t<float>(s, mu, lam, x, rhs, y, fixed_mask, rtol, max_iters, stats, (hipStream_t)stream);
-  return cg_solve_t<double>(s, mu, lam, x, rhs, y, fixed_mask, rtol, max_iters, stats, (hipStream_t)stream);
+    return cg_solve_t<float>(s, mu, lam, x, rhs, y, fixed_mask, rtol, max_iters, stats, (hipStream_t)stream, 0);
+  return cg_solve_t<double>(s, mu, lam, x, rhs, y, fixed_mask, rtol, max_iters, stats, (hipStream_t)stream, 0);
+}
+
+int apl_cg_solve_normal(apl_solver* s, const void* mu, const void* lam, const void* x, const void* rhs, void* y,
+                        const uint8_t* fixed_mask, double rtol, int max_iters, apl_pncg_stats* stats, void* stream) {
+  if (s == nullptr || (s->mesh->L.V > 0 && (x == nullptr || rhs == nullptr || y == nullptr)) ||
+      (s->mesh->L.T > 0 && (mu == nullptr || (needs_lambda(s->mesh, s->material) && lam == nullptr)))) {
+    set_error("NULL argument");
+    return -1;
+  }
+  if (!(rtol >= 0) || max_iters < 0) {
+    set_error("invalid CG parameters");
+    return -1;
+  }
+  APL_CHECK_HIP(hipSetDevice(s->mesh->device));
+  (void)hipGetLastError();  // drop errors left by unrelated earlier HIP calls
+  if (s->mesh->dtype == DT_F32)
+    return cg_solve_t<float>(s, mu, lam, x, rhs, y, fixed_mask, rtol, max_iters, stats, (hipStream_t)stream, 1);
+  return cg_solve_t<double>(s, mu, lam, x, rhs, y, fixed_mask, rtol, max_iters, stats, (hipStream_t)stream, 1);
 }
 
 }  // extern "C"

@@ -34,9 +34,25 @@ import numpy as np
 import torch
 
 from . import _lib
-from .common import ACTIVATION, FRACTION, LAMBDA, MU, default_name
+from .common import ACTIVATION, FRACTION, GLOBAL_POINT_ID, LAMBDA, MU, default_name
 from .mesh import Mesh, Region
 from .runtime import DeviceMesh, as_dtype
+
+
+def _global_cells(mesh: Mesh) -> tuple[np.ndarray, int]:
+    """Cells in the model's vertex numbering: through point_data['GlobalPointId']
+    when a ModelBuilder assigned one (the reference indexes
+    cells_global = GlobalPointId[cells], jax/fem/geometry/_geometry.py:40-41),
+    with the vertex count up to the highest global id; else as given."""
+    gid = mesh.point_data.get(GLOBAL_POINT_ID.vtk)
+    if gid is None:
+        return mesh.cells, mesh.n_points
+    gid = np.asarray(gid, np.int64).reshape(-1)
+    if gid.shape[0] != mesh.n_points:
+        raise ValueError("point_data['GlobalPointId'] must have one id per point")
+    if np.array_equal(gid, np.arange(mesh.n_points)):
+        return mesh.cells, mesh.n_points
+    return gid[mesh.cells].astype(np.int32), int(gid.max()) + 1
 
 
 def _fraction(region: Region) -> np.ndarray:
@@ -85,11 +101,15 @@ class FemPotential:
         """Upload the region (dhdX -> inverse rest shapes, dV * Fraction) -- _base.py:94-111."""
         dtype = as_dtype(dtype)
         dV = _fraction(region) * region.dV
-        mesh = DeviceMesh(
-            region.cells, region.dminv, dV, region.mesh.n_points, dtype=dtype, device=device, chunks=chunks
-        )
+        cells, n_points = _global_cells(region.mesh)
+        mesh = DeviceMesh(cells, region.dminv, dV, n_points, dtype=dtype, device=device, chunks=chunks)
         materials = cls.materials_from_region(region, mesh)
-        return cls(mesh, materials, name=name, requires_grad=requires_grad, region=region)
+        pot = cls(mesh, materials, name=name, requires_grad=requires_grad, region=region)
+        pot.cells_host = cells
+        # the element data as uploaded (the caller may change the mesh's
+        # cell data, e.g. its Fraction, before building the next potential)
+        pot._host_data = (np.asarray(cells, np.int32), region.dminv, dV)
+        return pot
 
     @classmethod
     def from_pyvista(cls, obj: Any, *, setup: str = "host", **kwargs):
@@ -120,14 +140,59 @@ class FemPotential:
         ``apl_mesh_create_from_points``; materials come from its cell data."""
         dtype = as_dtype(dtype)
         f = mesh.cell_data.get(FRACTION.vtk)
+        cells, n_points = _global_cells(mesh)
+        points = mesh.points
+        if n_points != mesh.n_points or cells is not mesh.cells:
+            # rest positions in the global numbering (rows no tet touches stay 0)
+            points = np.zeros((n_points, 3))
+            points[np.asarray(mesh.point_data[GLOBAL_POINT_ID.vtk], np.int64).reshape(-1)] = mesh.points
         dmesh = DeviceMesh(
-            mesh.cells, None, None, mesh.n_points, dtype=dtype, device=device, chunks=chunks, points=mesh.points,
+            cells, None, None, n_points, dtype=dtype, device=device, chunks=chunks, points=points,
             fraction=None if f is None else np.asarray(f, np.float64),
         )
         materials = cls.materials_from_region(mesh, dmesh)
         pot = cls(dmesh, materials, name=name, requires_grad=requires_grad)
-        pot.cells_host = np.asarray(mesh.cells)
+        pot.cells_host = np.asarray(cells)
+        # what the element data were computed from (host element data are
+        # derived from it on demand)
+        pot._source = (mesh.points.copy(), np.asarray(mesh.cells).copy(),
+                       None if f is None else np.asarray(f, np.float64).copy())
         return pot
+
+    def host_element_data(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """(cells (T,4) in the vertex numbering this potential evaluates in,
+        dminv (T,3,3), dV (T,) including Fraction) as FP64 host arrays, as they
+        were at construction (a DeviceModel and the FP64 shadow are built
+        from them)."""
+        hd = getattr(self, "_host_data", None)
+        if hd is None:
+            src = getattr(self, "_source", None)
+            if src is None:
+                raise ValueError("no host geometry for this potential")
+            points, cells, frac = src
+            r = Region.from_mesh(Mesh(points, cells))
+            dV = r.dV if frac is None else frac * r.dV
+            hd = (np.asarray(self.cells_host, np.int32), r.dminv, dV)
+            self._host_data = hd
+        return hd
+
+    def fp64_shadow(self) -> "FemPotential":
+        """The same tets with FP64 element data (from the host data, not from
+        this potential's rounded device data), for the FP64 residual passes of
+        mixed-precision refinement; built once.  Its materials are this
+        potential's at build time, widened: callers pass current materials
+        explicitly."""
+        if self.dtype == torch.float64:
+            return self
+        sh = getattr(self, "_shadow", None)
+        if sh is None:
+            cells, dminv, dV = self.host_element_data()
+            mesh = DeviceMesh(cells, dminv, dV, self.n_points, dtype=torch.float64, device=self.device)
+            mats = {k: v.double().contiguous() for k, v in self.materials.items()}
+            sh = type(self)(mesh, mats, name=f"{self.name}_fp64")
+            sh._host_data = self._host_data
+            self._shadow = sh
+        return sh
 
     @classmethod
     def materials_from_region(cls, region: Region, mesh: DeviceMesh) -> dict[str, torch.Tensor]:

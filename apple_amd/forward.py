@@ -10,9 +10,12 @@ Mirrors ``forward/`` of the reference:
 * ``Forward`` -- forward/_forward.py:18-56, ``step()`` runs ``PNCG``.
 
 ``PNCG.minimize`` runs the fused device solver (apple_amd.solver) when the
-model is one FEM potential plus external forces -- the configuration of
-every apple forward experiment -- and otherwise a PNCG driven from the host
-over the model's potentials (same algorithm, one call per potential).
+model's potentials are FEM energies plus external forces: one FEM potential
+directly, several (e.g. the fat / aponeurosis / muscle energies of one mesh,
+exp/2026/01/28/smas/src/31-inverse-activation-stable-neo-hookean.py:256-279,
+or the energies of several meshes) as one ``DeviceModel``.  Any other
+potential makes it fall back to a PNCG driven from the host over the
+model's potentials (same algorithm, one call per potential).
 """
 
 from __future__ import annotations
@@ -26,9 +29,10 @@ import torch
 from .common import FIXED_MASK, FIXED_VALUE, GLOBAL_POINT_ID
 from .fem import FemPotential
 from .mesh import Mesh
+from .model import DeviceModel
 from .potential import ExternalForce
 from .runtime import normalize_device
-from .solver import DeviceSolver, PncgConfig, Result, SolveStats, mask_bits
+from .solver import DeviceSolver, OptState, PncgConfig, Result, SolveStats, mask_bits
 
 
 class DofMapBuilder:
@@ -147,46 +151,76 @@ class Model:
     def _zeros(self):
         return torch.zeros(self.n_points, 3, dtype=self.dtype, device=self.device)
 
+    @staticmethod
+    def _view(t: torch.Tensor, pot) -> torch.Tensor:
+        """The rows a potential evaluates in: the vertex arrays are global, a
+        potential of a mesh added later covers a prefix up to its highest
+        global id (a contiguous view; += outputs leave the rest untouched)."""
+        n = pot.n_points
+        return t if t.shape[0] == n else t[:n]
+
     def fun(self, u: torch.Tensor) -> torch.Tensor:
         out = torch.zeros(1, dtype=torch.float64, device=self.device)
         for pot in self.potentials.values():
-            pot.fun(u, out)
+            pot.fun(self._view(u, pot), out)
         return out[0]
 
     def grad(self, u: torch.Tensor) -> torch.Tensor:
         out = self._zeros()
         for pot in self.potentials.values():
-            pot.grad(u, out)
+            pot.grad(self._view(u, pot), self._view(out, pot))
         return out
 
     def hess_diag(self, u: torch.Tensor) -> torch.Tensor:
         out = self._zeros()
         for pot in self.potentials.values():
-            pot.hess_diag(u, out)
+            pot.hess_diag(self._view(u, pot), self._view(out, pot))
         return out
 
     def hess_prod(self, u: torch.Tensor, p: torch.Tensor) -> torch.Tensor:
         out = self._zeros()
         for pot in self.potentials.values():
-            pot.hess_prod(u, p, out)
+            pot.hess_prod(self._view(u, pot), self._view(p, pot), self._view(out, pot))
         return out
 
     def hess_quad(self, u: torch.Tensor, p: torch.Tensor) -> torch.Tensor:
         out = torch.zeros(1, dtype=torch.float64, device=self.device)
         for pot in self.potentials.values():
-            pot.hess_quad(u, p, out)
+            pot.hess_quad(self._view(u, pot), self._view(p, pot), out)
         return out[0]
 
-    # fused-solver view: exactly one FEM potential plus external forces
+    def external_force(self) -> torch.Tensor | None:
+        """Sum of the ExternalForce potentials as one dense (V,3) load."""
+        f = None
+        for e in self.potentials.values():
+            if isinstance(e, ExternalForce):
+                d = e.dense()
+                if d.shape[0] < self.n_points:
+                    d = torch.cat([d, d.new_zeros(self.n_points - d.shape[0], 3)])
+                f = d if f is None else f + d
+        return f
+
+    def device_model(self) -> DeviceModel:
+        """All FEM potentials as one device model over the model's vertices
+        (built once; its materials are refreshed from the potentials)."""
+        fem = [p for p in self.potentials.values() if isinstance(p, FemPotential)]
+        dm = getattr(self, "_device_model", None)
+        if dm is None or [id(p) for p in dm.parts] != [id(p) for p in fem]:
+            dm = DeviceModel(fem, self.n_points, name="model")
+            self._device_model = dm
+        dm.sync_materials()
+        return dm
+
+    # fused-solver view: FEM potentials plus external forces
     def fused_parts(self):
         fem = [p for p in self.potentials.values() if isinstance(p, FemPotential)]
         ext = [p for p in self.potentials.values() if isinstance(p, ExternalForce)]
-        if len(fem) != 1 or len(fem) + len(ext) != len(self.potentials):
+        if not fem or len(fem) + len(ext) != len(self.potentials):
             return None
-        f = None
-        for e in ext:
-            f = e.dense() if f is None else f + e.dense()
-        return fem[0], f
+        f = self.external_force()
+        if len(fem) == 1 and fem[0].n_points == self.n_points:
+            return fem[0], f
+        return self.device_model(), f
 
 
 class ModelBuilder:
@@ -243,9 +277,20 @@ class ForwardProblem:
 
 
 @dataclasses.dataclass
+class SolutionState:
+    """The solver's end state: ``u`` (the full displacement, as ModelState)
+    and ``grad``, the free-DOF gradient there (solution.state.grad,
+    exp/2026/01/28/smas/src/31-inverse-activation-stable-neo-hookean.py:395-406)."""
+
+    u: torch.Tensor
+    grad: torch.Tensor
+    opt: OptState | None = None
+
+
+@dataclasses.dataclass
 class Solution:
     result: Result
-    state: ModelState
+    state: SolutionState
     stats: SolveStats
 
     @property
@@ -258,27 +303,76 @@ class Solution:
 
 
 class PNCG:
-    """Preconditioned nonlinear CG (see apple_amd/csrc/solver.hip)."""
+    """Preconditioned nonlinear CG (see apple_amd/csrc/solver.hip).
+
+    ``minimize`` solves to the end; ``init`` / ``step`` / ``terminate`` /
+    ``postprocess`` drive the same device solve one accepted step at a time,
+    as liblaf-peach's Optimizer does in apple's experiments
+    (exp/2026/05/06/toy/src/21-smas-prestrain-stable-neo-hookean-muscle.py:497-564);
+    the state they return carries the reference's field names (OptState)."""
 
     def __init__(self, config: PncgConfig | None = None, **kwargs):
         self.config = dataclasses.replace(config or PncgConfig(), **kwargs)
         self._solvers: dict[int, DeviceSolver] = {}
 
+    def _solver_for(self, fem) -> DeviceSolver:
+        key = id(fem)
+        if key not in self._solvers:
+            self._solvers[key] = DeviceSolver(fem)
+        return self._solvers[key]
+
     def minimize(self, problem: ForwardProblem, state: ModelState, u_free: torch.Tensor):
         model = problem.model
         parts = model.fused_parts()
         if parts is not None:
-            return self._fused(model, parts, u_free)
+            return self._fused(problem, parts, u_free)
         return self._host(problem, u_free)
 
-    def _fused(self, model: Model, parts, u_free):
+    def _solution(self, problem: ForwardProblem, x: torch.Tensor, stats: SolveStats, opt=None) -> Solution:
+        grad = problem.grad(ModelState(x))
+        return Solution(stats.result, SolutionState(x, grad, opt), stats)
+
+    def _fused(self, problem: ForwardProblem, parts, u_free):
+        model = problem.model
         fem, f = parts
-        key = id(fem)
-        if key not in self._solvers:
-            self._solvers[key] = DeviceSolver(fem)
         x = model.dof_map.to_full(u_free.to(fem.dtype)).contiguous()
-        stats = self._solvers[key].solve(x, f, model.dof_map.mask_bits, self.config)[0]
-        return Solution(stats.result, ModelState(x), stats), ModelState(x)
+        stats = self._solver_for(fem).solve(x, f, model.dof_map.mask_bits, self.config)[0]
+        return self._solution(problem, x, stats), ModelState(x)
+
+    # ------------------------------------------------------------ step-wise (Optimizer protocol)
+
+    def init(self, problem: ForwardProblem, state: ModelState, u_free: torch.Tensor) -> OptState:
+        parts = problem.model.fused_parts()
+        if parts is None:
+            raise NotImplementedError("step-wise PNCG runs on the fused device path (FEM + external forces)")
+        model = problem.model
+        fem, f = parts
+        self._run = dict(solver=self._solver_for(fem), x=model.dof_map.to_full(u_free.to(fem.dtype)).contiguous(),
+                         f=f)
+        self._run["solver"].begin(self._run["x"], f, model.dof_map.mask_bits, self.config)
+        # evaluate the start point (the solver's first trial), no step yet
+        opt = self._run["solver"].advance(steps=0, max_iters=1)[0]
+        self._run["prev"] = self._run["x"].clone()
+        return opt
+
+    def step(self, problem: ForwardProblem, state: ModelState, opt_state: OptState):
+        run = self._run
+        x = torch.empty_like(run["x"])
+        opt = run["solver"].advance(steps=1, x_out=x)[0]
+        if opt.line_search_state.alpha > 0:
+            opt.direction = problem.model.dof_map.to_free(x - run["prev"]) / opt.line_search_state.alpha
+        run["prev"] = x
+        return ModelState(x), opt
+
+    def terminate(self, problem: ForwardProblem, state: ModelState, opt_state: OptState):
+        done = opt_state.result != Result.RUNNING
+        return done, opt_state.result
+
+    def postprocess(self, problem: ForwardProblem, state: ModelState, opt_state: OptState, result) -> Solution:
+        run = self._run
+        stats = run["solver"].end()[0]
+        self._run = None
+        return self._solution(problem, run["x"], stats, opt_state)
 
     def _host(self, problem: ForwardProblem, u_free: torch.Tensor):
         """PNCG over the ForwardProblem protocol (any sum of potentials)."""
@@ -359,7 +453,7 @@ class PNCG:
                     break
                 restarts += 1
         stats = SolveStats(result, n_steps, n_evals, n_bt, f, float(torch.linalg.norm(g)), gn0, alpha, beta, f_first)
-        return Solution(result, st, stats), st
+        return Solution(result, SolutionState(st.u, g.to(st.u.dtype)), stats), st
 
 
 class Forward:

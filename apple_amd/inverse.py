@@ -118,9 +118,12 @@ class Inverse:
     """Adam over ``params`` of L(x*(make_materials(params)), materials).
 
     potential: a FEM potential (``fem.StableNeoHookeanMuscle`` for activations,
-    or SNH / ARAP for mu, lambda) or a ``batch.Batch``'s potential;
+    or SNH / ARAP for mu, lambda), a ``batch.Batch``'s potential, or a
+    ``model.DeviceModel`` (several potentials, e.g. fat + aponeurosis +
+    muscle of exp/2026/01/28/smas/src/31-inverse-activation-stable-neo-hookean.py:256-279);
     make_materials(params) -> dict with any of "mu", "lambda", "activation"
-    (missing ones keep the potential's values).  Every forward solve starts
+    (missing ones keep the potential's values), or for a DeviceModel a list
+    with one such dict (or None) per potential.  Every forward solve starts
     from the previous solution (warm start); the first from ``x0``.  The
     solver's rtol is relative to each solve's first gradient, which is small
     after a warm start; so the first (cold) solve fixes the absolute target:
@@ -157,8 +160,15 @@ class Inverse:
         self._warm_cfg: PncgConfig | None = None
 
     def _materials(self, params):
-        base = dict(self.potential.materials)
-        base.update(self.make_materials(params))
+        made = self.make_materials(params)
+        pot = self.potential
+        if isinstance(made, (list, tuple)):
+            # a DeviceModel: one dict per potential (None keeps it), gathered
+            # into the model's per-tet arrays (autograd reaches each part)
+            per = [dict(part.materials, **(d or {})) for part, d in zip(pot.parts, made)]
+            return pot.gather(per)
+        base = dict(pot.materials)
+        base.update(made)
         return base
 
     def loss_at(self, params: torch.Tensor, *, warm: bool = True) -> torch.Tensor:

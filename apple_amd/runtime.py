@@ -81,6 +81,7 @@ class DeviceMesh:
         chunks: int = 1,
         points=None,
         fraction=None,
+        segments=None,
     ):
         self.dtype = as_dtype(dtype)
         self.device = normalize_device(device)
@@ -115,7 +116,21 @@ class DeviceMesh:
         L = _lib.lib()
         common = (self.n_inst, self.inst_tet_off.ctypes.data, self.inst_vtx_off.ctypes.data, int(chunks),
                   self.device.index, ctypes.byref(self._handle))
-        if points is not None:
+        self.segments = None
+        if segments is not None:
+            # a model: (seg_law (n_seg,), seg_tet_off (n_inst * n_seg + 1,))
+            seg_law = np.ascontiguousarray(segments[0], dtype=np.int32)
+            seg_off = np.ascontiguousarray(segments[1], dtype=np.int64)
+            self.segments = (seg_law, seg_off)
+            _lib.check(
+                L.apl_model_create(
+                    _DT[self.dtype], T, self.n_verts, cells.ctypes.data, dminv.ctypes.data, dV.ctypes.data,
+                    self.n_inst, self.inst_tet_off.ctypes.data, self.inst_vtx_off.ctypes.data, int(seg_law.size),
+                    seg_law.ctypes.data, seg_off.ctypes.data, int(chunks), self.device.index,
+                    ctypes.byref(self._handle),
+                )
+            )
+        elif points is not None:
             _lib.check(
                 L.apl_mesh_create_from_points(
                     _DT[self.dtype], T, self.n_verts, cells.ctypes.data, points.ctypes.data,
@@ -215,7 +230,7 @@ class DeviceMesh:
                 self._handle,
                 int(material),
                 ptr(mu, name="mu", **tet),
-                ptr(lam, name="lambda", **tet) if material != _lib.APL_ARAP else None,
+                ptr(lam, name="lambda", **tet) if (material != _lib.APL_ARAP and lam is not None) else None,
                 ptr(u, **self.vec("u")),
                 ptr(p, **self.vec("p")),
                 what,

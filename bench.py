@@ -77,9 +77,9 @@ def bench_pass(pot, u, what, iters, kernel=True, full=True):
     V = m.n_verts
     a = torch.empty(V, 3, dtype=m.dtype, device=m.device)
     b = torch.empty_like(a)
-    e = torch.zeros(1, dtype=torch.float64, device=m.device)
+    e = torch.zeros(m.n_inst, dtype=torch.float64, device=m.device)  # one energy per instance
     lam = pot.materials.get("lambda")
-    pot._prepare()  # binds per-mesh material arrays (the muscle activation)
+    getattr(pot, "_prepare", lambda: None)()  # binds per-mesh material arrays (the muscle activation)
     p = u if what != _lib.APL_ENERGY | _lib.APL_GRAD | _lib.APL_HESS_DIAG else None
     k = ctypes.c_double()
     tp = ctypes.c_double()
@@ -159,19 +159,42 @@ def cpu_baseline(prob, target_s=12.0, threads=1):
         kind="port",
         sample=f"{n} passes of fun+grad+hess_diag (separate passes, atomic scatter, FP64) over the same "
         f"{m.n_cells}-tet mesh, {dt:.1f} s, {used} thread(s)",
+        cpu=host_cpu(),
     )
 
 
-def load_traffic(name):
+def host_cpus() -> int:
+    """Cores this process may run on (the affinity mask, else the CPU count)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        return os.cpu_count() or 1
+
+
+def host_cpu() -> dict:
+    """The host CPU: model name and the cores available to this process."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "cores_available": host_cpus(), "os_cpu_count": os.cpu_count()}
+
+
+def load_traffic(name, key="pass_bytes_per_launch"):
     """HBM bytes per launch measured by rocprofv3 (FETCH_SIZE and WRITE_SIZE
     passes, scaled by the copy-kernel calibration); the summary lives under
-    profiles/ as text with a line 'bytes_per_launch: <int>'."""
+    profiles/ as text with lines '<key>: <int>'."""
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     with open(path) as f:
         for line in f:
-            if line.strip().startswith("bytes_per_launch:"):
+            if line.strip().startswith(key + ":"):
                 return int(line.split(":", 1)[1].strip())
     return None
 
@@ -222,18 +245,21 @@ def main():
     ap.add_argument("--diff-only", action="store_true", help="only the 32^3 adjoint-vs-finite-difference check")
     ap.add_argument("--combined-only", action="store_true", help="only the FP32-vs-FP64 batched forward+adjoint")
     ap.add_argument("--no-diff", action="store_true")
-    ap.add_argument("--fd-tets", type=int, default=16,
+    ap.add_argument("--fd-tets", type=int, default=64,
                     help="central-difference tets of the differentiation check in the full run (--diff-only: 64)")
     ap.add_argument("--no-combined", action="store_true")
     ap.add_argument("--no-inverse", action="store_true")
     ap.add_argument("--config1-only", action="store_true", help="only the 10^3 cube solved end to end, CPU and GPU")
     ap.add_argument("--inverse-only", action="store_true", help="only the muscle-activation inverse loop")
     ap.add_argument("--batched-meshes", type=int, default=256)
-    ap.add_argument("--cpu-threads", type=int, default=16, help="threads of the multi-threaded CPU baseline")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads of the multi-threaded CPU baseline (0: every core available to the process)")
     ap.add_argument("--profile-pass", action="store_true",
                     help="only run pass-A launches and a reference copy (for rocprofv3 counter runs)")
     ap.add_argument("--profile-hess-prod", action="store_true",
                     help="only 20 pass-C (hess_prod, SNH) launches and 3 calibration copies, for rocprofv3")
+    ap.add_argument("--profile-batched", action="store_true",
+                    help="only 20 pass-A launches over the 256-cube batch (n ~ U{6..32}) and 3 calibration copies")
     ap.add_argument("--profile-solve", type=int, default=0, metavar="STEPS",
                     help="only run a PNCG solve of STEPS iterations (for rocprofv3 kernel traces)")
     args = ap.parse_args()
@@ -261,8 +287,28 @@ def main():
     if args.inverse_only:
         print(json.dumps(bench_inverse(dev)), flush=True)
         return
+    if args.cpu_threads <= 0:
+        args.cpu_threads = host_cpus()
     if args.config1_only:
         print(json.dumps(bench_config1(dev, args.cpu_threads)), flush=True)
+        return
+    if args.profile_batched:
+        from apple_amd import batch
+
+        insts, _ = make_cube_instances(args.batched_meshes, 6, 32, 2, 3)
+        b = batch.Batch(insts, _lib.APL_SNH, dtype=dtype, device=dev)
+        pot = b.potential
+        u = b.x0
+        k, pa = bench_pass(pot, u, _lib.APL_ENERGY | _lib.APL_GRAD | _lib.APL_HESS_DIAG, 20)
+        T, V = b.n_tets, b.n_verts
+        log(f"[bench] batched pass A: {T} tets, {V} verts, kernel {k * 1e3:.1f} us, pass {pa * 1e3:.1f} us, "
+            f"{(64 * T + 36 * V) / (pa / 1e3) / 1e9 / HBM_PEAK_GBS:.1%} of HBM (whole pass)")
+        a = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device=dev)  # 256 MiB copies: counter calibration
+        c = torch.empty_like(a)
+        for _ in range(3):
+            c.copy_(a)
+        torch.cuda.synchronize()
+        log("[bench] profile-batched done")
         return
     log(f"[bench] building {n}^3 x 5 cube (SNH, FP32)")
     prob = make_cube_problem(n, "snh", dtype, dev, chunks=args.chunks)
@@ -333,8 +379,14 @@ def main():
     ms_kernel, _ = bench_pass(pot, u, what_a, args.steps, full=False)  # the element kernel alone
     ms_per_step = wall / args.steps * 1e3
     value = T / (ms_per_step / 1e3) / 1e6
+    # algorithmic bytes of the whole pass (u in, element record in, gradient
+    # and Hdiag out) over the whole pass's time (element kernel + vertex sums),
+    # both measured here with HIP events on the launch stream
     bytes_a = 64 * T + 36 * V
-    achieved = bytes_a / (ms_kernel / 1e3) / 1e9
+    achieved = bytes_a / (ms_pass / 1e3) / 1e9
+    # the element kernel alone: its compulsory bytes are the element record
+    # and u (its partial sums are the reduction's own traffic)
+    bytes_elem = 64 * T + 12 * V
     traffic = load_traffic("pass_a_snh64_traffic.md") if n == 64 else None
     roofline = {
         "bound": "hbm",
@@ -343,10 +395,15 @@ def main():
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": traffic,
-        "kernel": "elem_kernel<SNH, E|G|H, float>",
-        "kernel_ms": round(ms_kernel, 5),
+        "traffic_source": ("profiles/pass_a_snh64_traffic.md: rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, of the "
+                           "element kernel and the vertex-sum kernel of this workload, scaled by a copy kernel"
+                           if traffic is not None else None),
+        "kernel": "elem_kernel<SNH, E|G|H, float> + finish_kernel<float, 6> (the whole pass)",
         "pass_ms": round(ms_pass, 5),
+        "kernel_ms": round(ms_kernel, 5),
         "algorithmic_bytes_per_launch": bytes_a,
+        "element_kernel_only": {"algorithmic_bytes": bytes_elem,
+                                "frac": round(bytes_elem / (ms_kernel / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
     }
     log(f"[bench] pass A: kernel {ms_kernel*1e3:.1f} us, pass {ms_pass*1e3:.1f} us, wall/step {ms_per_step*1e3:.1f} us,"
         f" {value:.0f} Mtet-evals/s, {achieved:.0f} GB/s ({achieved/HBM_PEAK_GBS:.1%} of HBM)")
@@ -436,7 +493,7 @@ def main():
     # ---------------------------------------------------------------- CPU baseline
     if not (args.quick or args.no_cpu):
         out["config1_solve"] = bench_config1(dev, args.cpu_threads)
-        log(f"[bench] CPU baseline (C restatement, {args.cpu_threads} threads and 1 thread)")
+        log(f"[bench] CPU baseline (C restatement, {args.cpu_threads} threads and 1 thread, {host_cpu()})")
         out["cpu_baseline"] = cpu_baseline(prob, target_s=12.0, threads=args.cpu_threads)
         out["cpu_baseline_1thread"] = cpu_baseline(prob, target_s=12.0, threads=1)
     print(json.dumps(out), flush=True)
@@ -530,10 +587,15 @@ def bench_batched(dev, dtype, n_meshes=256, lo=6, hi=32, seed_sizes=2, seed_mat=
         torch.cuda.synchronize()
         t_plain = time.perf_counter() - t0
         key = "forward_adjoint" if adjoint else "forward"
+        ok_b = int(sum(s.success for s in stb))
         res[key] = {
+            # solves/s count only the instances that reach rtol 1e-6
+            "batched_solves_per_s": round(ok_b / t_batch, 3),
+            "plain_solves_per_s": round(ok_fwd / t_plain, 3),
             "batched_instances_per_s": round(n_meshes / t_batch, 3),
             "plain_instances_per_s": round(n_meshes / t_plain, 3),
             "speedup": round(t_plain / t_batch, 3),
+            "speedup_converged_solves": round((ok_b / t_batch) / max(ok_fwd / t_plain, 1e-300), 3),
             "batched_s": round(t_batch, 4),
             "plain_s": round(t_plain, 4),
             "batched_forward_converged": int(sum(s.success for s in stb)),
@@ -545,7 +607,11 @@ def bench_batched(dev, dtype, n_meshes=256, lo=6, hi=32, seed_sizes=2, seed_mat=
         log(f"[bench] batched {key}: {t_batch:.3f} s vs plain {t_plain:.3f} s ({t_plain / t_batch:.2f}x)")
     res["workload"] = (f"{n_meshes} cubes, n ~ U{{{lo}..{hi}}} (seed {seed_sizes}), {b.n_tets} tets total, SNH FP32, "
                        f"mu ~ U[1e3,1e5], nu ~ U[0.3,0.45] (seed {seed_mat}), PNCG to 1e-6; adjoint: PCG to "
-                       f"{adj_rtol:g} on H p = -(x - x0), then the mixed pass for dL/d(mu, lambda)")
+                       f"{adj_rtol:g} on H p = -(x - x0), then the mixed pass for dL/d(mu, lambda); both paths "
+                       "replay captured graphs (PNCG and PCG)")
+    its = np.array([s.n_steps for s in stb])
+    res["batched_iterations"] = {"median": int(np.median(its)), "p95": int(np.percentile(its, 95)),
+                                 "max": int(its.max())}
     res["max_n"] = max(sizes)
     res["setup_s"] = round(t_setup, 2)
     res["batch_schedule"] = {"chunks": b.schedule.chunks, "grid_pct": b.schedule.grid_pct,
@@ -619,22 +685,31 @@ def bench_differentiation(dev, n=32, n_fd=64, seed_true=4, seed_init=5, seed_pic
     g_ref = (dmu + 10 * dlam).cpu().numpy()
     log(f"[bench] differentiation: FP64 reference equilibria and adjoint, {time.perf_counter() - t0:.1f} s")
 
-    # the product path: FP32 (and FP64) forward + adjoint through the autograd function
+    # the product path: FP32 (plain and refined) and FP64 forward + adjoint through the autograd function
     grads, times = {}, {}
-    for dt, cfg, acfg in ((torch.float32, PncgConfig(rtol=1e-6, max_steps=50000), adjoint.AdjointConfig(rtol=1e-6)),
-                          (torch.float64, PncgConfig(rtol=1e-10, max_steps=50000),
-                           adjoint.AdjointConfig(rtol=1e-10))):
+    refine = adjoint.RefineConfig(rtol=1e-10, adjoint_rtol=1e-10, f_ext64=f64)
+    variants = (("fp32", torch.float32, PncgConfig(rtol=1e-6, max_steps=50000), adjoint.AdjointConfig(rtol=1e-6), None),
+                ("fp32_refined", torch.float32, PncgConfig(rtol=1e-6, max_steps=50000),
+                 adjoint.AdjointConfig(rtol=1e-6), refine),
+                ("fp64", torch.float64, PncgConfig(rtol=1e-10, max_steps=50000), adjoint.AdjointConfig(rtol=1e-10),
+                 None))
+    pots[torch.float32].fp64_shadow()  # refinement setup, outside the timings
+    for name, dt, cfg, acfg, rf in variants:
         ft = torch.as_tensor(f, dtype=dt, device=dev)
-        xt = x_t.to(dt)
+        xt = x_t.to(dt) if rf is None else x_t
         for rep in range(2):  # the first run builds the graphs
             mu_t = torch.as_tensor(mu0, dtype=dt, device=dev).requires_grad_(True)
             torch.cuda.synchronize()
             t1 = time.perf_counter()
-            x = adjoint.equilibrium(pots[dt], mu_t, 10 * mu_t, ft, mask, torch.zeros_like(ft), cfg, acfg)
+            x = adjoint.equilibrium(pots[dt], mu_t, 10 * mu_t, ft, mask, torch.zeros_like(ft), cfg, acfg, refine=rf)
             ((x - xt) ** 2).sum().backward()
             torch.cuda.synchronize()
-            times[dt] = time.perf_counter() - t1
-        grads[dt] = mu_t.grad.double().cpu().numpy()
+            times[name] = time.perf_counter() - t1
+        grads[name] = mu_t.grad.double().cpu().numpy()
+        if rf is not None:
+            fr, ar = adjoint.last_refine_reports()
+            refine_steps = {"forward_steps": fr.steps, "forward_cg_iterations": fr.cg_iterations,
+                            "adjoint_steps": ar.steps, "adjoint_cg_iterations": ar.cg_iterations}
 
     # FP64 central differences, warm-started at the polished equilibrium of mu0
     def loss(mu_np):
@@ -664,19 +739,31 @@ def bench_differentiation(dev, n=32, n_fd=64, seed_true=4, seed_init=5, seed_pic
                 "max_abs_err_over_max_fd": float(e.max() / np.abs(fd).max())}
 
     out["fp64_reference_adjoint"] = errs(g_ref[pick])
-    for dt, name in ((torch.float32, "fp32"), (torch.float64, "fp64")):
-        out[name] = {"forward_adjoint_s": round(times[dt], 4), **errs(grads[dt][pick]),
-                     "vs_fp64_reference_norm_rel_err": float(np.linalg.norm(grads[dt] - g_ref) / np.linalg.norm(g_ref))}
+    for name in ("fp32", "fp32_refined", "fp64"):
+        out[name] = {"forward_adjoint_s": round(times[name], 4), **errs(grads[name][pick]),
+                     "vs_fp64_reference_norm_rel_err": float(np.linalg.norm(grads[name] - g_ref) / np.linalg.norm(g_ref)),
+                     "vs_fp64_reference_max_entry_err_over_max": float(np.abs(grads[name] - g_ref).max()
+                                                                       / np.abs(g_ref).max())}
         log(f"[bench] differentiation {name}: ||g - fd|| / ||fd|| = {out[name]['norm_rel_err']:.2e}, "
-            f"vs the polished FP64 adjoint {out[name]['vs_fp64_reference_norm_rel_err']:.2e}, {times[dt]:.3f} s")
+            f"vs the polished FP64 adjoint {out[name]['vs_fp64_reference_norm_rel_err']:.2e}, {times[name]:.3f} s")
+    out["fp32_refined"]["refinement"] = refine_steps
     log(f"[bench] differentiation: polished FP64 adjoint vs FD {out['fp64_reference_adjoint']['norm_rel_err']:.2e}")
     return out
 
 
-def bench_combined(dev, n_meshes=64, lo=16, hi=48, seed=7, adj_rtol32=1e-6, adj_rtol64=1e-10):
+def bench_combined(dev, n_meshes=64, lo=16, hi=48, seed=7, adj_rtol32=1e-6, tight=1e-10):
     """Config 5: forward + adjoint of one batch of uneven meshes with FP32
-    element kernels (FP64 iterate and reductions) against the same batch run
-    all in FP64; per-instance errors of energy, dL/dmu and positions."""
+    element kernels against the same batch run all in FP64 (forward and
+    adjoint to rtol 1e-10); per-instance errors of energy, dL/d(mu, lambda)
+    and positions, for
+      * "fp32": the FP32 path at the reference's tolerance (PNCG and PCG to
+        rtol 1e-6, FP64 iterate and reductions);
+      * "fp32_refined": the same FP32 solves followed by mixed-precision
+        refinement (FP64 residual passes, FP32 PCG corrections) to the FP64
+        run's tolerance, forward and adjoint;
+      * "fp64_rtol_1e-6": the FP64 run stopped at the FP32 run's tolerance,
+        to separate what the tolerance allows from what precision costs.
+    L = 1/2 ||x - x0||^2, so the adjoint right-hand side is x0 - x."""
     import torch
 
     from apple_amd import _lib, batch
@@ -684,116 +771,126 @@ def bench_combined(dev, n_meshes=64, lo=16, hi=48, seed=7, adj_rtol32=1e-6, adj_
 
     insts, sizes = make_cube_instances(n_meshes, lo, hi, seed, seed)
     res = {}
-    runs = {}
-    for dt, cfg, ar in ((torch.float32, PncgConfig(rtol=1e-6, max_steps=50000), adj_rtol32),
-                        (torch.float64, PncgConfig(rtol=1e-10, max_steps=50000), adj_rtol64)):
-        b = batch.Batch(insts, _lib.APL_SNH, dtype=dt, device=dev)
-        forward_adjoint(b, PncgConfig(rtol=1e-6, max_steps=16), ar, adjoint=False)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        x, st, cs, dmu, dlam = forward_adjoint(b, cfg, ar)
-        torch.cuda.synchronize()
-        res[str(dt).split(".")[-1] + "_s"] = round(time.perf_counter() - t0, 3)
-        runs[dt] = (b, x.double(), st, cs, dmu.double(), dlam.double())
-        log(f"[bench] combined {dt}: {res[str(dt).split('.')[-1] + '_s']} s")
-    b32, x32, st32, cs32, dmu32, dlam32 = runs[torch.float32]
-    b64, x64, st64, cs64, dmu64, dlam64 = runs[torch.float64]
-    # the same FP64 batch solved to the FP32 run's tolerance (rtol 1e-6): how
-    # far an rtol-1e-6 equilibrium may sit from the tight one whatever the
-    # precision (ill-conditioned instances)
-    x64l, st64l, *_ = forward_adjoint(b64, PncgConfig(rtol=1e-6, max_steps=50000), adj_rtol64, adjoint=False)
-    x64l = x64l.double()
-    # FP32 forward with the FP64 re-solve of the instances that stagnate
-    b32r = batch.Batch(insts, _lib.APL_SNH, dtype=torch.float32, device=dev)
-    cfg32 = PncgConfig(rtol=1e-6, max_steps=50000)
-    b32r.solve(config=PncgConfig(rtol=1e-6, max_steps=16))  # graph capture
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    b32r.solve(config=cfg32)
-    torch.cuda.synchronize()
-    t_plain = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    xr, _ = b32r.solve(config=cfg32, refine_fp64=True)
-    torch.cuda.synchronize()
-    t_ref = time.perf_counter() - t0
-    xr = xr.double()
-    n_ref = len(b32r.refined)
-    ref_runs_stats = list(zip(b32r.refined, getattr(b32r, "refine_stats", [])))
-    # "checked": an FP64 gradient check of every FP32 solution (first call
-    # builds the FP64 copy of the batch), then the FP64 re-solve
-    t0 = time.perf_counter()
-    xc, _ = b32r.solve(config=cfg32, refine_fp64="checked")
-    torch.cuda.synchronize()
-    t_chk_first = time.perf_counter() - t0
-    n_chk = len(b32r.refined)
-    t0 = time.perf_counter()
-    b32r.solve(config=cfg32, refine_fp64="checked")
-    torch.cuda.synchronize()
-    t_chk = time.perf_counter() - t0
-    xc = xc.double()
     diag = np.sqrt(3.0)
-    e_err, g_err, x_err, xl_err, xr_err, xc_err = [], [], [], [], [], []
-    for i in range(n_meshes):
-        sv = slice(int(b32.vtx_off[i]), int(b32.vtx_off[i + 1]))
-        stt = slice(int(b32.tet_off[i]), int(b32.tet_off[i + 1]))
-        e_err.append(abs(st32[i].f - st64[i].f) / max(abs(st64[i].f), 1e-300))
-        gg32 = torch.cat([dmu32[stt], dlam32[stt]])
-        gg64 = torch.cat([dmu64[stt], dlam64[stt]])
-        g_err.append(float(torch.linalg.norm(gg32 - gg64) / torch.linalg.norm(gg64)))
-        x_err.append(float((x32[sv] - x64[sv]).abs().max()) / diag)
-        xl_err.append(float((x64l[sv] - x64[sv]).abs().max()) / diag)
-        xr_err.append(float((xr[sv] - x64[sv]).abs().max()) / diag)
-        xc_err.append(float((xc[sv] - x64[sv]).abs().max()) / diag)
-    e_err, g_err, x_err, xl_err, xr_err, xc_err = map(np.asarray, (e_err, g_err, x_err, xl_err, xr_err, xc_err))
-    ok = np.array([s.success for s in st32])
+    cfg_tight = PncgConfig(rtol=tight, max_steps=50000)
+    cfg32 = PncgConfig(rtol=1e-6, max_steps=50000)
+    warm = PncgConfig(rtol=1e-6, max_steps=16)
+
+    # all-FP64 reference run
+    b64 = batch.Batch(insts, _lib.APL_SNH, dtype=torch.float64, device=dev)
+    forward_adjoint(b64, warm, tight, adjoint=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x64, st64, cs64, dmu64, dlam64 = forward_adjoint(b64, cfg_tight, tight)
+    torch.cuda.synchronize()
+    res["fp64_s"] = round(time.perf_counter() - t0, 3)
+    x64, dmu64, dlam64 = x64.double(), dmu64.double(), dlam64.double()
+    log(f"[bench] combined FP64: {res['fp64_s']} s")
+    x64l, st64l, *_ = forward_adjoint(b64, cfg32, tight, adjoint=False)
+    x64l = x64l.double()
+
+    # FP32 at the reference tolerance
+    b32 = batch.Batch(insts, _lib.APL_SNH, dtype=torch.float32, device=dev)
+    forward_adjoint(b32, warm, adj_rtol32, adjoint=False)
+    b32._fp64_copy()  # setup of the refinement's FP64 element data (excluded from the timings)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x32, st32, cs32, dmu32, dlam32 = forward_adjoint(b32, cfg32, adj_rtol32)
+    torch.cuda.synchronize()
+    res["fp32_s"] = round(time.perf_counter() - t0, 3)
+    x32 = x32.double()
+    log(f"[bench] combined FP32: {res['fp32_s']} s")
+
+    # FP32 + mixed-precision refinement of the forward and adjoint solutions
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    xr, str_ = b32.solve(config=cfg32, refine_fp64="newton", refine_rtol=tight)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    fwd_rep = b32.refine_report
+    resolved = list(b32.refine_resolved)
+    x64r = b32.x64
+    t1 = time.perf_counter()
+    pr, csr = b32.adjoint_solve(xr, b32.x0.double() - x64r, rtol=tight, refine_fp64=True,
+                                cg_rtol=adj_rtol32)
+    dmur = torch.empty(b32.n_tets, dtype=b32.dtype, device=dev)
+    dlamr = torch.empty_like(dmur)
+    b32.mesh.eval(b32.material, b32.mu, b32.lam, xr, pr, dmu=dmur, dlam=dlamr)
+    torch.cuda.synchronize()
+    t_all = time.perf_counter() - t0
+    adj_rep = b32.adjoint_report
+    res["fp32_refined_s"] = round(t_all, 3)
+    res["fp32_refined_forward_s"] = round(t_fwd, 3)
+    log(f"[bench] combined FP32 + refinement: forward {t_fwd:.3f} s, forward + adjoint {t_all:.3f} s "
+        f"({fwd_rep.steps} forward / {adj_rep.steps} adjoint refinement steps, {len(resolved)} FP64 re-solves)")
+
+    def errs(xx, st, dmu, dlam):
+        e_err, g_err, g_max, x_err = [], [], [], []
+        for i in range(n_meshes):
+            sv = slice(int(b32.vtx_off[i]), int(b32.vtx_off[i + 1]))
+            stt = slice(int(b32.tet_off[i]), int(b32.tet_off[i + 1]))
+            e_err.append(abs(st[i].f - st64[i].f) / max(abs(st64[i].f), 1e-300))
+            if dmu is not None:
+                gg = torch.cat([dmu[stt].double(), dlam[stt].double()])
+                gg64 = torch.cat([dmu64[stt], dlam64[stt]])
+                g_err.append(float(torch.linalg.norm(gg - gg64) / torch.linalg.norm(gg64)))
+                g_max.append(float((gg - gg64).abs().max() / gg64.abs().max()))
+            x_err.append(float((xx[sv] - x64[sv]).abs().max()) / diag)
+        return tuple(np.asarray(a) for a in (e_err, g_err, g_max, x_err))
+
+    def summary(e_err, g_err, g_max, x_err):
+        out = {"energy_rel_err_max": float(e_err.max()), "pos_err_over_diag_max": float(x_err.max()),
+               "pos_err_over_diag_median": float(np.median(x_err)),
+               "within_energy_1e-5": int((e_err <= 1e-5).sum()), "within_pos_1e-5_diag": int((x_err <= 1e-5).sum())}
+        if len(g_err):
+            out.update({"grad_rel_err_max": float(g_err.max()), "grad_rel_err_median": float(np.median(g_err)),
+                        "grad_max_entry_err_over_max": float(g_max.max()),
+                        "within_grad_1e-4": int((g_err <= 1e-4).sum())})
+        return out
+
+    E32 = errs(x32, st32, dmu32, dlam32)
+    ER = errs(x64r, str_, dmur, dlamr)
+    EL = errs(x64l, st64l, None, None)
+    # branch switches: the refined FP32 answer is an FP64 equilibrium (residual at the
+    # target) but not the FP64 run's one
+    switched = [i for i in range(n_meshes) if ER[3][i] > 1e-5 and fwd_rep.converged[i] and i not in resolved]
+    on_branch = np.array([i not in switched for i in range(n_meshes)])
     res.update({
-        "workload": f"{n_meshes} cubes, n ~ U{{{lo}..{hi}}} (seed {seed}), {b32.n_tets} tets, SNH, per-instance mu/nu "
-                    f"as the batched case; L = 1/2 ||x - x0||^2",
-        "fp32_converged": int(ok.sum()),
+        "workload": f"{n_meshes} cubes, n ~ U{{{lo}..{hi}}} (seed {seed}), {b32.n_tets} tets, SNH, per-instance "
+                    f"mu/nu as the batched case; L = 1/2 ||x - x0||^2; FP64 run to rtol {tight:g}",
+        "fp32_converged": int(sum(s.success for s in st32)),
         "fp32_stagnated": int(sum(s.result.name == "STAGNATION" for s in st32)),
         "fp64_converged": int(sum(s.success for s in st64)),
-        "energy_rel_err_max": float(e_err.max()),
-        "grad_rel_err_max": float(g_err.max()),
-        "grad_rel_err_median": float(np.median(g_err)),
-        "pos_err_over_diag_max": float(x_err.max()),
-        "pos_err_over_diag_max_converged": float(x_err[ok].max()) if ok.any() else None,
-        "within_energy_1e-5": int((e_err <= 1e-5).sum()),
-        "within_grad_1e-4": int((g_err <= 1e-4).sum()),
-        "within_pos_1e-5_diag": int((x_err <= 1e-5).sum()),
-        "fp32_forward_refine_fp64": {
-            "forward_s": round(t_plain, 3),
-            "forward_refined_s": round(t_ref, 3),
-            "refined_instances": n_ref,
-            "refine_runs": [{"n": sizes[i], "steps": r.n_steps, "result": r.result.name,
-                             "fp64_grad_at_start_over_g0": r.grad_norm_first / max(st32[i].grad_norm_first, 1e-300),
-                             "fp64_grad_end_over_g0": r.grad_norm / max(st32[i].grad_norm_first, 1e-300),
-                             "pos_err_over_diag": float(xr_err[i])}
-                            for i, r in ref_runs_stats],
-            "pos_err_over_diag_max": float(xr_err.max()),
-            "within_pos_1e-5_diag": int((xr_err <= 1e-5).sum()),
+        "fp32": summary(*E32),
+        "fp32_refined": {
+            **summary(*ER),
+            "forward_refinement_steps": fwd_rep.steps,
+            "forward_refinement_cg_iterations": fwd_rep.cg_iterations,
+            "adjoint_refinement_steps": adj_rep.steps,
+            "adjoint_refinement_cg_iterations": adj_rep.cg_iterations,
+            "fp64_pncg_resolved": resolved,
+            "forward_residual_converged": int(fwd_rep.converged.sum()),
+            "adjoint_residual_converged": int(adj_rep.converged.sum()),
+            "on_fp64_branch": int(on_branch.sum()),
+            "on_branch_energy_max": float(ER[0][on_branch].max()),
+            "on_branch_pos_max": float(ER[3][on_branch].max()),
+            "on_branch_grad_max": float(ER[1][on_branch].max()),
+            "branch_switches": [{"instance": i, "n": sizes[i], "pos_err_over_diag": float(ER[3][i]),
+                                 "energy_gap_rel": (str_[i].f - st64[i].f) / abs(st64[i].f),
+                                 "fp32_result": st32[i].result.name} for i in switched],
         },
-        "fp32_forward_checked_fp64": {
-            "forward_checked_first_s": round(t_chk_first, 3),
-            "forward_checked_s": round(t_chk, 3),
-            "refined_instances": n_chk,
-            "pos_err_over_diag_max": float(xc_err.max()),
-            "within_pos_1e-5_diag": int((xc_err <= 1e-5).sum()),
-        },
-        "fp64_at_rtol_1e-6": {
-            "pos_err_over_diag_max": float(xl_err.max()),
-            "within_pos_1e-5_diag": int((xl_err <= 1e-5).sum()),
-            "pos_err_over_diag_median": float(np.median(xl_err)),
-        },
-        "per_instance": [{"n": sizes[i], "fp32": st32[i].result.name, "e": float(e_err[i]), "g": float(g_err[i]),
-                          "x": float(x_err[i]), "x_fp64_rtol_1e-6": float(xl_err[i])} for i in range(n_meshes)],
+        "fp64_rtol_1e-6": summary(*EL),
+        "per_instance": [{"n": sizes[i], "fp32": st32[i].result.name, "e": float(E32[0][i]), "g": float(E32[1][i]),
+                          "x": float(E32[3][i]), "x_refined": float(ER[3][i]), "e_refined": float(ER[0][i]),
+                          "g_refined": float(ER[1][i]), "x_fp64_rtol_1e-6": float(EL[3][i])}
+                         for i in range(n_meshes)],
     })
-    log(f"[bench] combined: energy rel err max {e_err.max():.2e}, dL/dq rel err max {g_err.max():.2e}, "
-        f"positions max {x_err.max():.2e} of the bbox diagonal ({int((x_err <= 1e-5).sum())}/{n_meshes} within 1e-5); "
-        f"FP64 at the same rtol: max {xl_err.max():.2e} ({int((xl_err <= 1e-5).sum())}/{n_meshes} within 1e-5); "
-        f"FP32 + FP64 re-solve of {n_ref} stagnated: max {xr_err.max():.2e} "
-        f"({int((xr_err <= 1e-5).sum())}/{n_meshes}), forward {t_plain:.2f} s -> {t_ref:.2f} s; checked: {n_chk} "
-        f"re-solved, max {xc_err.max():.2e} ({int((xc_err <= 1e-5).sum())}/{n_meshes}), {t_chk_first:.2f} s first, "
-        f"{t_chk:.2f} s")
+    r = res["fp32_refined"]
+    log(f"[bench] combined: FP32 at rtol 1e-6: positions within 1e-5 {res['fp32']['within_pos_1e-5_diag']}/{n_meshes}, "
+        f"dL/dq within 1e-4 {res['fp32']['within_grad_1e-4']}/{n_meshes}; refined: energy max {r['energy_rel_err_max']:.2e}, "
+        f"positions within 1e-5 {r['within_pos_1e-5_diag']}/{n_meshes}, dL/dq within 1e-4 {r['within_grad_1e-4']}/{n_meshes}; "
+        f"on the FP64 branch ({r['on_fp64_branch']}): energy {r['on_branch_energy_max']:.2e}, positions "
+        f"{r['on_branch_pos_max']:.2e}, dL/dq {r['on_branch_grad_max']:.2e}; branch switches {switched}")
     return res
 
 

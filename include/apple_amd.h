@@ -29,7 +29,9 @@ enum { APL_F32 = 0, APL_F64 = 1 };
 /* materials: warp/fem/_stable_neo_hookean.py, warp/fem/_arap.py,
  * warp/fem/_stable_neo_hookean_muscle.py (SNH on F A, A = I + sym(activation);
  * the activation is bound to the mesh by apl_mesh_set_activation) */
-enum { APL_SNH = 0, APL_ARAP = 1, APL_SNH_MUSCLE = 2 };
+enum { APL_SNH = 0, APL_ARAP = 1, APL_SNH_MUSCLE = 2,
+       /* a model mesh (apl_model_create): each segment's own law, summed */
+       APL_MODEL = 3 };
 
 /* output selection for apl_elem_eval */
 enum {
@@ -79,6 +81,25 @@ int apl_mesh_create_from_points(int dtype, int64_t n_tets, int64_t n_verts, cons
                                 const double* points, const double* fraction, int64_t n_inst,
                                 const int64_t* inst_tet_off, const int64_t* inst_vtx_off, int chunks, int device,
                                 apl_mesh** out);
+/* A sum of element potentials over one vertex set: WarpModel's sum of its
+ * potentials (warp/model/_model.py:13-36), e.g. the three FEM energies of one
+ * mesh with their own law, Fraction and materials in
+ * exp/2026/01/28/smas/src/31-inverse-activation-stable-neo-hookean.py:256-279.
+ * As apl_mesh_create, plus n_seg segments per instance: segment s of
+ * instance i owns the tets seg_tet_off[i * n_seg + s] ..
+ * seg_tet_off[i * n_seg + s + 1] (host, n_inst * n_seg + 1 offsets) and
+ * evaluates the law seg_law[s] (APL_SNH, APL_ARAP, APL_SNH_MUSCLE).  Tets of
+ * different segments may repeat the same cell (each carries its own dV, e.g.
+ * its potential's Fraction).  Element passes and solvers on a model mesh take
+ * material APL_MODEL with per-tet material arrays over all its tets (lambda
+ * is read for SNH and muscle tets; the activation, bound by
+ * apl_mesh_set_activation, for muscle tets); each law runs its own element
+ * launch over its own blocks and all of them add into one vertex reduction. */
+int apl_model_create(int dtype, int64_t n_tets, int64_t n_verts, const int32_t* cells, const double* dminv,
+                     const double* dv, int64_t n_inst, const int64_t* inst_tet_off, const int64_t* inst_vtx_off,
+                     int n_seg, const int32_t* seg_law, const int64_t* seg_tet_off, int chunks, int device,
+                     apl_mesh** out);
+
 /* Read back the element data in the caller's tet order: dminv (T x 3 x 3,
  * the inverse of dX/dr = rows 1..3 of the region's dhdX) and dv (T, dV
  * including Fraction), device arrays in the mesh dtype; either may be NULL. */
@@ -134,7 +155,9 @@ int apl_elem_eval_static(apl_mesh* mesh, int material, const void* mu, const voi
                          double* hess_quad, void* dmu, void* dlam, void* stream);
 size_t apl_static_scratch_bytes(void);
 
-/* Mixed derivatives of the muscle law for the activation adjoint
+/* Mixed derivatives of the muscle law for the activation adjoint (on a model
+ * mesh: dact of its muscle-law tets, other rows untouched, and dmu / dlam of
+ * every tet)
  * (mixed_derivative_prod of exp/2026/01/28/smas/src/
  * 31-inverse-activation-stable-neo-hookean.py:472-486): per tet,
  * dact[t, k] = p^T d(grad)/d(a_k) (T, 6), and optionally dmu, dlam (T,) as
@@ -199,6 +222,35 @@ int apl_solver_destroy(apl_solver* solver);
 int apl_pncg_solve(apl_solver* solver, const void* mu, const void* lam, void* x, const void* f_ext,
                    const uint8_t* fixed_mask, const apl_pncg_params* params, apl_pncg_stats* stats, void* stream);
 
+/* Step-wise PNCG: liblaf-peach's Optimizer.init / step / terminate /
+ * postprocess as apple's experiments drive them
+ * (exp/2026/05/06/toy/src/21-smas-prestrain-stable-neo-hookean-muscle.py:497-564).
+ * apl_pncg_begin binds the arguments of apl_pncg_solve and resets the state;
+ * apl_pncg_advance runs until every running instance has taken `steps` more
+ * accepted steps (steps > 0; one trial evaluation per launch, the status read
+ * after each), or else until none runs or max_iters more trial evaluations
+ * (max_iters > 0; graph launches of iters_per_launch) have run; x_out (V,3,
+ * nullable) receives the current iterate and state (host, n_inst, nullable)
+ * the per-instance optimizer state (the reference's opt_state fields);
+ * apl_pncg_end writes the iterate into the x given to apl_pncg_begin and
+ * fills stats.  apl_pncg_solve is begin + advance to the end + end. */
+typedef struct {
+  int32_t status, n_steps, n_evals;
+  int32_t ls_steps; /* line_search_state.step: backtracks of the last accepted step */
+  int32_t ok;       /* line_search_state.ok: 0 while the last line search failed (restart) */
+  double alpha;     /* line_search_state.alpha: the last accepted step */
+  double f0;        /* line_search_state.f0: energy before it */
+  double f_alpha;   /* line_search_state.f_alpha: energy after it */
+  double grad_norm, grad_norm_first; /* convergence_state */
+  double hess_diag_mean;             /* hess_damping_state.hess_diag_mean: mean positive diag H */
+  double beta;
+} apl_pncg_state;
+int apl_pncg_begin(apl_solver* solver, const void* mu, const void* lam, void* x, const void* f_ext,
+                   const uint8_t* fixed_mask, const apl_pncg_params* params, void* stream);
+int apl_pncg_advance(apl_solver* solver, int32_t steps, int64_t max_iters, void* x_out, apl_pncg_state* state,
+                     void* stream);
+int apl_pncg_end(apl_solver* solver, apl_pncg_stats* stats, void* stream);
+
 /* Adjoint linear solve (exp/2025/08/27/inverse/src/30-inverse.py:85-93):
  * Jacobi-preconditioned CG on H(x) y = rhs over the free DOFs, H applied by
  * the unclamped Hessian-vector pass; y (V,3) is overwritten (zero at fixed
@@ -206,6 +258,14 @@ int apl_pncg_solve(apl_solver* solver, const void* mu, const void* lam, void* x,
  * f = final relative residual.  Reports APL_NON_FINITE if d^T H d <= 0. */
 int apl_cg_solve(apl_solver* solver, const void* mu, const void* lam, const void* x, const void* rhs, void* y,
                  const uint8_t* fixed_mask, double rtol, int max_iters, apl_pncg_stats* stats, void* stream);
+
+/* The adjoint solve for an indefinite Hessian: CG on the normal equations
+ * H^2 y = H rhs over the free DOFs (lineax.NormalCG, the adjoint solver of
+ * exp/2025/08/27/inverse/src/30-inverse.py:85-93), two Hessian-vector passes
+ * per iteration, Jacobi on diag(H)^2.  Arguments as apl_cg_solve; stops at
+ * ||H (rhs - H y)|| <= rtol ||H rhs|| (stats.f is that ratio). */
+int apl_cg_solve_normal(apl_solver* solver, const void* mu, const void* lam, const void* x, const void* rhs, void* y,
+                        const uint8_t* fixed_mask, double rtol, int max_iters, apl_pncg_stats* stats, void* stream);
 
 #ifdef __cplusplus
 }

@@ -65,3 +65,19 @@ def test_mesh_size_limits_rejected_before_any_device_call(n_tets, n_verts):
                     ctypes.byref(out))
         assert rc == -1 and out.value is None
         assert b"2^27" in L.apl_last_error()
+
+
+def test_fp64_tet_count_limit_of_32bit_element_offsets():
+    """FP64 meshes: the inverse rest shapes' 9 SoA rows are addressed by 32-bit
+    byte offsets, so at most 2^26 - 1024 tets; rejected before any device call."""
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    L = _lib.lib()
+    cells = (ctypes.c_int32 * 4)(0, 1, 2, 3)
+    dummy = (ctypes.c_double * 9)()
+    p = lambda a: ctypes.cast(a, ctypes.c_void_p)
+    out = ctypes.c_void_p()
+    rc = L.apl_mesh_create(_lib.APL_F64, (1 << 26) - 1023, 8, p(cells), p(dummy), p(dummy), 1, None, None, 1, 0,
+                           ctypes.byref(out))
+    assert rc == -1 and out.value is None
+    assert b"2^26" in L.apl_last_error()

@@ -16,6 +16,7 @@ import oracle.pncg as op
 from apple_amd import adjoint, batch, fem
 from apple_amd.mesh import Region, cube
 from apple_amd.solver import PncgConfig, mask_bits
+from helpers import rel
 
 pytestmark = pytest.mark.gpu
 
@@ -188,3 +189,45 @@ def test_batched_fp32_vs_fp64_forward_and_adjoint():
         g_err = np.linalg.norm(g32[st_] - g64[st_]) / np.linalg.norm(g64[st_])
         print(f"instance {i}: energy {e_err:.2e}, positions/diag {x_err:.2e}, dL/dq {g_err:.2e}")
         assert e_err <= 1e-5 and x_err <= 1e-5 and g_err <= 2e-4
+
+
+def test_normal_cg_solves_an_indefinite_hessian():
+    """Far from equilibrium the SNH Hessian is indefinite: the PCG meets
+    negative curvature and reports NON_FINITE, and CG on the normal
+    equations (apl_cg_solve_normal, lineax.NormalCG's method) solves H y = b
+    to the oracle's sparse direct solution."""
+    import scipy.sparse.linalg as spla
+
+    from apple_amd.solver import DeviceSolver, Result, mask_bits
+
+    print("[adjoint] normal CG on an indefinite Hessian", flush=True)
+    m = cube(3)
+    rng = np.random.default_rng(12)
+    mu = np.full(m.n_cells, 1e4)
+    m.cell_data["mu"] = mu
+    m.cell_data["lambda"] = 10 * mu
+    region = Region.from_mesh(m)
+    fm = np.zeros((m.n_points, 3), bool)
+    fm[m.points[:, 2] == 0] = True
+    u = 0.25 * rng.standard_normal((m.n_points, 3))
+    u[fm] = 0
+    b = rng.standard_normal((m.n_points, 3))
+    b[fm] = 0
+    dof = of.DofMap(fm, np.zeros((m.n_points, 3)))
+    prob = op.Problem(of.Model([of.FemPotential(of.SNH, region.cells, region.dhdX, region.dV, mu, 10 * mu)]), dof)
+    H = prob.hessian(dof.to_free(u))
+    ev = np.linalg.eigvalsh(H.toarray())
+    assert ev.min() < 0 < ev.max(), "the test state must have an indefinite Hessian"
+    y_ref = dof.to_full_grad(spla.spsolve(H.tocsc(), dof.to_free(b)))
+    pot = fem.StableNeoHookean.from_region(region, dtype=torch.float64, device="cuda")
+    solver = DeviceSolver(pot)
+    ut = torch.as_tensor(u, device="cuda")
+    bt = torch.as_tensor(b, device="cuda")
+    mask = mask_bits(fm, "cuda")
+    y = torch.zeros_like(ut)
+    st = solver.cg(ut, bt, y, mask, rtol=1e-10, max_iters=20000)[0]
+    assert st.result == Result.NON_FINITE
+    y = torch.zeros_like(ut)
+    st = solver.cg(ut, bt, y, mask, rtol=1e-12, max_iters=200000, normal=True)[0]
+    assert st.result == Result.PRIMARY_SUCCESS, st
+    assert rel(y.cpu().numpy(), y_ref) <= 1e-6

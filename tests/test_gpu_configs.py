@@ -1,0 +1,179 @@
+"""The benchmark's five workloads at their own sizes, on the GPU.
+
+Each compares the FP32 product path against a reference that does not
+start from the GPU's FP32 answer, and reports both the ell-2 and the
+max-over-entries errors:
+
+1. 10^3 cube, SNH, PNCG to 1e-6: the oracle's Newton equilibrium (CPU, sparse
+   direct solves) from the same seeded x0.
+2. 64^3 cube (1.31M tets), SNH and ARAP: an FP64 device solve from the same
+   x0 to rtol 1e-10, Newton-polished.
+3. 256 cubes, n ~ U{6..32}: every instance against the same batch solved in
+   FP64 to rtol 1e-10.
+4. 32^3 differentiation: dL/dmu of the FP32 forward + adjoint (refined)
+   against the polished FP64 adjoint and FP64 central differences on 64 tets.
+5. 64 cubes, n ~ U{16..48}, mu ~ U[1e3, 1e5]: FP32 forward + adjoint,
+   refined, against the all-FP64 run, per instance.
+
+"Refined" is the FP32 path followed by mixed-precision iterative refinement
+(FP64 residual passes, FP32 PCG corrections; apple_amd.refine) to the FP64
+reference's tolerance: at rtol 1e-6 the positions of ill-conditioned
+instances sit up to ~2e-5 of the diagonal from the rtol-1e-10 equilibrium in
+FP64 as well, so the 1e-5 position target needs the tighter solve.
+"""
+
+import os
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+import oracle.fem as of
+import oracle.pncg as op
+from apple_amd import _lib, adjoint, batch, fem, refine
+from apple_amd.mesh import Region, cube
+from apple_amd.solver import DeviceSolver, PncgConfig, Result, mask_bits
+
+sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+import bench  # noqa: E402  (the workload generators of the benchmark)
+
+pytestmark = pytest.mark.gpu
+DIAG = np.sqrt(3.0)
+
+
+def _errs(x, x_ref):
+    d = np.abs(np.asarray(x, np.float64) - np.asarray(x_ref, np.float64))
+    return float(d.max()) / DIAG, float(np.linalg.norm(d) / max(np.linalg.norm(x_ref), 1e-300))
+
+
+def _refine_single(pot, solver, x, f64, mask, g0, rtol, config=None):
+    """FP64-residual refinement of one FP32 equilibrium, continued by the FP64
+    PNCG when the corrections stall (ARAP's projected Hessian)."""
+    norms = refine.InstanceNorms(pot.mesh.inst_vtx_off, x.device)
+    shadow = pot.fp64_shadow()
+    target = np.array([rtol * g0])
+    x64, rep = refine.refine_equilibrium(solver, shadow, x, f64, mask, target, norms)
+    if not rep.converged.all():
+        cfg = config or PncgConfig(max_steps=200000, stall_steps=20000)
+        st, _ = refine.finish_fp64(shadow, x64, f64, mask, target, [0], norms, cfg)
+        assert st[0].success, st[0]
+        print(f"  refinement stalled after {rep.steps} corrections; FP64 PNCG continued {st[0].n_steps} steps")
+    return x64
+
+
+def test_config1_10cube_matches_oracle_newton_from_the_same_start():
+    print("[config 1] 10^3 SNH", flush=True)
+    m = cube(10)
+    m.cell_data["mu"] = np.full(m.n_cells, 1e4)
+    m.cell_data["lambda"] = np.full(m.n_cells, 1e5)
+    region = Region.from_mesh(m)
+    f = np.zeros((m.n_points, 3))
+    np.add.at(f[:, 2], m.cells.reshape(-1), np.repeat(-9.8e3 * region.dV / 4.0, 4))
+    fixed = np.zeros((m.n_points, 3), bool)
+    fixed[m.points[:, 2] == 0.0] = True
+    x0 = 0.01 * np.random.default_rng(0).standard_normal((m.n_points, 3))
+    x0[fixed] = 0.0
+    dof = of.DofMap(fixed, np.zeros_like(f))
+    prob = op.Problem(of.Model([of.FemPotential(of.SNH, region.cells, region.dhdX, region.dV, m.cell_data["mu"],
+                                                m.cell_data["lambda"]), of.ExternalForce(np.arange(m.n_points), f)]),
+                      dof)
+    x_ref = dof.to_full(op.newton(prob, dof.to_free(x0), tol=1e-9, max_steps=60))
+    assert np.linalg.norm(prob.grad(dof.to_free(x_ref))) <= 1e-8 * np.linalg.norm(prob.grad(dof.to_free(x0)))
+    mask = mask_bits(fixed, "cuda")
+    for dtype, rtol, tol in ((torch.float32, 1e-6, 1e-5), (torch.float64, 1e-10, 1e-9)):
+        pot = fem.StableNeoHookean.from_region(region, dtype=dtype, device="cuda")
+        solver = DeviceSolver(pot)
+        x = torch.as_tensor(x0, dtype=dtype, device="cuda").contiguous()
+        ft = torch.as_tensor(f, dtype=dtype, device="cuda")
+        st = solver.solve(x, ft, mask, PncgConfig(rtol=rtol, max_steps=20000))[0]
+        assert st.result == Result.PRIMARY_SUCCESS, st
+        emax, e2 = _errs(x.double().cpu().numpy(), x_ref)
+        print(f"  {dtype}: {st.n_steps} steps, positions max {emax:.2e} of the diagonal, l2 {e2:.2e}")
+        assert emax <= tol
+        if dtype == torch.float32:
+            x64 = _refine_single(pot, solver, x, torch.as_tensor(f, device="cuda"), mask, st.grad_norm_first, 1e-10)
+            emax_r, _ = _errs(x64.cpu().numpy(), x_ref)
+            print(f"  FP32 refined: positions max {emax_r:.2e}")
+            assert emax_r <= 1e-8
+
+
+@pytest.mark.parametrize("material", ["snh", "arap"])
+def test_config2_64cube_fp32_vs_polished_fp64(material):
+    print(f"[config 2] 64^3 {material}", flush=True)
+    n = 64
+    p32 = bench.make_cube_problem(n, material, torch.float32, "cuda")
+    p64 = bench.make_cube_problem(n, material, torch.float64, "cuda")
+    mask = mask_bits(p32["fixed"], "cuda")
+    s32, s64 = DeviceSolver(p32["pot"]), DeviceSolver(p64["pot"])
+    steps = 60000 if material == "arap" else 20000
+    x32 = p32["x0"].clone()
+    st = s32.solve(x32, p32["f"], mask, PncgConfig(rtol=1e-6, max_steps=steps))[0]
+    assert st.result == Result.PRIMARY_SUCCESS, st
+    x64 = p64["x0"].clone()
+    st64 = s64.solve(x64, p64["f"], mask, PncgConfig(rtol=1e-10, max_steps=4 * steps, stall_steps=5000))[0]
+    pot64 = p64["pot"]
+    mats = pot64.materials
+    x64 = bench.newton_polish(pot64, s64, mats["mu"], mats.get("lambda"), x64, p64["f"], mask)
+    g = torch.zeros_like(x64)
+    pot64.mesh.eval(pot64.material, mats["mu"], mats.get("lambda"), x64, grad=g)
+    r = torch.where(refine._free(mask, g), p64["f"] - g, torch.zeros_like(g))
+    assert float(torch.linalg.norm(r)) <= 1e-9 * st.grad_norm_first, (st64, float(torch.linalg.norm(r)))
+    ref = x64.cpu().numpy()
+    emax, e2 = _errs(x32.double().cpu().numpy(), ref)
+    print(f"  FP32 rtol 1e-6 ({st.n_steps} steps): positions max {emax:.2e} of the diagonal, l2 {e2:.2e}")
+    xr = _refine_single(p32["pot"], s32, x32, p64["f"], mask, st.grad_norm_first, 1e-10)
+    emax_r, e2_r = _errs(xr.cpu().numpy(), ref)
+    print(f"  refined: max {emax_r:.2e}, l2 {e2_r:.2e}")
+    assert emax <= 1e-2  # the rtol-1e-6 answer: tolerance-limited
+    assert emax_r <= 1e-5
+
+
+def test_config3_batch_of_256_each_instance_vs_fp64():
+    print("[config 3] 256 cubes", flush=True)
+    insts, sizes = bench.make_cube_instances(256, 6, 32, 2, 3)
+    b32 = batch.Batch(insts, _lib.APL_SNH, dtype=torch.float32, device="cuda")
+    b64 = batch.Batch(insts, _lib.APL_SNH, dtype=torch.float64, device="cuda")
+    x64, st64 = b64.solve(config=PncgConfig(rtol=1e-10, max_steps=200000, stall_steps=5000))
+    x32, st32 = b32.solve(config=PncgConfig(rtol=1e-6, max_steps=50000), refine_fp64="newton", refine_rtol=1e-10)
+    ref = b64.split(x64)
+    got = b32.split(b32.x64)
+    errs = np.array([_errs(a.cpu().numpy(), r.cpu().numpy())[0] for a, r in zip(got, ref)])
+    conv64 = np.array([s.success for s in st64])
+    conv32 = np.array([s.success for s in st32])  # the FP64 residual of the refined FP32 answer met 1e-10 g0
+    print(f"  FP64 converged {conv64.sum()}/256, refined FP32 {conv32.sum()}/256 (FP64 residuals); positions "
+          f"max {errs.max():.2e}, median {np.median(errs):.2e}, within 1e-5: {(errs <= 1e-5).sum()}/256; "
+          f"FP64 PNCG continuations {b32.refined}")
+    assert conv64.all() and conv32.all()
+    # both answers are FP64 equilibria: a position gap above 1e-5 is another
+    # equilibrium (a buckled branch of a soft, heavily loaded cube)
+    switches = [(i, sizes[i], float(errs[i]), (st32[i].f - st64[i].f) / abs(st64[i].f))
+                for i in range(256) if errs[i] > 1e-5]
+    print(f"  other equilibria (instance, n, position gap, relative energy gap): {switches}")
+    assert len(switches) <= 13  # at most 5 % of the instances
+    on = errs <= 1e-5
+    assert errs[on].max() <= 1e-6
+
+
+def test_config4_differentiation_32cube_64_fd_tets():
+    print("[config 4] 32^3 differentiation", flush=True)
+    out = bench.bench_differentiation(torch.device("cuda", 0), n=32, n_fd=64)
+    print(f"  {out}")
+    assert out["fp64_reference_adjoint"]["norm_rel_err"] <= 1e-6
+    assert out["fp64"]["norm_rel_err"] <= 1e-6
+    assert out["fp32_refined"]["vs_fp64_reference_norm_rel_err"] <= 1e-4
+    assert out["fp32_refined"]["norm_rel_err"] <= 1e-4
+
+
+def test_config5_combined_fp32_refined_vs_all_fp64():
+    print("[config 5] 64 cubes, forward + adjoint", flush=True)
+    out = bench.bench_combined(torch.device("cuda", 0))
+    r = out["fp32_refined"]
+    print(f"  FP32 at rtol 1e-6: {out['fp32']}\n  refined: { {k: v for k, v in r.items() if k != 'branch_switches'} }\n"
+          f"  branch switches: {r['branch_switches']}")
+    assert out["fp64_converged"] == 64
+    # instances on the FP64 run's branch meet the targets; branch switches are listed
+    assert r["on_branch_energy_max"] <= 1e-5
+    assert r["on_branch_pos_max"] <= 1e-5
+    assert r["on_branch_grad_max"] <= 1e-4
+    assert r["on_fp64_branch"] >= 60

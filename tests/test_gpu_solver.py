@@ -196,8 +196,35 @@ def test_failure_codes():
     assert st.result == Result.NON_FINITE
 
 
-def test_host_driven_pncg_for_general_models():
-    """Two FEM potentials on one mesh: the host-driven PNCG path, same equilibrium."""
+class _LoadPotential:
+    """A potential the fused path does not know (the reference protocol over
+    an ExternalForce): forces the host-driven PNCG."""
+
+    def __init__(self, ext):
+        self.ext, self.name, self.dtype, self.device = ext, "load", ext.dtype, ext.device
+        self.n_points = ext.n_points
+
+    def fun(self, u, output):
+        self.ext.fun(u, output)
+
+    def grad(self, u, output):
+        self.ext.grad(u, output)
+
+    def hess_diag(self, u, output):
+        pass
+
+    def hess_prod(self, u, p, output):
+        pass
+
+    def hess_quad(self, u, p, output):
+        pass
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_two_fem_potentials_fused_and_host_driven(fused):
+    """Two FEM potentials on one mesh: the fused device model (DeviceModel)
+    and, with a potential the fused path does not know, the host-driven PNCG
+    reach the oracle's equilibrium."""
     m, region, mu, lam, f, fm, x0 = _gravity_case(4, "snh", seed=3)
     m.cell_data["mu"] = 0.5 * mu
     m.cell_data["lambda"] = 0.5 * lam
@@ -210,9 +237,10 @@ def test_host_driven_pncg_for_general_models():
     b2 = fem.StableNeoHookean.from_region(region, dtype=torch.float64, device="cuda")
     builder.add_potential(a)
     builder.add_potential(b2)
-    builder.add_potential(potential.ExternalForce(np.arange(m.n_points), f, m.n_points, dtype=torch.float64, device="cuda"))
+    ext = potential.ExternalForce(np.arange(m.n_points), f, m.n_points, dtype=torch.float64, device="cuda")
+    builder.add_potential(ext if fused else _LoadPotential(ext))
     model = builder.finalize()
-    assert model.fused_parts() is None
+    assert (model.fused_parts() is None) == (not fused)
     fwd = forward.Forward(model, forward.PNCG(rtol=1e-8, max_steps=5000))
     sol = fwd.step()
     assert sol.success
