@@ -503,6 +503,80 @@ __host__ __device__ inline Svd<R> svd_rv_refined(const M3<R>& G) {
   return S;
 }
 
+// The same SVD with FP32 arithmetic except one step: F^T F - I is formed
+// in FP64 from the FP32 G (exact products, one rounding to FP32), which
+// removes the cancellation that G + G^T + G^T G suffers in FP32 under a
+// large rotation (|G| ~ 1 while the strain is small).  Then FP32 Jacobi
+// sweeps with hardware-approximate rotations, the FP32 QR of F V for U and
+// the FP32 singular values (sigma - 1 from the eigenvalue near 1).  The
+// stress mu U diag(sigma - 1) V^T and the twist-mode projection are smooth
+// functions of F^T F, so the eigenvectors' FP32 error within nearly
+// degenerate pairs does not reach them.
+#ifndef E32_SWEEPS
+#define E32_SWEEPS 4
+#endif
+// F^T F - I in FP64 from the FP32 G, rounded once to FP32
+__host__ __device__ inline void ftf_minus_i_f64(const M3<float>& G, float (&a)[3][3]) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = i; j < 3; ++j) {
+      const double gtg = (double)G.m[0][i] * G.m[0][j] + (double)G.m[1][i] * G.m[1][j] +
+                         (double)G.m[2][i] * G.m[2][j];
+      a[i][j] = a[j][i] = (float)((double)G.m[i][j] + (double)G.m[j][i] + gtg);
+    }
+}
+
+__host__ __device__ inline Svd<float> svd_rv_e32_from(const M3<float>& G, float (&a)[3][3]) {
+  M3<float> V;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) V.m[i][j] = (i == j) ? 1.0f : 0.0f;
+  for (int sw = 0; sw < E32_SWEEPS; ++sw) {
+    jacobi_rotate_fast(a, V, 0, 1);
+    jacobi_rotate_fast(a, V, 0, 2);
+    jacobi_rotate_fast(a, V, 1, 2);
+  }
+  float e[3];
+  sort_eig(a, V, e);
+  return qr_sigma(G, V, e);
+}
+
+__host__ __device__ inline Svd<float> svd_rv_e32(const M3<float>& G) {
+  float a[3][3];
+  ftf_minus_i_f64(G, a);
+  return svd_rv_e32_from(G, a);
+}
+
+// The ARAP kernels' SVD: svd_rv_e32 for every element with det F > 1/64
+// (large rotations and large strains included), svd_rv_refined for the
+// inverted and nearly flat ones, where the rotation-variant sign convention
+// hinges on the sign of det F and on comparisons of nearly equal |sigma|
+// that FP32 can get the other way.  The threshold keeps the FP32 det3's
+// rounding (~1e-7 |F|^3) far from deciding the branch; one branch per
+// element keeps a wave on one path unless it holds an inverted element.
+__host__ __device__ inline Svd<float> svd_rv_fast(const M3<float>& G) {
+  M3<float> F = G;
+  F.m[0][0] += 1.0f;
+  F.m[1][1] += 1.0f;
+  F.m[2][2] += 1.0f;
+  if (det3(F) > 0.015625f) {
+    float a[3][3];
+    ftf_minus_i_f64(G, a);
+    return svd_rv_e32_from(G, a);
+  }
+  return svd_rv_refined<float>(G);
+}
+
+template <class R>
+__host__ __device__ inline Svd<R> arap_svd(const M3<R>& G) {
+  if constexpr (sizeof(R) == 4)
+    return svd_rv_fast(G);
+  else
+    return svd_rv<R>(G);
+}
+
 // U diag(d) V^T
 template <class R>
 __host__ __device__ inline M3<R> udv(const Svd<R>& S, const R d[3]) {
@@ -725,7 +799,7 @@ __host__ __device__ inline void tet_eval(const TetGeom<R>& g0, R mu, R lam, cons
       }
     }
   } else {  // ARAP
-    const Svd<R> S = sizeof(R) == 4 ? svd_rv_refined<R>(G) : svd_rv<R>(G);
+    const Svd<R> S = arap_svd(G);
     if constexpr ((WHAT & W_E) != 0)
       o.e = R(0.5) * muV * (S.sm1[0] * S.sm1[0] + S.sm1[1] * S.sm1[1] + S.sm1[2] * S.sm1[2]);
     if constexpr ((WHAT & W_G) != 0) {

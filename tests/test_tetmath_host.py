@@ -1,5 +1,5 @@
 """The per-tet algebra compiled for the host (tetmath.h is host/device code):
-the refined ARAP SVD of the FP32 kernels against the plain FP64 SVD.  The
+the ARAP SVDs of the FP32 kernels against the plain FP64 SVD.  The
 reference's Warp svd3 cannot run here; its conventions are pinned by the
 oracle tests (rotation-variant signs, ordering), this pins the build's own
 fast path against the FP64 one."""
@@ -25,13 +25,24 @@ def harness(tmp_path_factory):
     return exe
 
 
-def test_refined_svd_matches_fp64_svd(harness):
+def test_kernel_svds_match_fp64_svd(harness):
     out = subprocess.run([harness], check=True, capture_output=True, text=True).stdout
     rows = np.array([[float(x) for x in line.split()] for line in out.strip().splitlines()])
-    assert rows.shape == (8, 6)
-    p_err, e_err, s_err, orth = rows[:, 2], rows[:, 3], rows[:, 4], rows[:, 5]
-    # FP32 outputs: errors at the level of FP32 rounding of the results
-    assert p_err.max() <= 1e-6
-    assert e_err.max() <= 1e-6
-    assert s_err.max() <= 1e-6
-    assert orth.max() <= 1e-6
+    assert rows.shape == (22, 8)
+    which, mode, scale = rows[:, 0], rows[:, 1], rows[:, 2]
+    p_err, e_err, s_err, orth, q_err = rows[:, 3], rows[:, 4], rows[:, 5], rows[:, 6], rows[:, 7]
+    fast = which == 1
+    # the kernels' SVD (svd_rv_fast): FP32-level errors everywhere, large
+    # rotations with small strains included
+    assert p_err[fast].max() <= 5e-6
+    assert e_err[fast].max() <= 5e-6
+    # absolute: the singular values reach ~4 at scale 1, where the FP32
+    # eigenvalue of F^T F carries a few ulp of its largest entry
+    assert s_err[fast & (scale < 1)].max() <= 1e-6
+    assert s_err[fast].max() <= 4e-6
+    assert orth[fast].max() <= 1e-6
+    assert q_err[fast].max() <= 2e-6
+    # the FP32-basis refinement alone loses the stress of rotated elements
+    # with strains near 1e-5 (F^T F - I cancels in its FP32 basis search)
+    rot_small = (which == 0) & (mode == 2) & (scale == 1e-5)
+    assert p_err[rot_small].max() > 1e-5
