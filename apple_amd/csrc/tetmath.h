@@ -277,6 +277,13 @@ __host__ __device__ inline float fast_rsq(float x) {
 #endif
 }
 
+// 1/x and 1/sqrt(x): the hardware approximations (about 1 ulp) in FP32,
+// correctly rounded division and square root in FP64
+__host__ __device__ inline float recip(float x) { return fast_rcp(x); }
+__host__ __device__ inline double recip(double x) { return 1.0 / x; }
+__host__ __device__ inline float rsqrt_r(float x) { return fast_rsq(x); }
+__host__ __device__ inline double rsqrt_r(double x) { return 1.0 / sqrt(x); }
+
 __host__ __device__ inline void jacobi_rotate_fast(float (&a)[3][3], M3<float>& V, int p, int q) {
   const float apq = a[p][q];
   const float scale = fabsf(a[p][p]) + fabsf(a[q][q]);
@@ -321,7 +328,7 @@ __host__ __device__ inline void swap_cols(M3<R>& V, R (&e)[3], int i, int j) {
 template <class R>
 __host__ __device__ inline void sigma_from(R e, R q, R& s, R& sm1) {
   if (fabs(e) < R(0.25) && q > R(0)) {
-    sm1 = e / (R(1) + sqrt(R(1) + e));
+    sm1 = e * recip(R(1) + sqrt(R(1) + e));
     s = R(1) + sm1;
   } else {
     s = q;
@@ -384,23 +391,29 @@ __host__ __device__ inline Svd<R> qr_sigma(const M3<R>& G, const M3<R>& V0, cons
   V3<R> f0 = v0 + mul(G, v0);
   V3<R> f1 = v1 + mul(G, v1);
   V3<R> f2 = v2 + mul(G, v2);
-  R n0 = sqrt(dot(f0, f0));
-  V3<R> u0;
-  if (n0 > R(1e-30)) {
-    u0 = (R(1) / n0) * f0;
-  } else {
-    u0 = v0;
+  // squared-norm floors: normal numbers in either precision
+  const R tiny = sizeof(R) == 4 ? R(1e-36) : R(1e-60);
+  const R d0 = dot(f0, f0);
+  R n0 = R(0);
+  V3<R> u0 = v0;
+  if (d0 > tiny) {
+    const R r0 = rsqrt_r(d0);
+    n0 = d0 * r0;
+    u0 = r0 * f0;
   }
   V3<R> w = f1 - dot(u0, f1) * u0;
-  R n1 = sqrt(dot(w, w));
+  const R d1 = dot(w, w);
+  R n1 = R(0);
   V3<R> u1;
-  if (n1 > R(1e-30) * (R(1) + n0)) {
-    u1 = (R(1) / n1) * w;
+  if (d1 > tiny * (R(1) + n0) * (R(1) + n0)) {
+    const R r1 = rsqrt_r(d1);
+    n1 = d1 * r1;
+    u1 = r1 * w;
   } else {
     // any unit vector orthogonal to u0
     V3<R> t = fabs(u0.x) < R(0.6) ? v3(R(1), R(0), R(0)) : v3(R(0), R(1), R(0));
     w = t - dot(u0, t) * u0;
-    u1 = (R(1) / sqrt(dot(w, w))) * w;
+    u1 = rsqrt_r(dot(w, w)) * w;
   }
   V3<R> u2 = cross(u0, u1);
   set_col(r.U, 0, u0);
@@ -440,32 +453,18 @@ __host__ __device__ inline Svd<R> svd_rv(const M3<R>& G) {
 #ifndef REFINE_F64_SWEEPS
 #define REFINE_F64_SWEEPS 1
 #endif
+// FP64 finish from an FP32 eigenbasis Vf of F^T F - I: re-orthonormalise
+// Vf in FP64, rotate the FP64 F^T F - I into it (nearly diagonal), one FP64
+// sweep, then the FP64 sort and QR; results rounded to R.
 template <class R>
-__host__ __device__ inline Svd<R> svd_rv_refined(const M3<R>& G) {
-  M3<float> Gf;
+__host__ __device__ inline Svd<R> svd_finish_f64(const M3<R>& G, const M3<float>& Vf) {
   M3<double> Gd;
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      Gf.m[i][j] = (float)G.m[i][j];
-      Gd.m[i][j] = (double)G.m[i][j];
-    }
-  float af[3][3];
-  ftf_minus_i(Gf, af);
-  M3<float> Vf;
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) Vf.m[i][j] = (i == j) ? 1.0f : 0.0f;
-  for (int sw = 0; sw < REFINE_F32_SWEEPS; ++sw) {
-    jacobi_rotate_fast(af, Vf, 0, 1);
-    jacobi_rotate_fast(af, Vf, 0, 2);
-    jacobi_rotate_fast(af, Vf, 1, 2);
-  }
+    for (int j = 0; j < 3; ++j) Gd.m[i][j] = (double)G.m[i][j];
   double ad[3][3], t[3][3], b[3][3];
   ftf_minus_i(Gd, ad);
-  // re-orthonormalise the FP32 basis in FP64 (Gram-Schmidt, det +1)
   V3<double> c0 = v3((double)Vf.m[0][0], (double)Vf.m[1][0], (double)Vf.m[2][0]);
   V3<double> c1 = v3((double)Vf.m[0][1], (double)Vf.m[1][1], (double)Vf.m[2][1]);
   c0 = (1.0 / sqrt(dot(c0, c0))) * c0;
@@ -503,6 +502,28 @@ __host__ __device__ inline Svd<R> svd_rv_refined(const M3<R>& G) {
   return S;
 }
 
+template <class R>
+__host__ __device__ inline Svd<R> svd_rv_refined(const M3<R>& G) {
+  M3<float> Gf;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) Gf.m[i][j] = (float)G.m[i][j];
+  float af[3][3];
+  ftf_minus_i(Gf, af);
+  M3<float> Vf;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) Vf.m[i][j] = (i == j) ? 1.0f : 0.0f;
+  for (int sw = 0; sw < REFINE_F32_SWEEPS; ++sw) {
+    jacobi_rotate_fast(af, Vf, 0, 1);
+    jacobi_rotate_fast(af, Vf, 0, 2);
+    jacobi_rotate_fast(af, Vf, 1, 2);
+  }
+  return svd_finish_f64(G, Vf);
+}
+
 // The same SVD with FP32 arithmetic except one step: F^T F - I is formed
 // in FP64 from the FP32 G (exact products, one rounding to FP32), which
 // removes the cancellation that G + G^T + G^T G suffers in FP32 under a
@@ -527,8 +548,8 @@ __host__ __device__ inline void ftf_minus_i_f64(const M3<float>& G, float (&a)[3
     }
 }
 
-__host__ __device__ inline Svd<float> svd_rv_e32_from(const M3<float>& G, float (&a)[3][3]) {
-  M3<float> V;
+// FP32 Jacobi sweeps on a (FP32 F^T F - I, diagonalised in place) from V = I
+__host__ __device__ inline void jacobi_fast_sweeps(float (&a)[3][3], M3<float>& V) {
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -538,35 +559,41 @@ __host__ __device__ inline Svd<float> svd_rv_e32_from(const M3<float>& G, float 
     jacobi_rotate_fast(a, V, 0, 2);
     jacobi_rotate_fast(a, V, 1, 2);
   }
-  float e[3];
-  sort_eig(a, V, e);
-  return qr_sigma(G, V, e);
 }
 
 __host__ __device__ inline Svd<float> svd_rv_e32(const M3<float>& G) {
   float a[3][3];
   ftf_minus_i_f64(G, a);
-  return svd_rv_e32_from(G, a);
+  M3<float> V;
+  jacobi_fast_sweeps(a, V);
+  float e[3];
+  sort_eig(a, V, e);
+  return qr_sigma(G, V, e);
 }
 
-// The ARAP kernels' SVD: svd_rv_e32 for every element with det F > 1/64
-// (large rotations and large strains included), svd_rv_refined for the
-// inverted and nearly flat ones, where the rotation-variant sign convention
-// hinges on the sign of det F and on comparisons of nearly equal |sigma|
-// that FP32 can get the other way.  The threshold keeps the FP32 det3's
-// rounding (~1e-7 |F|^3) far from deciding the branch; one branch per
-// element keeps a wave on one path unless it holds an inverted element.
+// The ARAP kernels' SVD.  Every element gets the FP32 basis (F^T F - I
+// formed in FP64, FP32 Jacobi sweeps); elements with det F > 1/64 (large
+// rotations and large strains included) finish in FP32 (sort, QR), the
+// inverted and nearly flat ones in FP64 from that basis (svd_finish_f64):
+// there the rotation-variant sign convention hinges on the sign of det F and
+// on comparisons of nearly equal |sigma| that FP32 can get the other way.
+// The threshold keeps the FP32 det3's rounding (~1e-7 |F|^3) far from
+// deciding the branch; only the finish is divergent within a wave.
 __host__ __device__ inline Svd<float> svd_rv_fast(const M3<float>& G) {
+  float a[3][3];
+  ftf_minus_i_f64(G, a);
+  M3<float> V;
+  jacobi_fast_sweeps(a, V);
   M3<float> F = G;
   F.m[0][0] += 1.0f;
   F.m[1][1] += 1.0f;
   F.m[2][2] += 1.0f;
   if (det3(F) > 0.015625f) {
-    float a[3][3];
-    ftf_minus_i_f64(G, a);
-    return svd_rv_e32_from(G, a);
+    float e[3];
+    sort_eig(a, V, e);
+    return qr_sigma(G, V, e);
   }
-  return svd_rv_refined<float>(G);
+  return svd_finish_f64(G, V);
 }
 
 template <class R>
@@ -594,9 +621,9 @@ __host__ __device__ inline M3<R> udv(const Svd<R>& S, const R d[3]) {
 template <class R>
 __host__ __device__ inline void arap_lambdas(const R s[3], R lam[3]) {
   R p0 = s[0] + s[1], p1 = s[1] + s[2], p2 = s[2] + s[0];
-  lam[0] = R(2) / (p0 > R(2) ? p0 : R(2));
-  lam[1] = R(2) / (p1 > R(2) ? p1 : R(2));
-  lam[2] = R(2) / (p2 > R(2) ? p2 : R(2));
+  lam[0] = R(2) * recip(p0 > R(2) ? p0 : R(2));
+  lam[1] = R(2) * recip(p1 > R(2) ? p1 : R(2));
+  lam[2] = R(2) * recip(p2 > R(2) ? p2 : R(2));
 }
 
 // Twist-mode coordinates of a vector d: (Q0 d, Q1 d, Q2 d) (func/_misc.py:56-70)
