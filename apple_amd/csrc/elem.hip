@@ -734,6 +734,17 @@ int nloc_limit(int dtype) {
 // blocks one launch of law MAT visits (model meshes: the law's own blocks)
 inline int64_t launch_blocks(const apl_mesh* m, int mat) { return m->model ? m->law_nblk[mat] : m->L.nblocks; }
 
+// Default element grid per pass, percent of occupancy x CUs: one resident
+// round of persistent workgroups, except for the VALU-heavy ARAP gradient
+// passes (4 waves per SIMD) and the Hessian-vector pass, which run faster
+// with a second, non-resident round that the hardware dispatches as
+// workgroups of the first finish (a dynamic balance of the last blocks;
+// measured on the 64^3 cube: ARAP pass A 58.6 -> 55.1 us, SNH H v 30.3 ->
+// 27.4 us, while SNH pass A and the quadratic-form passes are fastest at 100).
+constexpr int default_grid_pct(int mat, unsigned what) {
+  return ((what & W_HP) != 0 || (mat == MAT_ARAP && (what & (W_G | W_H)) != 0)) ? 200 : 100;
+}
+
 template <int MAT, unsigned WHAT, class R, class X, int SPT>
 static int grid_for_spt(apl_mesh* m, bool filtered) {
   static_assert(WHAT < 128, "pass variant index");
@@ -744,7 +755,8 @@ static int grid_for_spt(apl_mesh* m, bool filtered) {
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, elem_kernel<MAT, WHAT, R, X, SPT>, NTH, smem) != hipSuccess ||
       occ < 1)
     occ = 1;
-  long long g = (long long)occ * m->n_cu * m->grid_pct / 100;
+  const int pct = m->grid_pct > 0 ? m->grid_pct : default_grid_pct(MAT, WHAT);
+  long long g = (long long)occ * m->n_cu * pct / 100;
   long long nb8 = ((launch_blocks(m, MAT) + 7) / 8) * 8;
   if (g > nb8) g = nb8;
   g = ((g + 7) / 8) * 8;

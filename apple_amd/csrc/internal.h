@@ -68,7 +68,7 @@ struct apl_mesh {
   size_t device_bytes = 0;
   int grid_cache[3][512] = {};  // persistent grid per (material, pass variant, FP64 displacements, filtered)
   int refs = 1;                 // the caller's handle plus one per solver built on the mesh
-  int grid_pct = 100;           // persistent element grid, percent of occupancy x CUs (autotuned)
+  int grid_pct = 0;             // persistent element grid, percent of occupancy x CUs (autotuned; 0: per-pass default)
   const void* activation = nullptr;  // caller-owned (T, 6) muscle activation, read at every muscle launch
   // model meshes (apl_model_create): several potentials' tets over one
   // vertex set, in segments of one material law each
