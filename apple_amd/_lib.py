@@ -19,6 +19,24 @@ APL_SNH, APL_ARAP, APL_SNH_MUSCLE, APL_MODEL = 0, 1, 2, 3
 APL_ENERGY, APL_GRAD, APL_HESS_DIAG, APL_HESS_PROD, APL_HESS_QUAD, APL_MIXED = 1, 2, 4, 8, 16, 32
 APL_ACCUMULATE = 1 << 16
 
+# layout arrays (apl_mesh_layout): name -> (id, numpy dtype, entries as a function of the mesh info)
+LAYOUT_ARRAYS = {
+    "corner": (0, "uint16"),
+    "blk_tet_off": (1, "int32"),
+    "blk_loc_off": (2, "int32"),
+    "blk_inst": (3, "int32"),
+    "inst_blk_off": (4, "int64"),
+    "loc_gid": (5, "int32"),
+    "blk_co_off": (6, "int32"),
+    "chunk_off": (7, "uint16"),
+    "vtx_slot_off": (8, "int32"),
+    "loc_dst": (9, "int32"),
+    "vblk_off": (10, "int32"),
+    "vblk_inst": (11, "int32"),
+    "inst_vblk_off": (12, "int64"),
+}
+LAYOUT_DEVICE, LAYOUT_HOST = 0, 1
+
 # solver result codes
 APL_RUNNING, APL_PRIMARY_SUCCESS, APL_MAX_STEPS_REACHED, APL_NON_FINITE, APL_STAGNATION = 0, 1, 2, 3, 4
 
@@ -118,6 +136,10 @@ def _declare(lib):
     lib.apl_mesh_destroy.argtypes = [c_vp]
     lib.apl_mesh_info.argtypes = [c_vp, ctypes.POINTER(MeshInfo)]
     lib.apl_mesh_set_schedule.argtypes = [c_vp, ctypes.c_int]
+    lib.apl_layout_mode.argtypes = [ctypes.c_int]
+    lib.apl_mesh_layout.argtypes = [c_vp, ctypes.c_int, c_vp, c_i64]
+    lib.apl_morton_order.argtypes = [c_vp, c_i64, c_vp, c_i64, c_vp, c_vp, ctypes.c_int, c_vp]
+    lib.apl_mesh_lumped_volume.argtypes = [c_vp, c_vp, c_vp, c_vp]
     lib.apl_elem_eval.argtypes = [
         c_vp, ctypes.c_int, c_vp, c_vp, c_vp, c_vp, ctypes.c_uint, c_dp, c_vp, c_vp, c_vp, c_dp, c_vp, c_vp, c_vp,
     ]

@@ -14,6 +14,8 @@ namespace apl {
 
 static thread_local std::string g_err;
 void set_error(const std::string& s) { g_err = s; }
+// 0: block partitions on the device where supported, 1: on the host
+static int g_layout_mode = 0;
 
 template <class T>
 static int upload(apl_mesh* m, T** dst, const T* src, size_t n) {
@@ -84,6 +86,16 @@ extern "C" {
 
 const char* apl_last_error(void) { return g_err.c_str(); }
 
+int apl_layout_mode(int mode) {
+  if (mode != 0 && mode != 1) {
+    set_error("layout mode must be 0 (device) or 1 (host)");
+    return -1;
+  }
+  const int prev = g_layout_mode;
+  g_layout_mode = mode;
+  return prev;
+}
+
 int apl_version(void) { return 10000; }
 
 }  // extern "C"
@@ -122,11 +134,18 @@ static int create_common(int dtype, int64_t n_tets, int64_t n_verts, const int32
     set_error("cells must be non-NULL");
     return -1;
   }
-  for (int64_t i = 0; i < 4 * n_tets; ++i)
-    if (cells[i] < 0 || cells[i] >= n_verts) {
-      set_error("cell index out of range [0, n_verts)");
-      return -1;
-    }
+  // LDS budget of the element workgroups: FP64 blocks hold at most 478 slots and one chunk
+  const int nloc_cap = nloc_limit(dtype);
+  if (chunks < 1 || dtype == APL_F64) chunks = 1;
+  // the partition runs on the device unless the host pass is asked for (or
+  // the blocks span more chunks than the device pass takes)
+  const bool dev_layout = g_layout_mode == 0 && device_layout_supported(chunks, nloc_cap);
+  if (!dev_layout)
+    for (int64_t i = 0; i < 4 * n_tets; ++i)
+      if (cells[i] < 0 || cells[i] >= n_verts) {
+        set_error("cell index out of range [0, n_verts)");
+        return -1;
+      }
   int64_t one_t[2] = {0, n_tets}, one_v[2] = {0, n_verts};
   if (n_inst <= 1 || inst_tet_off == nullptr || inst_vtx_off == nullptr) {
     if (n_inst > 1) {
@@ -145,14 +164,14 @@ static int create_common(int dtype, int64_t n_tets, int64_t n_verts, const int32
   int ncu = 0;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
     m->n_cu = ncu;
-  // LDS budget of the element workgroups: FP64 blocks hold at most 478 slots and one chunk
-  const int nloc_cap = nloc_limit(dtype);
-  if (chunks < 1 || dtype == APL_F64) chunks = 1;
-  std::string err = build_layout(cells, n_tets, n_verts, n_inst, inst_tet_off, inst_vtx_off, NTH, chunks, nloc_cap,
-                                 m->L, seg_tet_off != nullptr ? n_inst * (int64_t)n_seg : 0, seg_tet_off);
+  const int64_t n_seg_total = seg_tet_off != nullptr ? n_inst * (int64_t)n_seg : 0;
+  std::string err = dev_layout ? build_layout_device(m, cells, n_tets, n_verts, n_inst, inst_tet_off, inst_vtx_off,
+                                                     chunks, nloc_cap, n_seg_total, seg_tet_off)
+                               : build_layout(cells, n_tets, n_verts, n_inst, inst_tet_off, inst_vtx_off, NTH, chunks,
+                                              nloc_cap, m->L, n_seg_total, seg_tet_off);
   if (!err.empty()) {
     set_error(err);
-    delete m;
+    free_mesh(m);
     return -1;
   }
   Layout& L = m->L;
@@ -161,25 +180,27 @@ static int create_common(int dtype, int64_t n_tets, int64_t n_verts, const int32
   // the element kernels address the slot partials by 32-bit byte offsets
   if ((uint64_t)L.n_slots * MAX_CH * rs >= (uint64_t(1) << 32)) {
     set_error("too many block-local vertex slots for 32-bit partial offsets (reorder the mesh for locality)");
-    delete m;
+    free_mesh(m);
     return -1;
   }
   int rc = 0;
   rc |= alloc_bytes(m, &d.dminv, 9 * (size_t)n_tets * rs);
   rc |= alloc_bytes(m, &d.dV, (size_t)n_tets * rs);
-  rc |= upload(m, &d.corner, L.corner.data(), L.corner.size());
-  rc |= upload(m, &d.blk_tet_off, L.blk_tet_off.data(), L.blk_tet_off.size());
-  rc |= upload(m, &d.blk_loc_off, L.blk_loc_off.data(), L.blk_loc_off.size());
-  rc |= upload(m, &d.blk_inst, L.blk_inst.data(), L.blk_inst.size());
-  rc |= upload(m, &d.inst_blk_off, L.inst_blk_off.data(), L.inst_blk_off.size());
-  rc |= upload(m, &d.loc_gid, L.loc_gid.data(), L.loc_gid.size());
-  rc |= upload(m, &d.blk_co_off, L.blk_co_off.data(), L.blk_co_off.size());
-  rc |= upload(m, &d.chunk_off, L.chunk_off.data(), L.chunk_off.size());
-  rc |= upload(m, &d.vtx_slot_off, L.vtx_slot_off.data(), L.vtx_slot_off.size());
-  rc |= upload(m, &d.loc_dst, L.loc_dst.data(), L.loc_dst.size());
-  rc |= upload(m, &d.vblk_off, L.vblk_off.data(), L.vblk_off.size());
-  rc |= upload(m, &d.vblk_inst, L.vblk_inst.data(), L.vblk_inst.size());
-  rc |= upload(m, &d.inst_vblk_off, L.inst_vblk_off.data(), L.inst_vblk_off.size());
+  if (!dev_layout) {
+    rc |= upload(m, &d.corner, L.corner.data(), L.corner.size());
+    rc |= upload(m, &d.blk_tet_off, L.blk_tet_off.data(), L.blk_tet_off.size());
+    rc |= upload(m, &d.blk_loc_off, L.blk_loc_off.data(), L.blk_loc_off.size());
+    rc |= upload(m, &d.blk_inst, L.blk_inst.data(), L.blk_inst.size());
+    rc |= upload(m, &d.inst_blk_off, L.inst_blk_off.data(), L.inst_blk_off.size());
+    rc |= upload(m, &d.loc_gid, L.loc_gid.data(), L.loc_gid.size());
+    rc |= upload(m, &d.blk_co_off, L.blk_co_off.data(), L.blk_co_off.size());
+    rc |= upload(m, &d.chunk_off, L.chunk_off.data(), L.chunk_off.size());
+    rc |= upload(m, &d.vtx_slot_off, L.vtx_slot_off.data(), L.vtx_slot_off.size());
+    rc |= upload(m, &d.loc_dst, L.loc_dst.data(), L.loc_dst.size());
+    rc |= upload(m, &d.vblk_off, L.vblk_off.data(), L.vblk_off.size());
+    rc |= upload(m, &d.vblk_inst, L.vblk_inst.data(), L.vblk_inst.size());
+    rc |= upload(m, &d.inst_vblk_off, L.inst_vblk_off.data(), L.inst_vblk_off.size());
+  }
   rc |= upload(m, &d.inst_vtx_off, inst_vtx_off, (size_t)n_inst + 1);
   rc |= alloc_bytes(m, &d.partial, (size_t)MAX_CH * std::max<int64_t>(L.n_slots, 1) * rs);
   rc |= alloc_bytes(m, (void**)&d.blk_red, RED_STRIDE * (size_t)std::max<int64_t>(L.nblocks, 1) * sizeof(double));
@@ -205,6 +226,7 @@ static int create_common(int dtype, int64_t n_tets, int64_t n_verts, const int32
   L.chunk_off = std::vector<uint16_t>();
   L.loc_gid = std::vector<int32_t>();
   L.loc_dst = std::vector<int32_t>();
+  L.vtx_slot_off = std::vector<int32_t>();
   *out = m;
   return 0;
 }

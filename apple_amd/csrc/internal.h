@@ -132,6 +132,13 @@ int elem_grid(apl_mesh* m, int material, unsigned what, int x_f64 = 0, bool filt
 // fraction) of a created mesh from host cells / points / fraction, and the
 // read-back of the element data in the caller's tet order (T x 9, T)
 int region_from_points(apl_mesh* m, const int32_t* cells, const double* points, const double* fraction);
+// device block partition (partition.hip): the Layout of build_layout from
+// host cells, built by kernels; fills the mesh's layout arrays (device) and
+// the sizes and block / vertex-block tables (host).  "" or an error.
+bool device_layout_supported(int chunks, int nloc_cap);
+std::string build_layout_device(apl_mesh* m, const int32_t* cells, int64_t T, int64_t V, int64_t n_inst,
+                                const int64_t* inst_tet_off, const int64_t* inst_vtx_off, int chunks, int nloc_cap,
+                                int64_t n_seg_total, const int64_t* seg_off);
 int region_readback(apl_mesh* m, void* dminv_out, void* dv_out, hipStream_t s);
 
 }  // namespace apl

@@ -185,27 +185,48 @@ class Region:
         return self.mesh.n_cells
 
 
-def reorder(mesh: Mesh) -> tuple[Mesh, np.ndarray, np.ndarray]:
-    """Morton-sort the vertices and tets of an arbitrary mesh.
+def reorder(mesh: Mesh, device=None) -> tuple[Mesh, np.ndarray, np.ndarray]:
+    """Morton-sort the vertices and tets of an arbitrary mesh on the device.
 
     Returns (new_mesh, vertex_perm, cell_perm) with new = old[perm].  The
     element kernels are correct for any order; this only restores the
-    locality their block staging relies on.
+    locality their block staging relies on.  The codes and the stable sorts
+    run on the GPU (``apl_morton_order``); the host applies the permutations
+    to the mesh's arrays.  ``reorder_host`` is the same order computed with
+    numpy, for preprocessing without a GPU.
     """
-    P = mesh.points
-    lo, hi = P.min(0), P.max(0)
-    q = ((P - lo) / np.maximum(hi - lo, 1e-30) * 2047).astype(np.int64)
-    vperm = np.argsort(_morton3(q[:, 0], q[:, 1], q[:, 2]), kind="stable")
+    import torch
+
+    from .runtime import morton_order, normalize_device
+
+    dev = normalize_device(device)
+    pts = torch.as_tensor(np.ascontiguousarray(mesh.points, np.float64), device=dev)
+    cells = torch.as_tensor(np.ascontiguousarray(mesh.cells, np.int32), device=dev)
+    vp, cp = morton_order(pts, cells)
+    vperm = vp.cpu().numpy().astype(np.int64)
+    cperm = cp.cpu().numpy().astype(np.int64)
+    return _permuted(mesh, vperm, cperm), vperm, cperm
+
+
+def _permuted(mesh: Mesh, vperm: np.ndarray, cperm: np.ndarray) -> Mesh:
     vrank = np.empty_like(vperm)
     vrank[vperm] = np.arange(vperm.size)
     cells = vrank[mesh.cells]
-    C = P[mesh.cells].mean(1)
-    qc = ((C - lo) / np.maximum(hi - lo, 1e-30) * 2047).astype(np.int64)
-    cperm = np.argsort(_morton3(qc[:, 0], qc[:, 1], qc[:, 2]), kind="stable")
-    new = Mesh(
-        P[vperm].copy(),
+    return Mesh(
+        mesh.points[vperm].copy(),
         cells[cperm].astype(np.int32),
         {k: np.asarray(v)[vperm] for k, v in mesh.point_data.items()},
         {k: np.asarray(v)[cperm] for k, v in mesh.cell_data.items()},
     )
-    return new, vperm, cperm
+
+
+def reorder_host(mesh: Mesh) -> tuple[Mesh, np.ndarray, np.ndarray]:
+    """``reorder`` computed on the host with numpy (same permutations)."""
+    P = mesh.points
+    lo, hi = P.min(0), P.max(0)
+    q = ((P - lo) / np.maximum(hi - lo, 1e-30) * 2047).astype(np.int64)
+    vperm = np.argsort(_morton3(q[:, 0], q[:, 1], q[:, 2]), kind="stable")
+    C = P[mesh.cells].mean(1)
+    qc = ((C - lo) / np.maximum(hi - lo, 1e-30) * 2047).astype(np.int64)
+    cperm = np.argsort(_morton3(qc[:, 0], qc[:, 1], qc[:, 2]), kind="stable")
+    return _permuted(mesh, vperm, cperm), vperm, cperm

@@ -19,21 +19,25 @@ from .runtime import DeviceMesh, as_dtype, normalize_device
 
 class ExternalForce:
     def __init__(self, indices, force, n_points: int, *, name: str | None = None, dtype=torch.float32, device=None):
-        idx = np.asarray(indices, np.int64).reshape(-1)
-        force = np.asarray(force, np.float64).reshape(-1, 3)
-        if idx.shape[0] != force.shape[0]:
-            raise ValueError("one force per index")
-        if idx.size and (idx.min() < 0 or idx.max() >= n_points):
-            raise ValueError("force index out of range")
-        if np.unique(idx).size != idx.size:
-            raise ValueError("ExternalForce indices must be unique")
         self.name = name if name is not None else default_name(self)
         self.requires_grad = ()
         self.dtype = as_dtype(dtype)
         self.device = normalize_device(device)
         self.n_points = int(n_points)
-        self.indices = torch.as_tensor(idx, dtype=torch.int32, device=self.device).contiguous()
-        self.force = torch.as_tensor(force, dtype=self.dtype, device=self.device).contiguous()
+        # host arrays or device tensors (the checks run where the data are)
+        idx = torch.as_tensor(np.asarray(indices) if not isinstance(indices, torch.Tensor) else indices)
+        idx = idx.to(device=self.device, dtype=torch.int64).reshape(-1)
+        if not isinstance(force, torch.Tensor):
+            force = torch.as_tensor(np.asarray(force, np.float64))
+        force = force.to(device=self.device).reshape(-1, 3)
+        if idx.shape[0] != force.shape[0]:
+            raise ValueError("one force per index")
+        if idx.numel() and (int(idx.min()) < 0 or int(idx.max()) >= n_points):
+            raise ValueError("force index out of range")
+        if torch.unique(idx).numel() != idx.numel():
+            raise ValueError("ExternalForce indices must be unique")
+        self.indices = idx.to(torch.int32).contiguous()
+        self.force = force.to(self.dtype).contiguous()
         self._mesh = DeviceMesh(np.zeros((0, 4), np.int32), np.zeros((0, 3, 3)), np.zeros(0), n_points,
                                 dtype=self.dtype, device=self.device)
         self._energy = torch.zeros(1, dtype=torch.float64, device=self.device)
@@ -88,17 +92,23 @@ def gravity(region: Region, density, g=(0.0, 0.0, -9.8), **kwargs) -> ExternalFo
 
 
 def gravity_from_potential(potential, density, g=(0.0, 0.0, -9.8), **kwargs) -> ExternalForce:
-    """Lumped gravity load for a potential built without a host region (device
-    setup): f_v = rho g sum_{t ni v} dV_t / 4 with the device's dV (which
-    includes the tets' Fraction), read back once."""
+    """Lumped gravity load of a potential's device mesh: f_v = g sum_{t ni v}
+    rho_t dV_t / 4 with the device's dV (which includes the tets' Fraction),
+    summed on the device in a fixed order over the mesh's block partition
+    (``apl_mesh_lumped_volume``); density is a scalar or one value per tet."""
     mesh = potential.mesh
-    if getattr(potential, "cells_host", None) is None:
-        raise ValueError("the potential has no host cells (build it with from_region / from_pyvista)")
-    _, dV = mesh.region()
-    dV = dV.double().cpu().numpy()
-    cells = potential.cells_host
-    rho = np.broadcast_to(np.asarray(density, np.float64), (dV.shape[0],))
-    mass = np.zeros(mesh.n_verts)
-    np.add.at(mass, cells.reshape(-1), np.repeat(rho * dV / 4.0, 4))
-    force = mass[:, None] * np.asarray(g, np.float64)[None, :]
-    return ExternalForce(np.arange(mesh.n_verts), force, mesh.n_verts, dtype=mesh.dtype, device=mesh.device, **kwargs)
+    rho = None
+    scale = 1.0
+    if np.ndim(density) == 0:
+        scale = float(density)
+    else:
+        rho = torch.as_tensor(np.asarray(density, np.float64).reshape(-1), dtype=mesh.dtype, device=mesh.device)
+        if rho.shape[0] != mesh.n_tets:
+            raise ValueError("density must be a scalar or one value per tet")
+        rho = rho.contiguous()
+    vol = mesh.lumped_volume(rho)
+    if scale != 1.0:
+        vol = vol * scale
+    force = vol[:, None] * torch.as_tensor(np.asarray(g, np.float64), device=mesh.device)[None, :]
+    idx = torch.arange(mesh.n_verts, dtype=torch.int32, device=mesh.device)
+    return ExternalForce(idx, force, mesh.n_verts, dtype=mesh.dtype, device=mesh.device, **kwargs)

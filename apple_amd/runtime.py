@@ -174,6 +174,38 @@ class DeviceMesh:
         _lib.check(_lib.lib().apl_mesh_info(self._handle, ctypes.byref(inf)))
         return {k: getattr(inf, k) for k, _ in _lib.MeshInfo._fields_}
 
+    def layout(self) -> dict[str, np.ndarray]:
+        """The block partition's arrays (``apl_mesh_layout``), read back to the host."""
+        inf = self.info()
+        T, V, nb, ni, nvb = inf["n_tets"], inf["n_verts"], inf["n_blocks"], inf["n_inst"], inf["n_vertex_blocks"]
+        L = _lib.lib()
+
+        def get(name, n):
+            which, dt = _lib.LAYOUT_ARRAYS[name]
+            a = np.empty(n, dtype=dt)
+            _lib.check(L.apl_mesh_layout(self._handle, which, a.ctypes.data, int(n)))
+            return a
+
+        out = {"blk_co_off": get("blk_co_off", nb + 1)}
+        sizes = {"corner": 8 * T, "blk_tet_off": nb + 1, "blk_loc_off": nb + 1, "blk_inst": nb,
+                 "inst_blk_off": ni + 1, "loc_gid": inf["n_partials"], "chunk_off": int(out["blk_co_off"][-1]),
+                 "vtx_slot_off": V + 1, "loc_dst": inf["n_partials"], "vblk_off": nvb + 1, "vblk_inst": nvb,
+                 "inst_vblk_off": ni + 1}
+        for name, n in sizes.items():
+            out[name] = get(name, n)
+        return out
+
+    def lumped_volume(self, density: torch.Tensor | None = None) -> torch.Tensor:
+        """Per-vertex sum of density_t dV_t / 4 over the vertex's tets (FP64
+        device tensor (V,)), summed in a fixed order on the device."""
+        out = torch.empty(self.n_verts, dtype=torch.float64, device=self.device)
+        dens = None
+        if density is not None:
+            dens = ptr(density, dtype=self.dtype, device=self.device, shape=(self.n_tets,), name="density")
+        _lib.check(_lib.lib().apl_mesh_lumped_volume(self._handle, dens, ptr(out), current_stream(self.device)))
+        return out
+
+
     def close(self) -> None:
         if getattr(self, "_handle", None) is not None and self._handle.value:
             _lib.lib().apl_mesh_destroy(self._handle)
@@ -279,3 +311,35 @@ class DeviceMesh:
                 current_stream(self.device),
             )
         )
+
+
+def layout_mode(mode: int) -> int:
+    """Where later meshes build their block partition: ``_lib.LAYOUT_DEVICE``
+    (default) or ``_lib.LAYOUT_HOST``; returns the previous mode."""
+    prev = _lib.lib().apl_layout_mode(int(mode))
+    if prev < 0:
+        _lib.check(prev)
+    return prev
+
+
+def morton_order(points: torch.Tensor, cells: torch.Tensor | None = None) -> tuple[torch.Tensor, torch.Tensor | None]:
+    """(vertex_perm, cell_perm) of the Morton order on the device
+    (``apl_morton_order``): points (V,3) FP64, cells (T,4) int32, both on
+    one GPU; new = old[perm]."""
+    dev = points.device
+    points = points.to(torch.float64).contiguous()
+    V = points.shape[0]
+    vperm = torch.empty(V, dtype=torch.int32, device=dev)
+    cperm = None
+    T = 0
+    cptr = None
+    if cells is not None:
+        cells = cells.to(device=dev, dtype=torch.int32).contiguous()
+        T = cells.shape[0]
+        cperm = torch.empty(T, dtype=torch.int32, device=dev)
+        cptr = cells.data_ptr() if T else None
+    _lib.check(_lib.lib().apl_morton_order(points.data_ptr() if V else None, V, cptr, T,
+                                           vperm.data_ptr() if V else None,
+                                           cperm.data_ptr() if cperm is not None and T else None, dev.index,
+                                           current_stream(dev)))
+    return vperm, cperm

@@ -105,6 +105,59 @@ int apl_model_create(int dtype, int64_t n_tets, int64_t n_verts, const int32_t* 
  * including Fraction), device arrays in the mesh dtype; either may be NULL. */
 int apl_mesh_region(apl_mesh* mesh, void* dminv, void* dv, void* stream);
 
+/* The rest of the setup path on the device (SURVEY's "setup on GPU" row:
+ * CSR incidence and reordering, lumped-mass loads).
+ *
+ * The block partition (the vertex-major reduction layout of every element
+ * pass) is built by kernels from the cells when a mesh is created with at
+ * most 4 chunks (the default), else by the host pass; both give the same
+ * arrays.  apl_layout_mode selects it for later creations: 0 the device
+ * (default), 1 the host; returns the previous mode (no reference
+ * counterpart: a test and fallback knob, process-wide). */
+int apl_layout_mode(int mode);
+
+/* Read back one layout array of a mesh (host out, n entries, which =
+ * APL_LAYOUT_*): u16 corner records (8 T), i32 block tet / slot / chunk
+ * offsets (n_blocks + 1), i32 block instances (n_blocks), i64 instance block
+ * offsets (n_inst + 1), i32 slot vertices and vertex-major places
+ * (n_partials), u16 chunk slot offsets, i32 vertex slot offsets (V + 1),
+ * i32 vertex blocks and their instances, i64 instance vertex-block
+ * offsets.  No reference counterpart (inspection and tests). */
+enum {
+  APL_LAYOUT_CORNER = 0,
+  APL_LAYOUT_BLK_TET_OFF = 1,
+  APL_LAYOUT_BLK_LOC_OFF = 2,
+  APL_LAYOUT_BLK_INST = 3,
+  APL_LAYOUT_INST_BLK_OFF = 4,
+  APL_LAYOUT_LOC_GID = 5,
+  APL_LAYOUT_BLK_CO_OFF = 6,
+  APL_LAYOUT_CHUNK_OFF = 7,
+  APL_LAYOUT_VTX_SLOT_OFF = 8,
+  APL_LAYOUT_LOC_DST = 9,
+  APL_LAYOUT_VBLK_OFF = 10,
+  APL_LAYOUT_VBLK_INST = 11,
+  APL_LAYOUT_INST_VBLK_OFF = 12
+};
+int apl_mesh_layout(const apl_mesh* mesh, int which, void* out /* host */, int64_t n);
+
+/* Morton (Z-curve) order of a mesh on the device, the locality the block
+ * partition relies on for arbitrary meshes: points (device, V x 3 FP64),
+ * cells (device, T x 4).  Each point is quantised to 11 bits per axis over
+ * the bounding box, each tet by its centroid; vertex_perm (device, V) and
+ * cell_perm (device, T) are the stable sorts by code (new = old[perm]).  The
+ * reference takes meshes in the order they are given; this replaces the
+ * host reorder of apple_amd.mesh. */
+int apl_morton_order(const double* points, int64_t n_verts, const int32_t* cells, int64_t n_tets,
+                     int32_t* vertex_perm, int32_t* cell_perm, int device, void* stream);
+
+/* Lumped vertex volumes: out[v] (device, V, FP64) = sum over the tets t of v
+ * of density[t] dV_t / 4 (density: device, T in the mesh dtype, or NULL for
+ * 1; dV includes Fraction), summed in a fixed order through the mesh's
+ * block partition.  times g it is the lumped-mass gravity load that the
+ * experiments hand to ExternalForce (warp/potential/_ext_force.py:42-61).
+ * Uses the mesh's pass scratch: stream-ordered with its other passes. */
+int apl_mesh_lumped_volume(apl_mesh* mesh, const void* density, double* out, void* stream);
+
 /* Releases the caller's handle.  Solvers built on the mesh hold their own
  * reference: the device memory is freed when the last of them is destroyed,
  * so the handles may be destroyed in any order. */

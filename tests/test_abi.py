@@ -81,3 +81,33 @@ def test_fp64_tet_count_limit_of_32bit_element_offsets():
                            ctypes.byref(out))
     assert rc == -1 and out.value is None
     assert b"2^26" in L.apl_last_error()
+
+
+def test_layout_mode_switch_is_validated():
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    L = _lib.lib()
+    assert L.apl_layout_mode(2) == -1
+    assert "layout mode" in L.apl_last_error().decode()
+    prev = L.apl_layout_mode(_lib.LAYOUT_HOST)
+    assert prev == _lib.LAYOUT_DEVICE
+    assert L.apl_layout_mode(prev) == _lib.LAYOUT_HOST
+
+
+def test_morton_order_arguments_checked_before_any_device_call():
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    L = _lib.lib()
+    assert L.apl_morton_order(None, 10, None, 0, None, None, 0, None) == -1
+    assert "Morton" in L.apl_last_error().decode()
+    assert L.apl_morton_order(None, 0, None, 0, None, None, 0, None) == 0  # nothing to order
+
+
+def test_device_mesh_binding_methods_present():
+    # the binding layer's methods (their calls need a GPU; their presence does not)
+    from apple_amd import runtime
+
+    for name in ("close", "eval", "region", "info", "layout", "lumped_volume", "set_schedule", "set_activation",
+                 "mixed_activation", "ext_force"):
+        assert callable(getattr(runtime.DeviceMesh, name, None)), name
+    assert callable(runtime.layout_mode) and callable(runtime.morton_order)
