@@ -34,6 +34,7 @@ namespace apl {
 namespace {
 
 constexpr int PNT = 256;  // threads of the partition workgroups
+constexpr int MAXK = 4;   // tets per thread of a greedy window (max_tets <= 4 PNT)
 
 // open-addressing table of a window's / block's vertices in LDS: tv holds the
 // vertex (-1 empty), ti the minimal corner index at which it occurs
@@ -62,21 +63,28 @@ __device__ inline unsigned ht_find(const int* tv, unsigned ts, int v) {
 __host__ __device__ inline unsigned ht_size(int n) { return (unsigned)(n + n / 2 + 1); }
 
 // exclusive scan of one value per thread over the workgroup (ascending
-// thread order); returns the thread's offset, *total the sum
+// thread order); returns the thread's offset, *total the sum.  A shuffle
+// scan per wave, then the wave totals through LDS (sm: one int per wave).
 __device__ inline int wg_exclusive_scan(int x, int* sm, int* total) {
-  const int tid = threadIdx.x;
-  sm[tid] = x;
-  __syncthreads();
-  for (int d = 1; d < PNT; d <<= 1) {
-    const int y = tid >= d ? sm[tid - d] : 0;
-    __syncthreads();
-    sm[tid] += y;
-    __syncthreads();
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  int incl = x;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int y = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += y;
   }
-  const int incl = sm[tid];
-  *total = sm[PNT - 1];
+  if (lane == 63) sm[w] = incl;
   __syncthreads();
-  return incl - x;
+  int off = 0, tot = 0;
+#pragma unroll
+  for (int k = 0; k < PNT / 64; ++k) {
+    const int v = sm[k];
+    off += k < w ? v : 0;
+    tot += v;
+  }
+  *total = tot;
+  __syncthreads();  // sm is free for the next scan
+  return off + incl - x;
 }
 
 // instance of tet t: the last i with inst_tet_off[i] <= t (binary search)
@@ -136,30 +144,36 @@ __global__ __launch_bounds__(PNT) void greedy_blocks_kernel(const int32_t* __res
       const int nw = (int)std::min<long long>(max_tets, tend - t);
       const int n = 4 * nw;
       const unsigned ts = ht_size(n);
+      // tets [k0, k1) of this thread (at most MAXK), contiguous so the scan
+      // runs in tet order; their corners are loaded once, into registers
+      const int per = (nw + PNT - 1) / PNT;
+      const int k0 = std::min(nw, tid * per), k1 = std::min(nw, k0 + per);
+      int vv[4 * MAXK];
+#pragma unroll
+      for (int j = 0; j < 4 * MAXK; ++j) vv[j] = 4 * (k0 + j / 4) + (j & 3) < 4 * k1 ? cells[4 * (t + k0) + j] : -1;
       for (unsigned h = tid; h < ts; h += PNT) {
         tv[h] = -1;
         ti[h] = INT_MAX;
       }
       __syncthreads();
-      for (int i = tid; i < n; i += PNT) ht_insert(tv, ti, ts, cells[4 * t + i], i);
+#pragma unroll
+      for (int j = 0; j < 4 * MAXK; ++j)
+        if (vv[j] >= 0) ht_insert(tv, ti, ts, vv[j], 4 * k0 + j);
       __syncthreads();
-      // tets [k0, k1) of this thread, contiguous so the scan runs in tet order
-      const int per = (nw + PNT - 1) / PNT;
-      const int k0 = std::min(nw, tid * per), k1 = std::min(nw, k0 + per);
+      // first occurrences per tet (bit j of the thread's corners)
+      unsigned first = 0;
       int mine = 0;
-      for (int k = k0; k < k1; ++k)
-        for (int c = 0; c < 4; ++c) {
-          const int i = 4 * k + c;
-          mine += ti[ht_find(tv, ts, cells[4 * t + i])] == i ? 1 : 0;
+#pragma unroll
+      for (int j = 0; j < 4 * MAXK; ++j)
+        if (vv[j] >= 0 && ti[ht_find(tv, ts, vv[j])] == 4 * k0 + j) {
+          first |= 1u << j;
+          ++mine;
         }
       int total = 0;
       int run = wg_exclusive_scan(mine, scan_sm, &total);
       int fits = 0;
       for (int k = k0; k < k1; ++k) {
-        for (int c = 0; c < 4; ++c) {
-          const int i = 4 * k + c;
-          run += ti[ht_find(tv, ts, cells[4 * t + i])] == i ? 1 : 0;
-        }
+        run += __popc((first >> (4 * (k - k0))) & 15u);
         fits += run <= cap ? 1 : 0;  // D(k) is non-decreasing: the fitting k form a prefix
       }
       int len = 0;
@@ -461,12 +475,16 @@ std::string build_layout_device(apl_mesh* m, const int32_t* cells, int64_t T, in
     }
   }
 
-  // 1. blocks: runs of max_tets within each segment when the cap cannot bind
-  //    (every block then has at most 4 max_tets <= cap slots), else the greedy
-  //    kernel's ends
-  L.blk_tet_off.push_back(0);
-  L.inst_blk_off.push_back(0);
-  if (4 * max_tets <= nloc_cap) {
+  // 1. blocks.  Runs of max_tets within each segment are the greedy's blocks
+  //    whenever none of them holds more than cap distinct vertices (a prefix
+  //    of a run never holds more than the run): always so when 4 max_tets <=
+  //    cap, else checked by computing their slots first, which a
+  //    locality-ordered mesh passes.  Otherwise the greedy kernel's ends.
+  auto fixed_blocks = [&]() {
+    L.blk_tet_off.assign(1, 0);
+    L.inst_blk_off.assign(1, 0);
+    L.blk_inst.clear();
+    L.seg_of_blk.clear();
     for (int64_t i = 0; i < n_inst; ++i) {
       for (int64_t k = inst_seg[i]; k < inst_seg[i + 1]; ++k)
         for (int64_t t = segs[k]; t < segs[k + 1]; t += max_tets) {
@@ -476,7 +494,43 @@ std::string build_layout_device(apl_mesh* m, const int32_t* cells, int64_t T, in
         }
       L.inst_blk_off.push_back((int64_t)L.blk_inst.size());
     }
-  } else if (T > 0) {
+  };
+  // 2a. slots of the current blocks: the block tables to the device, the
+  //     first-touch slots into the corner records, the slot counts back
+  d.corner = A.get<uint16_t>(8 * (size_t)T);
+  int32_t* d_gid_tmp = A.get<int32_t>(4 * (size_t)T);
+  PART_CHECK(A.err);
+  std::vector<int32_t> nloc;
+  auto block_slots = [&]() -> std::string {
+    const int64_t nb = (int64_t)L.blk_inst.size();
+    d.blk_tet_off = A.get<int32_t>((size_t)nb + 1);
+    d.blk_inst = A.get<int32_t>((size_t)nb);
+    d.inst_blk_off = A.get<int64_t>((size_t)n_inst + 1);
+    int32_t* d_nloc = A.get<int32_t>((size_t)nb);
+    PART_CHECK(A.err);
+    PART_CHECK(hipMemcpy(d.blk_tet_off, L.blk_tet_off.data(), ((size_t)nb + 1) * sizeof(int32_t),
+                         hipMemcpyHostToDevice));
+    if (nb > 0)
+      PART_CHECK(hipMemcpy(d.blk_inst, L.blk_inst.data(), (size_t)nb * sizeof(int32_t), hipMemcpyHostToDevice));
+    PART_CHECK(hipMemcpy(d.inst_blk_off, L.inst_blk_off.data(), ((size_t)n_inst + 1) * sizeof(int64_t),
+                         hipMemcpyHostToDevice));
+    nloc.assign((size_t)nb, 0);
+    if (nb > 0) {
+      const size_t lds = 2 * (size_t)ht_size(4 * max_tets) * sizeof(int);
+      hipLaunchKernelGGL(block_slots_kernel, dim3((unsigned)nb), dim3(PNT), lds, nullptr, d_cells, d.blk_tet_off,
+                         d.corner, d_nloc, d_gid_tmp, max_tets);
+      PART_CHECK(hipGetLastError());
+      PART_CHECK(hipMemcpy(nloc.data(), d_nloc, (size_t)nb * sizeof(int32_t), hipMemcpyDeviceToHost));
+    }
+    return std::string();
+  };
+  fixed_blocks();
+  std::string err = block_slots();
+  if (!err.empty()) return err;
+  int fixed_max = 0;
+  for (const int32_t x : nloc) fixed_max = std::max(fixed_max, (int)x);
+  const bool fixed_ok = fixed_max <= nloc_cap;
+  if (!fixed_ok) {
     int64_t* d_segs = A.get<int64_t>(segs.size());
     int64_t* d_iseg = A.get<int64_t>(inst_seg.size());
     int32_t* d_ends = A.get<int32_t>((size_t)T);
@@ -492,6 +546,10 @@ std::string build_layout_device(apl_mesh* m, const int32_t* cells, int64_t T, in
     std::vector<int32_t> ends((size_t)T);
     PART_CHECK(hipMemcpy(counts.data(), d_counts, (size_t)n_inst * sizeof(int64_t), hipMemcpyDeviceToHost));
     PART_CHECK(hipMemcpy(ends.data(), d_ends, (size_t)T * sizeof(int32_t), hipMemcpyDeviceToHost));
+    L.blk_tet_off.assign(1, 0);
+    L.inst_blk_off.assign(1, 0);
+    L.blk_inst.clear();
+    L.seg_of_blk.clear();
     for (int64_t i = 0; i < n_inst; ++i) {
       int64_t k = inst_seg[i];
       for (int64_t j = 0; j < counts[i]; ++j) {
@@ -503,33 +561,14 @@ std::string build_layout_device(apl_mesh* m, const int32_t* cells, int64_t T, in
       }
       L.inst_blk_off.push_back((int64_t)L.blk_inst.size());
     }
-  } else {
-    for (int64_t i = 0; i < n_inst; ++i) L.inst_blk_off.push_back(0);
+    err = block_slots();
+    if (!err.empty()) return err;
   }
   L.nblocks = (int64_t)L.blk_inst.size();
   const int64_t nb = L.nblocks;
 
-  // 2. slots per block, then the offsets over blocks (host prefix sums over
-  //    the block counts), then chunk offsets, positions and the slot list
-  d.corner = A.get<uint16_t>(8 * (size_t)T);
-  d.blk_tet_off = A.get<int32_t>((size_t)nb + 1);
-  d.blk_inst = A.get<int32_t>((size_t)nb);
-  d.inst_blk_off = A.get<int64_t>((size_t)n_inst + 1);
-  int32_t* d_nloc = A.get<int32_t>((size_t)nb);
-  int32_t* d_gid_tmp = A.get<int32_t>(4 * (size_t)T);
-  PART_CHECK(A.err);
-  PART_CHECK(hipMemcpy(d.blk_tet_off, L.blk_tet_off.data(), ((size_t)nb + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
-  if (nb > 0) PART_CHECK(hipMemcpy(d.blk_inst, L.blk_inst.data(), (size_t)nb * sizeof(int32_t), hipMemcpyHostToDevice));
-  PART_CHECK(hipMemcpy(d.inst_blk_off, L.inst_blk_off.data(), ((size_t)n_inst + 1) * sizeof(int64_t),
-                       hipMemcpyHostToDevice));
-  std::vector<int32_t> nloc((size_t)nb);
-  if (nb > 0) {
-    const size_t lds = 2 * (size_t)ht_size(4 * max_tets) * sizeof(int);
-    hipLaunchKernelGGL(block_slots_kernel, dim3((unsigned)nb), dim3(PNT), lds, nullptr, d_cells, d.blk_tet_off,
-                       d.corner, d_nloc, d_gid_tmp, max_tets);
-    PART_CHECK(hipGetLastError());
-    PART_CHECK(hipMemcpy(nloc.data(), d_nloc, (size_t)nb * sizeof(int32_t), hipMemcpyDeviceToHost));
-  }
+  // 2b. the offsets over blocks (host prefix sums over the block counts),
+  //     then chunk offsets, positions and the slot list
   L.blk_loc_off.push_back(0);
   L.blk_co_off.push_back(0);
   int64_t n_slots = 0, n_co = 0;

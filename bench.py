@@ -251,6 +251,7 @@ def main():
     ap.add_argument("--no-inverse", action="store_true")
     ap.add_argument("--config1-only", action="store_true", help="only the 10^3 cube solved end to end, CPU and GPU")
     ap.add_argument("--inverse-only", action="store_true", help="only the muscle-activation inverse loop")
+    ap.add_argument("--setup-only", action="store_true", help="only the mesh setup timings (device vs host)")
     ap.add_argument("--batched-meshes", type=int, default=256)
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="threads of the multi-threaded CPU baseline (0: every core available to the process)")
@@ -286,6 +287,9 @@ def main():
         return
     if args.inverse_only:
         print(json.dumps(bench_inverse(dev)), flush=True)
+        return
+    if args.setup_only:
+        print(json.dumps(bench_setup(dev, args.n)), flush=True)
         return
     if args.cpu_threads <= 0:
         args.cpu_threads = host_cpus()
@@ -954,30 +958,76 @@ def bench_config1(dev, cpu_threads):
 def bench_setup(dev, n):
     """Mesh setup of the n^3 x 5 cube: the host region (numpy inverse / det,
     as the reference's Region) + apl_mesh_create, against
-    apl_mesh_create_from_points (element data by a device kernel); both
-    include the host block partition and the uploads."""
+    apl_mesh_create_from_points (element data by a device kernel), with the
+    block partition on the device (default) and on the host; FP64 meshes
+    (whose slot cap makes the greedy block ends bind); the Morton order of
+    the cube with its tets and vertices shuffled, device vs numpy; lumped
+    vertex volumes, device vs numpy's np.add.at."""
     import torch
 
-    from apple_amd.mesh import Region, cube
-    from apple_amd.runtime import DeviceMesh
+    from apple_amd import _lib
+    from apple_amd.mesh import Mesh, Region, cube, reorder, reorder_host
+    from apple_amd.runtime import DeviceMesh, layout_mode
 
     m = cube(n)
     DeviceMesh(m.cells, None, None, m.n_points, dtype=torch.float32, device=dev, points=m.points).close()  # warm
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    r = Region.from_mesh(m)
-    t1 = time.perf_counter()
-    a = DeviceMesh(r.cells, r.dminv, r.dV, m.n_points, dtype=torch.float32, device=dev)
-    torch.cuda.synchronize()
-    t2 = time.perf_counter()
-    b = DeviceMesh(m.cells, None, None, m.n_points, dtype=torch.float32, device=dev, points=m.points)
-    torch.cuda.synchronize()
-    t3 = time.perf_counter()
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        return out, time.perf_counter() - t0
+
+    r, t_region = timed(lambda: Region.from_mesh(m))
+    a, t_host_create = timed(lambda: DeviceMesh(r.cells, r.dminv, r.dV, m.n_points, dtype=torch.float32, device=dev))
     a.close()
+    out = {"n_tets": m.n_cells, "host_region_s": round(t_region, 4), "host_mesh_create_s": round(t_host_create, 4)}
+    for name, dt in (("fp32", torch.float32), ("fp64", torch.float64)):
+        b, t_dev = timed(lambda: DeviceMesh(m.cells, None, None, m.n_points, dtype=dt, device=dev, points=m.points))
+        b.close()
+        prev = layout_mode(_lib.LAYOUT_HOST)
+        try:
+            c, t_hp = timed(lambda: DeviceMesh(m.cells, None, None, m.n_points, dtype=dt, device=dev,
+                                               points=m.points))
+        finally:
+            layout_mode(prev)
+        c.close()
+        out[f"device_setup_{name}_s"] = round(t_dev, 4)
+        out[f"device_setup_{name}_host_partition_s"] = round(t_hp, 4)
+    out["device_setup_s"] = out["device_setup_fp32_s"]
+    rng = np.random.default_rng(0)
+    vp = rng.permutation(m.n_points)
+    rank = np.empty_like(vp)
+    rank[vp] = np.arange(vp.size)
+    shuffled = Mesh(m.points[vp], rank[m.cells][rng.permutation(m.n_cells)].astype(np.int32))
+    reorder(shuffled, dev)  # warm
+    (_, vpd, _), t_rd = timed(lambda: reorder(shuffled, dev))
+    (_, vph, _), t_rh = timed(lambda: reorder_host(shuffled))
+    out["morton_reorder_device_s"] = round(t_rd, 4)
+    out["morton_reorder_numpy_s"] = round(t_rh, 4)
+    out["morton_reorder_equal"] = bool(np.array_equal(vpd, vph))
+    b = DeviceMesh(m.cells, None, None, m.n_points, dtype=torch.float64, device=dev, points=m.points)
+    b.lumped_volume()
+    vol, t_ld = timed(lambda: b.lumped_volume())
+    dV = b.region()[1].cpu().numpy()
+
+    def host_lumped():
+        acc = np.zeros(m.n_points)
+        np.add.at(acc, m.cells.reshape(-1), np.repeat(dV / 4.0, 4))
+        return acc
+
+    ref, t_lh = timed(host_lumped)
     b.close()
-    log(f"[bench] setup {n}^3: host region {t1 - t0:.3f} s + mesh {t2 - t1:.3f} s; device setup {t3 - t2:.3f} s")
-    return {"n_tets": m.n_cells, "host_region_s": round(t1 - t0, 4), "host_mesh_create_s": round(t2 - t1, 4),
-            "device_setup_s": round(t3 - t2, 4)}
+    out["lumped_volume_device_s"] = round(t_ld, 5)
+    out["lumped_volume_numpy_s"] = round(t_lh, 4)
+    out["lumped_volume_rel_err"] = float(np.abs(vol.cpu().numpy() - ref).max() / np.abs(ref).max())
+    log(f"[bench] setup {n}^3: host region {t_region:.3f} s + mesh {t_host_create:.3f} s; device setup FP32 "
+        f"{out['device_setup_fp32_s']:.3f} s (host partition {out['device_setup_fp32_host_partition_s']:.3f} s), "
+        f"FP64 {out['device_setup_fp64_s']:.3f} s (host partition {out['device_setup_fp64_host_partition_s']:.3f} s); "
+        f"Morton reorder device {t_rd:.3f} s vs numpy {t_rh:.3f} s; lumped volumes {t_ld * 1e3:.2f} ms vs "
+        f"{t_lh * 1e3:.1f} ms")
+    return out
 
 
 def bench_inverse(dev, n=32, steps=12, prestrain=0.9, lr=0.05, seed=9):
