@@ -223,7 +223,12 @@ struct Pack {
 // position range over tps = 4, 2 or 1 adjacent lanes (the most that fit
 // SPT * NTH items), so all four waves share the sums; the lanes' interleaved
 // partial sums are combined by a fixed xor-shuffle tree.
-template <int MAT, unsigned WHAT, class R, class X, int SPT>
+// LIST: the launch visits a list of blocks (a model mesh's law), else every
+// block.  A separate instantiation: the list's lane-varying load in the
+// block gather otherwise costs the scatter-free passes their next-block
+// prefetch (the compiler's conservative memory waits, measured 17.5 ->
+// 20.2 us for pass B).
+template <int MAT, unsigned WHAT, class R, class X, int SPT, bool LIST>
 __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R, X> A) {
   constexpr bool HAS_G = (WHAT & W_G) != 0;
   constexpr bool HAS_H = (WHAT & W_H) != 0;
@@ -260,7 +265,10 @@ __global__ __launch_bounds__(NTH) void elem_kernel(ElemArgs<R, X> A) {
   const int nb_x = my_chunks * C - (owns_last ? nchunk * C - nb : 0);
   auto xcd_block = [&](int j) {
     const int k = ((j / C) * 8 + xcd) * C + j % C;
-    return A.blk_list != nullptr ? A.blk_list[k] : k;
+    if constexpr (LIST)
+      return A.blk_list[k];
+    else
+      return k;
   };
 
   // This workgroup's blocks are xcd_block(k0 + j K), j < n_cand; in a solve
@@ -752,8 +760,10 @@ static int grid_for_spt(apl_mesh* m, bool filtered) {
   if (cached > 0) return cached;
   const size_t smem = smem_bytes<MAT, WHAT, R, X>(m, filtered);
   int occ = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, elem_kernel<MAT, WHAT, R, X, SPT>, NTH, smem) != hipSuccess ||
-      occ < 1)
+  const hipError_t oe =
+      m->model ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, elem_kernel<MAT, WHAT, R, X, SPT, true>, NTH, smem)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, elem_kernel<MAT, WHAT, R, X, SPT, false>, NTH, smem);
+  if (oe != hipSuccess || occ < 1)
     occ = 1;
   const int pct = m->grid_pct > 0 ? m->grid_pct : default_grid_pct(MAT, WHAT);
   long long g = (long long)occ * m->n_cu * pct / 100;
@@ -798,12 +808,19 @@ static int launch_one(apl_mesh* m, const void* mu, const void* lam, const void* 
   }
   constexpr bool SCATTER = (WHAT & (W_G | W_H | W_HP)) != 0;
   const int spt = SCATTER ? spt_for(m) : 1;
-  if (spt == 1)
-    hipLaunchKernelGGL((elem_kernel<MAT, WHAT, R, X, 1>), dim3(grid), dim3(NTH), smem, s, A);
-  else if (spt == 2)
-    hipLaunchKernelGGL((elem_kernel<MAT, WHAT, R, X, SCATTER ? 2 : 1>), dim3(grid), dim3(NTH), smem, s, A);
+  auto go = [&](auto list_c) {
+    constexpr bool LIST = decltype(list_c)::value;
+    if (spt == 1)
+      hipLaunchKernelGGL((elem_kernel<MAT, WHAT, R, X, 1, LIST>), dim3(grid), dim3(NTH), smem, s, A);
+    else if (spt == 2)
+      hipLaunchKernelGGL((elem_kernel<MAT, WHAT, R, X, SCATTER ? 2 : 1, LIST>), dim3(grid), dim3(NTH), smem, s, A);
+    else
+      hipLaunchKernelGGL((elem_kernel<MAT, WHAT, R, X, SCATTER ? 4 : 1, LIST>), dim3(grid), dim3(NTH), smem, s, A);
+  };
+  if (m->model)
+    go(std::integral_constant<bool, true>{});
   else
-    hipLaunchKernelGGL((elem_kernel<MAT, WHAT, R, X, SCATTER ? 4 : 1>), dim3(grid), dim3(NTH), smem, s, A);
+    go(std::integral_constant<bool, false>{});
   APL_CHECK_HIP(hipGetLastError());
   return 0;
 }
