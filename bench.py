@@ -224,6 +224,9 @@ def bench_solve(prob, material, dev, rtol=1e-6):
         "evals": st.n_evals,
         "relative_grad_norm": st.relative_grad_norm,
         "us_per_iteration": round(dt / max(st.n_evals, 1) * 1e6, 2),
+        # FP32 element kernels; the solver keeps its own FP64 copy of the
+        # iterate (24 B per vertex staged per block slot instead of 12)
+        "iterate": "f64 copy" if prob["pot"].mesh.dtype == torch.float32 else "f64",
     }
 
 

@@ -40,6 +40,14 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
 
 
+def rel_max(a, b):
+    """Maximum over entries: max |a - b| / max |b|."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    mb = np.abs(b).max() if b.size else 0.0
+    return float(np.abs(a - b).max() / (mb if mb > 0 else 1.0)) if b.size else 0.0
+
+
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 

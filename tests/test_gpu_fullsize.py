@@ -24,7 +24,7 @@ import pytest
 import torch
 
 from apple_amd import fem
-from helpers import rel, snh_case
+from helpers import rel, rel_max, snh_case
 
 pytestmark = pytest.mark.gpu
 
@@ -75,7 +75,9 @@ def test_snh_fp32_matches_c_oracle_at_full_size(case):
           f"hdiag rel {rel(h, h_ref):.2e}, hq rel {abs(hq - hq_ref) / abs(hq_ref):.2e}", flush=True)
     assert abs(e - e_ref) <= 1e-5 * abs(e_ref)
     assert rel(g, g_ref) <= 1e-4
+    assert rel_max(g, g_ref) <= 1e-4
     assert rel(h, h_ref) <= 1e-4
+    assert rel_max(h, h_ref) <= 1e-4
     assert abs(hq - hq_ref) <= 1e-4 * abs(hq_ref)
 
 
@@ -128,7 +130,9 @@ def test_arap_fp32_matches_fp64_device_pass_at_full_size(case):
     (e32, g32, h32, hq32), (e64, g64, h64, hq64) = out[torch.float32], out[torch.float64]
     assert abs(e32 - e64) <= 1e-5 * abs(e64)
     assert rel(g32, g64) <= 1e-4
+    assert rel_max(g32, g64) <= 1e-4
     assert rel(h32, h64) <= 1e-4
+    assert rel_max(h32, h64) <= 1e-4
     assert abs(hq32 - hq64) <= 1e-4 * abs(hq64)
 
 
@@ -159,8 +163,11 @@ def test_muscle_fp32_matches_fp64_device_pass_at_full_size(case):
     (e32, g32, h32, hq32, mx32), (e64, g64, h64, hq64, mx64) = out[torch.float32], out[torch.float64]
     assert abs(e32 - e64) <= 1e-5 * abs(e64)
     assert rel(g32, g64) <= 1e-4
+    assert rel_max(g32, g64) <= 1e-4
     assert rel(h32, h64) <= 1e-4
+    assert rel_max(h32, h64) <= 1e-4
     assert abs(hq32 - hq64) <= 1e-4 * abs(hq64)
     assert set(mx64) == {"mu", "lambda", "activation"}
     for k in mx64:
         assert rel(mx32[k], mx64[k]) <= 1e-4, k
+        assert rel_max(mx32[k], mx64[k]) <= 1e-4, k
