@@ -40,6 +40,37 @@ def _run(pot, u, v):
 
 
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_arap_compressed_state_takes_the_clamp_branch(dtype):
+    """ARAP on a cube compressed to 0.85 of its size plus noise: nearly every
+    twist pair has s_i + s_j < 2, so the lambda clamp decides Hdiag, H v and
+    v^T H v; the kernels match the oracle there (whose clamp branch is pinned
+    by the closed form of test_oracle_fem.test_arap_lambda_clamp_closed_form)."""
+    import oracle.fem as of
+
+    m, region, _, v64 = snh_case(8, seed=3)
+    u64 = -0.15 * m.points + 0.005 * np.random.default_rng(4).standard_normal(m.points.shape)
+    npdt = np.float32 if dtype == torch.float32 else np.float64
+    u_r = u64.astype(npdt).astype(np.float64)
+    v_r = v64.astype(npdt).astype(np.float64)
+    ref = oracle_pot(region, "arap", m.cell_data["mu"], None, dtype_round=npdt)
+    _, s, _ = of.svd_rv(ref.F(u_r))
+    pairs = np.stack([s[:, 0] + s[:, 1], s[:, 1] + s[:, 2], s[:, 2] + s[:, 0]], 1)
+    print(f"[elem] ARAP compressed {dtype}: clamped pairs {np.mean(pairs < 2):.1%}", flush=True)
+    assert np.mean(pairs < 2) > 0.9
+    pot = fem.Arap.from_region(region, dtype=dtype, device="cuda")
+    got = _run(pot, torch.as_tensor(u_r, dtype=dtype, device="cuda"), torch.as_tensor(v_r, dtype=dtype, device="cuda"))
+    tol = TOL[dtype]
+    for k, r in (("g", ref.grad(u_r)), ("h", ref.hess_diag(u_r)), ("hp", ref.hess_prod(u_r, v_r))):
+        assert rel(got[k], r) <= tol["g"], k
+        assert rel_max(got[k], r) <= tol["g"], k
+    hq_ref = ref.hess_quad(u_r, v_r)
+    assert abs(got["hq"] - hq_ref) <= tol["g"] * abs(hq_ref)
+    # the projected quadratic form differs from the exact one here
+    assert hq_ref > 1.01 * of.arap_hess_quad(ref.F(u_r), of.cell_values(v_r, region.cells), region.dhdX,
+                                             ref.mu, clamp=False).clip(min=0).dot(region.dV)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
 @pytest.mark.parametrize("material", ["snh", "arap"])
 @pytest.mark.parametrize("n,noise,chunks", [(4, 0.01, 1), (10, 0.01, 1), (12, 0.003, 4)])
 def test_element_pass_matches_oracle(dtype, material, n, noise, chunks):

@@ -405,3 +405,46 @@ def test_external_force_sign_convention():
     g = ext.grad(u)
     np.testing.assert_allclose(g[[1, 3]], -f)
     np.testing.assert_allclose(g[[0, 2, 4]], 0.0)
+
+
+@pytest.mark.parametrize("sigma", [(1.5, 0.9, 0.8), (0.9, 0.8, 0.7), (1.3, 1.2, 1.1)])
+def test_arap_lambda_clamp_closed_form(sigma):
+    """The clamp branch of lambda_k = 2 / max(s_i + s_j, 2) (func/_misc.py:31-43)
+    pinned by the mathematics: for F = U diag(s) V^T and the unit twist
+    W = U K_ab V^T (K_ab = (e_a e_b^T - e_b e_a^T)/sqrt2), the polar factor
+    moves as dR[W] = 2/(s_a + s_b) W, so the exact curvature of
+    Psi = mu/2 |F - R|^2 along W is mu (1 - 2/(s_a + s_b)) (negative under
+    compression, s_a + s_b < 2) and the projected one mu (1 - min(1, 2/(s_a + s_b)));
+    along U E_aa V^T (a stretch) both are mu.  The exact one is also the
+    central difference of the stress (arap_pk1) along W."""
+    rng = np.random.default_rng(11)
+    U, V = _rot(rng, 2)  # two rotations
+    s = np.asarray(sigma, np.float64)
+    F = (U * s[None, :]) @ V.T
+    mu = np.array([2.0])
+    dhdX = np.array([[[-1.0, -1.0, -1.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]]])
+
+    def quad(W, clamp):
+        p = np.zeros((1, 4, 3))
+        p[0, 1:] = W.T  # corner j + 1 moves by column j of W: dF = W
+        assert np.allclose(of.jvp(dhdX, p)[0], W)
+        return float((p * of.arap_hess_prod(F[None], p, dhdX, mu, clamp=clamp)).sum())
+
+    for a, b in ((0, 1), (1, 2), (2, 0)):
+        K = np.zeros((3, 3))
+        K[a, b], K[b, a] = 1 / np.sqrt(2), -1 / np.sqrt(2)
+        W = U @ K @ V.T
+        lam = 2.0 / (s[a] + s[b])
+        assert quad(W, True) == pytest.approx(mu[0] * (1 - min(1.0, lam)), abs=1e-12)
+        assert quad(W, False) == pytest.approx(mu[0] * (1 - lam), rel=1e-10)
+        h = 1e-6
+        fd = (of.arap_pk1((F + h * W)[None], mu)[0] - of.arap_pk1((F - h * W)[None], mu)[0]) / (2 * h)
+        assert float((fd * W).sum()) == pytest.approx(mu[0] * (1 - lam), rel=1e-6)
+        if s[a] + s[b] < 2:
+            assert quad(W, True) == pytest.approx(0.0, abs=1e-12) and quad(W, False) < 0
+    for a in range(3):
+        E = np.zeros((3, 3))
+        E[a, a] = 1.0
+        S = U @ E @ V.T
+        assert quad(S, True) == pytest.approx(mu[0], rel=1e-12)
+        assert quad(S, False) == pytest.approx(mu[0], rel=1e-12)
