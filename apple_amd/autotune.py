@@ -29,14 +29,14 @@ from . import _lib
 from .runtime import DeviceMesh, current_stream, ptr
 
 CHUNKS = (1, 2, 4)
-GRID_PCT = (100, 150, 200)
+GRID_PCT = (0, 100, 150, 200, 300)  # 0: the library's per-pass default grids
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "apple_amd", "autotune.json")
 
 
 @dataclasses.dataclass
 class Schedule:
     chunks: int = 1
-    grid_pct: int = 100
+    grid_pct: int = 0  # 0: the per-pass default grids
     us_per_iteration: float | None = None  # pass A + pass B element kernels at the pick
     table: dict | None = None  # "chunks/grid_pct" -> microseconds
 

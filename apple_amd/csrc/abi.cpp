@@ -354,8 +354,8 @@ int apl_mesh_destroy(apl_mesh* mesh) {
 }
 
 int apl_mesh_set_schedule(apl_mesh* mesh, int grid_pct) {
-  if (mesh == nullptr || grid_pct < 25 || grid_pct > 800) {
-    set_error("grid_pct must be in [25, 800]");
+  if (mesh == nullptr || (grid_pct != 0 && (grid_pct < 25 || grid_pct > 800))) {
+    set_error("grid_pct must be 0 (the per-pass default) or in [25, 800]");
     return -1;
   }
   mesh->grid_pct = grid_pct;

@@ -172,9 +172,9 @@ int apl_mesh_info(const apl_mesh* mesh, apl_mesh_info_t* info /* host */);
 
 /* Schedule knob (no reference counterpart; set by the autotuner,
  * apple_amd/autotune.py): the persistent element-kernel grid as a percentage
- * (25..800) of occupancy x compute units (until set: a per-pass default,
- * 200 for the ARAP gradient passes and the Hessian-vector pass, 100
- * otherwise).
+ * (25..800) of occupancy x compute units, or 0 for a per-pass default (the
+ * initial setting: 200 for the ARAP gradient passes and the Hessian-vector
+ * pass, 100 otherwise).
  * Together with the block size
  * chosen at apl_mesh_create (chunks) it is the schedule the autotuner picks.
  * Takes effect for later launches; a solver's captured graph keeps its grid

@@ -323,3 +323,30 @@ def test_batch_refine_fp64_checked_meets_the_fp64_gradient_target():
         # the FP32 rounding of the returned positions may add a little
         assert gn <= 2 * cfg.rtol * st32[i].grad_norm_first, (i, gn, st32[i].grad_norm_first, b32.refined)
     print(f"refined instances: {b32.refined}")
+
+
+def test_batch_refine_fp64_follows_replaced_materials():
+    """After the FP32 batch's materials are replaced (as adjoint.equilibrium
+    and Inverse do), the FP64 check and re-solve use the new values widened,
+    not the instances' original ones."""
+    print("[solver] batch refine_fp64 after a material update", flush=True)
+    insts = []
+    for k, n in enumerate((3, 4)):
+        m, region, mu, lam, f, fm, x0 = _gravity_case(n, "snh", seed=60 + k)
+        insts.append(batch.Instance(region, mu, lam, f, fm, x0))
+    cfg = PncgConfig(rtol=1e-7, max_steps=20000)
+    b32 = batch.Batch(insts, _lib_snh(), dtype=torch.float32, device="cuda")
+    b32.solve(config=cfg, refine_fp64="checked")  # builds the FP64 copy with the original materials
+    mats = b32.potential.materials
+    mats["mu"].mul_(1.5)
+    x32, st32 = b32.solve(config=cfg, refine_fp64="checked")
+    b64 = b32._fp64_copy()
+    assert torch.equal(b64.potential.materials["mu"], mats["mu"].double())
+    g = torch.zeros_like(x32, dtype=torch.float64)
+    b64.mesh.eval(b64.material, b64.potential.materials["mu"], b64.potential.materials["lambda"], x32.double(), grad=g)
+    g -= b64.f_ext
+    free = torch.stack([(b64.mask >> c) & 1 == 0 for c in range(3)], dim=1)
+    g = torch.where(free, g, torch.zeros_like(g))
+    for i in range(b32.n_inst):
+        sl = slice(int(b32.vtx_off[i]), int(b32.vtx_off[i + 1]))
+        assert float(torch.linalg.norm(g[sl])) <= 2 * cfg.rtol * st32[i].grad_norm_first, i
