@@ -105,7 +105,7 @@ int apl_model_create(int dtype, int64_t n_tets, int64_t n_verts, const int32_t* 
  * including Fraction), device arrays in the mesh dtype; either may be NULL. */
 int apl_mesh_region(apl_mesh* mesh, void* dminv, void* dv, void* stream);
 
-/* The rest of the setup path on the device (SURVEY's "setup on GPU" row:
+/* The rest of the setup path on the device (mesh setup done by kernels:
  * CSR incidence and reordering, lumped-mass loads).
  *
  * The block partition (the vertex-major reduction layout of every element
