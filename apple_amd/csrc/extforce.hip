@@ -2,6 +2,8 @@
 // E = -sum_k f_k . u[idx_k], grad[idx_k] -= f_k, no Hessian.
 // Indices are unique (GLOBAL_POINT_ID in the reference), so the gradient
 // scatter needs no atomics; the energy is a fixed-order FP64 two-level sum.
+// An index outside [0, V) contributes nothing and touches no memory (the
+// Python layer rejects such indices before they get here).
 #include "../../include/apple_amd.h"
 #include <algorithm>
 
@@ -12,15 +14,15 @@ namespace apl {
 constexpr int EXT_PER_BLOCK = NTH * 8;
 
 template <class R>
-__global__ __launch_bounds__(NTH) void ext_energy_partial(long long n, const int32_t* idx, const R* f, const R* u,
-                                                          double* part) {
+__global__ __launch_bounds__(NTH) void ext_energy_partial(long long n, long long V, const int32_t* idx, const R* f,
+                                                          const R* u, double* part) {
   __shared__ double red[NTH / 64];
   const long long base = (long long)blockIdx.x * EXT_PER_BLOCK;
   double acc = 0.0;
   for (int k = threadIdx.x; k < EXT_PER_BLOCK; k += NTH) {
     const long long i = base + k;
-    if (i < n) {
-      const long long v = idx[i];
+    const long long v = i < n ? idx[i] : -1;
+    if (v >= 0 && v < V) {
       acc -= (double)f[3 * i] * (double)u[3 * v] + (double)f[3 * i + 1] * (double)u[3 * v + 1] +
              (double)f[3 * i + 2] * (double)u[3 * v + 2];
     }
@@ -53,11 +55,12 @@ __global__ __launch_bounds__(NTH) void ext_energy_final(long long nparts, const 
 }
 
 template <class R>
-__global__ __launch_bounds__(NTH) void ext_grad(long long n, const int32_t* idx, const R* f, R* out,
+__global__ __launch_bounds__(NTH) void ext_grad(long long n, long long V, const int32_t* idx, const R* f, R* out,
                                                 int accumulate) {
   const long long i = (long long)blockIdx.x * NTH + threadIdx.x;
   if (i >= n) return;
   const long long v = idx[i];
+  if (v < 0 || v >= V) return;
   if (accumulate) {
     out[3 * v] -= f[3 * i];
     out[3 * v + 1] -= f[3 * i + 1];
@@ -83,14 +86,14 @@ static int ext_eval(apl_mesh* m, long long n, const int32_t* idx, const void* f,
       set_error("too many external-force entries for this mesh");
       return -1;
     }
-    hipLaunchKernelGGL((ext_energy_partial<R>), dim3((unsigned)nparts), dim3(NTH), 0, s, n, idx, (const R*)f,
-                       (const R*)u, m->d.vblk_dots);
+    hipLaunchKernelGGL((ext_energy_partial<R>), dim3((unsigned)nparts), dim3(NTH), 0, s, n, (long long)m->L.V, idx,
+                       (const R*)f, (const R*)u, m->d.vblk_dots);
     hipLaunchKernelGGL(ext_energy_final, dim3(1), dim3(NTH), 0, s, nparts, m->d.vblk_dots, energy, accumulate);
     APL_CHECK_HIP(hipGetLastError());
   }
   if (grad != nullptr) {
-    hipLaunchKernelGGL((ext_grad<R>), dim3((unsigned)((n + NTH - 1) / NTH)), dim3(NTH), 0, s, n, idx, (const R*)f,
-                       (R*)grad, accumulate);
+    hipLaunchKernelGGL((ext_grad<R>), dim3((unsigned)((n + NTH - 1) / NTH)), dim3(NTH), 0, s, n, (long long)m->L.V, idx,
+                       (const R*)f, (R*)grad, accumulate);
     APL_CHECK_HIP(hipGetLastError());
   }
   return 0;

@@ -762,18 +762,34 @@ __global__ __launch_bounds__(PNT) void morton_vertex_kernel(const double* __rest
   }
 }
 
-// tet centroids ((p0 + p1) + p2) + p3, then / 4, of the original points
-__global__ __launch_bounds__(PNT) void morton_cell_kernel(const double* __restrict__ P, const int32_t* __restrict__ cells,
-                                                          long long T, const double* __restrict__ lohi,
-                                                          uint64_t* __restrict__ codes, int32_t* __restrict__ idx) {
+// tet centroids ((p0 + p1) + p2) + p3, then / 4, of the original points.
+// A tet with a corner outside [0, V) reads no point: it gets code 0 and
+// *bad the smallest such tet (the caller turns that into an error).
+__global__ __launch_bounds__(PNT) void morton_cell_kernel(const double* __restrict__ P, long long V,
+                                                          const int32_t* __restrict__ cells, long long T,
+                                                          const double* __restrict__ lohi,
+                                                          uint64_t* __restrict__ codes, int32_t* __restrict__ idx,
+                                                          unsigned long long* __restrict__ bad) {
   for (long long t = (long long)blockIdx.x * PNT + threadIdx.x; t < T; t += (long long)gridDim.x * PNT) {
-    double x[3];
-    for (int a = 0; a < 3; ++a) {
-      double s = P[3 * (long long)cells[4 * t] + a];
-      for (int c = 1; c < 4; ++c) s += P[3 * (long long)cells[4 * t + c] + a];
-      x[a] = s / 4.0;
+    long long id[4];
+    bool ok = true;
+    for (int c = 0; c < 4; ++c) {
+      id[c] = cells[4 * t + c];
+      ok = ok && id[c] >= 0 && id[c] < V;
     }
-    codes[t] = morton_of(x, lohi);
+    uint64_t code = 0;
+    if (ok) {
+      double x[3];
+      for (int a = 0; a < 3; ++a) {
+        double s = P[3 * id[0] + a];
+        for (int c = 1; c < 4; ++c) s += P[3 * id[c] + a];
+        x[a] = s / 4.0;
+      }
+      code = morton_of(x, lohi);
+    } else {
+      atomicMin(bad, (unsigned long long)t);
+    }
+    codes[t] = code;
     idx[t] = (int32_t)t;
   }
 }
@@ -855,6 +871,10 @@ int apl_morton_order(const double* points, int64_t n_verts, const int32_t* cells
     set_error("invalid Morton-order arguments");
     return -1;
   }
+  if (n_verts == 0 && n_tets > 0) {
+    set_error("cell index out of range [0, n_verts)");
+    return -1;
+  }
   if (n_verts == 0) return 0;
   APL_CHECK_HIP(hipSetDevice(device));
   (void)hipGetLastError();
@@ -864,6 +884,7 @@ int apl_morton_order(const double* points, int64_t n_verts, const int32_t* cells
   double* part = A.get<double>(6 * (size_t)np);
   uint64_t* codes = A.get<uint64_t>((size_t)std::max(n_verts, n_tets));
   int32_t* idx = A.get<int32_t>((size_t)std::max(n_verts, n_tets));
+  unsigned long long* bad = A.get<unsigned long long>(1);
   if (A.err != hipSuccess) {
     set_error(std::string("hipMalloc: ") + hipGetErrorString(A.err));
     return -1;
@@ -877,16 +898,26 @@ int apl_morton_order(const double* points, int64_t n_verts, const int32_t* cells
     set_error("Morton sort of the vertices failed");
     return -1;
   }
+  const unsigned long long none = ULLONG_MAX;
+  unsigned long long bad_t = none;
   if (n_tets > 0) {
-    hipLaunchKernelGGL(morton_cell_kernel, dim3(grid_n(n_tets)), dim3(PNT), 0, s, points, cells, (long long)n_tets,
-                       part, codes, idx);
+    APL_CHECK_HIP(hipMemcpyAsync(bad, &none, sizeof none, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(morton_cell_kernel, dim3(grid_n(n_tets)), dim3(PNT), 0, s, points, (long long)n_verts, cells,
+                       (long long)n_tets, part, codes, idx, bad);
     APL_CHECK_HIP(hipGetLastError());
     if (sort_codes(A, codes, idx, n_tets, cell_perm, s)) {
       set_error("Morton sort of the tets failed");
       return -1;
     }
+    APL_CHECK_HIP(hipMemcpyAsync(&bad_t, bad, sizeof bad_t, hipMemcpyDeviceToHost, s));
   }
   APL_CHECK_HIP(hipStreamSynchronize(s));  // the temporaries are freed on return
+  if (bad_t != none) {
+    char msg[128];
+    snprintf(msg, sizeof msg, "cell index out of range [0, n_verts) in tet %llu", bad_t);
+    set_error(msg);
+    return -1;
+  }
   return 0;
 }
 

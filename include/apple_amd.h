@@ -144,8 +144,9 @@ int apl_mesh_layout(const apl_mesh* mesh, int which, void* out /* host */, int64
  * partition relies on for arbitrary meshes: points (device, V x 3 FP64),
  * cells (device, T x 4).  Each point is quantised to 11 bits per axis over
  * the bounding box, each tet by its centroid; vertex_perm (device, V) and
- * cell_perm (device, T) are the stable sorts by code (new = old[perm]).  The
- * reference takes meshes in the order they are given; this replaces the
+ * cell_perm (device, T) are the stable sorts by code (new = old[perm]); a
+ * cell index outside [0, n_verts) is an error (nothing out of range is read).
+ * The reference takes meshes in the order they are given; this replaces the
  * host reorder of apple_amd.mesh. */
 int apl_morton_order(const double* points, int64_t n_verts, const int32_t* cells, int64_t n_tets,
                      int32_t* vertex_perm, int32_t* cell_perm, int device, void* stream);
@@ -222,7 +223,9 @@ int apl_elem_mixed_activation(apl_mesh* mesh, const void* mu, const void* lam, c
                               void* dact, void* dmu, void* dlam, int accumulate, void* stream);
 
 /* ExternalForce (warp/potential/_ext_force.py:17-39): E -= f . u[idx],
- * grad[idx] -= f for n entries with unique indices.  energy: (1,) float64. */
+ * grad[idx] -= f for n entries with unique indices; an index outside
+ * [0, n_verts) contributes nothing (no memory is touched for it).
+ * energy: (1,) float64. */
 int apl_ext_force_eval(apl_mesh* mesh, int64_t n, const int32_t* indices, const void* force, const void* u,
                        double* energy, void* grad, int accumulate, void* stream);
 

@@ -89,3 +89,31 @@ def test_unordered_tets_fill_the_lds_cap_and_still_agree(dtype, cap):
     assert np.abs(xs[0][0] - xs[1][0]).max() / diag <= tol_x
     assert rel(xs[0][1], xs[1][1]) <= (1e-6 if dtype == torch.float64 else 1e-3)
     assert rel_max(xs[0][1], xs[1][1]) <= (1e-6 if dtype == torch.float64 else 1e-3)
+
+
+def test_external_force_skips_indices_outside_the_mesh():
+    """At the C ABI an ExternalForce index outside [0, V) contributes nothing
+    and touches no memory; the valid entries give E = -f . u[idx] and
+    grad[idx] -= f exactly."""
+    from apple_amd.runtime import DeviceMesh
+
+    print("[limits] external force with out-of-range indices", flush=True)
+    m = cube(3)
+    V = m.n_points
+    dm = DeviceMesh(m.cells, np.tile(np.eye(3), (m.n_cells, 1, 1)), np.full(m.n_cells, 1.0 / 6.0), V,
+                    dtype=torch.float64, device="cuda")
+    rng = np.random.default_rng(11)
+    idx = np.array([0, V, 5, -1, V - 1, 1 << 30, 7], np.int32)
+    f = rng.standard_normal((idx.size, 3))
+    u = rng.standard_normal((V, 3))
+    ok = (idx >= 0) & (idx < V)
+    e_ref = -float(np.sum(f[ok] * u[idx[ok]]))
+    g_ref = np.full((V, 3), 2.0)
+    g_ref[idx[ok]] -= f[ok]
+    e = torch.zeros(1, dtype=torch.float64, device="cuda")
+    g = torch.full((V, 3), 2.0, dtype=torch.float64, device="cuda")
+    dm.ext_force(torch.as_tensor(idx, device="cuda"), torch.as_tensor(f, device="cuda"),
+                 torch.as_tensor(u, device="cuda"), energy=e, grad=g, accumulate=True)
+    torch.cuda.synchronize()
+    assert e.item() == pytest.approx(e_ref, rel=1e-14)
+    np.testing.assert_array_equal(g.cpu().numpy(), g_ref)

@@ -163,6 +163,27 @@ def test_morton_order_equals_numpy():
     assert ordered < 0.6 * shuffled
 
 
+def test_morton_order_rejects_out_of_range_cells():
+    """A cell index outside [0, V) is an error that names the first bad tet;
+    the kernel reads no point for it."""
+    from apple_amd.runtime import morton_order
+
+    print("[partition] Morton order, out-of-range cells", flush=True)
+    mesh = _irregular(5, 7)
+    pts = torch.as_tensor(mesh.points, device="cuda")
+    for t, c, v in ((17, 2, mesh.n_points), (40, 0, -3), (3, 3, 1 << 30)):
+        cells = mesh.cells.copy()
+        cells[t, c] = v
+        with pytest.raises(RuntimeError, match=f"out of range.*tet {t}"):
+            morton_order(pts, torch.as_tensor(cells, device="cuda"))
+    with pytest.raises(RuntimeError, match="out of range"):
+        morton_order(pts[:0], torch.as_tensor(mesh.cells[:2], device="cuda"))
+    # the same call with valid cells still works afterwards
+    vp, cp = morton_order(pts, torch.as_tensor(mesh.cells, device="cuda"))
+    np.testing.assert_array_equal(vp.cpu().numpy(), reorder_host(mesh)[1])
+    np.testing.assert_array_equal(cp.cpu().numpy(), reorder_host(mesh)[2])
+
+
 @pytest.mark.parametrize("dtype,chunks", [(torch.float64, 1), (torch.float32, 2)])
 def test_lumped_volume_equals_add_at(dtype, chunks):
     print(f"[partition] lumped volume {dtype} chunks {chunks}", flush=True)

@@ -352,31 +352,36 @@ def test_batch_refine_fp64_follows_replaced_materials():
         assert float(torch.linalg.norm(g[sl])) <= 2 * cfg.rtol * st32[i].grad_norm_first, i
 
 
-@pytest.mark.parametrize("material", ["snh", "arap"])
+@pytest.mark.parametrize("material", ["snh", "arap", "muscle"])
 def test_trajectory_matches_oracle_pncg_step_by_step(material):
     """FP64: the device PNCG after k accepted steps equals the oracle's PNCG
     (oracle/pncg.py, the same Jacobi / Dai-Kou / closed-form alpha / Armijo or
     approximate-Wolfe rules) after k iterations from the same start, for k up
     to 60 -- the per-iteration logic, not only the end state.  The mesh has
     one extra free vertex that no tet touches: its Hessian diagonal is 0, so
-    the Jacobi preconditioner's mean fallback is exercised."""
+    the Jacobi preconditioner's mean fallback is exercised.  The muscle law
+    runs with a random activation (A = I + sym(a), |a| ~ 0.1)."""
     print(f"[solver] trajectory vs oracle {material}", flush=True)
     m, region, mu, lam, f, fm, x0 = _gravity_case(4, material, seed=5)
     V = m.n_points + 1  # the dangling vertex
     f = np.vstack([f, np.zeros((1, 3))])
     fm = np.vstack([fm, np.zeros((1, 3), bool)])
     x0 = np.vstack([x0, np.zeros((1, 3))])
-    mat = of.SNH if material == "snh" else of.ARAP
-    opot = of.FemPotential(mat, region.cells, region.dhdX, region.dV, mu, lam if mat == of.SNH else None)
+    act = 0.1 * np.random.default_rng(6).standard_normal((m.n_cells, 6)) if material == "muscle" else None
+    mat = {"snh": of.SNH, "arap": of.ARAP, "muscle": of.MUSCLE}[material]
+    opot = of.FemPotential(mat, region.cells, region.dhdX, region.dV, mu, None if mat == of.ARAP else lam,
+                           activation=act)
     dof = of.DofMap(fm, np.zeros_like(f))
     prob = op.Problem(of.Model([opot, of.ExternalForce(np.arange(V), f)]), dof)
     from apple_amd.runtime import DeviceMesh
 
-    cls = fem.StableNeoHookean if material == "snh" else fem.Arap
+    cls = {"snh": fem.StableNeoHookean, "arap": fem.Arap, "muscle": fem.StableNeoHookeanMuscle}[material]
     dmesh = DeviceMesh(region.cells, region.dhdX[:, 1:], region.dV, V, dtype=torch.float64, device="cuda")
     mats = {"mu": torch.as_tensor(mu, dtype=torch.float64, device="cuda")}
-    if material == "snh":
+    if material != "arap":
         mats["lambda"] = torch.as_tensor(lam, dtype=torch.float64, device="cuda")
+    if material == "muscle":
+        mats["activation"] = torch.as_tensor(act, dtype=torch.float64, device="cuda").contiguous()
     pot = cls(dmesh, mats)
     solver = DeviceSolver(pot)
     ft = torch.as_tensor(f, dtype=torch.float64, device="cuda").contiguous()
