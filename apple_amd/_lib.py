@@ -152,6 +152,10 @@ def _declare(lib):
         c_vp, ctypes.c_int, ctypes.c_uint, c_vp, c_vp, c_vp, c_vp, c_dp, c_vp, c_vp, ctypes.c_int,
         ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), c_vp,
     ]
+    lib.apl_bench_elem_graph.argtypes = [
+        c_vp, ctypes.c_int, ctypes.c_uint, c_vp, c_vp, c_vp, c_vp, c_dp, c_vp, c_vp, ctypes.c_int,
+        ctypes.POINTER(ctypes.c_double), c_vp,
+    ]
     if hasattr(lib, "apl_solver_create"):
         lib.apl_solver_create.argtypes = [c_vp, ctypes.c_int, ctypes.POINTER(c_vp)]
         lib.apl_solver_destroy.argtypes = [c_vp]

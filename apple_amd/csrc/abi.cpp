@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -46,6 +47,7 @@ static void free_mesh(apl_mesh* m) {
                   d.law_blk[1],   d.law_blk[2]};
   for (void* p : ptrs)
     if (p) drop_hip_error(hipFree(p));
+  if (m->bench_exec) drop_hip_error(hipGraphExecDestroy(m->bench_exec));
   delete m;
 }
 
@@ -506,6 +508,46 @@ int apl_elem_mixed_activation(apl_mesh* m, const void* mu, const void* lam, cons
                      accumulate ? 1 : 0, 0, (hipStream_t)stream, 0, dact);
 }
 
+}  // extern "C"
+
+namespace apl {
+
+// Timing events of the measurement entry points: created on a device's
+// first call and kept for the process (concurrent calls on one device race).
+static hipEvent_t* bench_events(int device) {
+  static std::mutex mu;
+  static hipEvent_t cache[64][4];
+  if (device < 0 || device >= 64) {
+    set_error("device index out of range");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(mu);
+  for (int k = 0; k < 4; ++k)
+    if (cache[device][k] == nullptr && hipEventCreate(&cache[device][k]) != hipSuccess) {
+      (void)hipGetLastError();
+      set_error("hipEventCreate failed");
+      return nullptr;
+    }
+  return cache[device];
+}
+
+// Spin until `ev` has completed (a blocking wait adds its wake-up latency).
+static int spin_wait(hipEvent_t ev) {
+  hipError_t q;
+  while ((q = hipEventQuery(ev)) == hipErrorNotReady) {
+  }
+  if (q != hipSuccess) {
+    (void)hipGetLastError();
+    set_error(std::string("hipEventQuery: ") + hipGetErrorString(q));
+    return -1;
+  }
+  return 0;
+}
+
+}  // namespace apl
+
+extern "C" {
+
 int apl_bench_elem_pass(apl_mesh* m, int material, unsigned what, const void* mu, const void* lam, const void* u,
                         const void* p, double* energy, void* out_a, void* out_b, int iters, double* ms_kernel,
                         double* ms_pass, void* stream) {
@@ -523,35 +565,101 @@ int apl_bench_elem_pass(apl_mesh* m, int material, unsigned what, const void* mu
   hipStream_t s = (hipStream_t)stream;
   // ms_kernel (if non-NULL): iters element launches back to back between two
   // events (the kernel's duration plus the launch gap); ms_pass (if
-  // non-NULL): iters full passes (element kernel + finish kernel) the same way
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-  for (auto& e : ev) APL_CHECK_HIP(hipEventCreate(&e));
+  // non-NULL): iters full passes (element kernel + finish kernel) the same way.
+  // The events are created once per device and kept, so that a timed call
+  // holds the launches and little else.
+  hipEvent_t* ev = bench_events(m->device);
+  if (ev == nullptr) return -1;
   int rc = 0;
-  APL_CHECK_HIP(hipEventRecord(ev[0], s));
-  if (ms_kernel != nullptr)
+  // only the events a caller reads are recorded (each is a marker packet on
+  // the queue); the host then spins on the last one rather than sleeping in
+  // a blocking wait, so the call returns as soon as the work is done
+  if (ms_kernel != nullptr) {
+    APL_CHECK_HIP(hipEventRecord(ev[0], s));
     for (int it = 0; it < iters && rc == 0; ++it)
       rc = launch_elem(m, material, w, mu, lam, u, p, nullptr, nullptr, out_a, out_b, 0, 0, s);
-  APL_CHECK_HIP(hipEventRecord(ev[1], s));
-  APL_CHECK_HIP(hipEventRecord(ev[2], s));
-  if (ms_pass != nullptr)
+    APL_CHECK_HIP(hipEventRecord(ev[1], s));
+  }
+  if (ms_pass != nullptr && rc == 0) {
+    APL_CHECK_HIP(hipEventRecord(ev[2], s));
     for (int it = 0; it < iters && rc == 0; ++it) {
       rc = launch_elem(m, material, w, mu, lam, u, p, nullptr, nullptr, out_a, out_b, 0, 0, s);
       if (rc == 0)
         rc = launch_finish(m, w, out_a, out_b, (w & W_E) ? energy : nullptr, (w & W_HQ) ? energy : nullptr, 0,
                            nullptr, s);
     }
-  APL_CHECK_HIP(hipEventRecord(ev[3], s));
+    APL_CHECK_HIP(hipEventRecord(ev[3], s));
+  }
   if (rc == 0) {
-    APL_CHECK_HIP(hipEventSynchronize(ev[3]));
+    if (spin_wait(ms_pass != nullptr ? ev[3] : ev[1])) return -1;
     float a = 0, b = 0;
-    APL_CHECK_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
-    APL_CHECK_HIP(hipEventElapsedTime(&b, ev[2], ev[3]));
+    if (ms_kernel != nullptr) APL_CHECK_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
+    if (ms_pass != nullptr) APL_CHECK_HIP(hipEventElapsedTime(&b, ev[2], ev[3]));
     if (ms_kernel != nullptr) *ms_kernel = a / iters;
     if (ms_pass != nullptr) *ms_pass = b / iters;
   }
-  for (auto& e : ev)
-    if (e) drop_hip_error(hipEventDestroy(e));
   return rc;
+}
+
+int apl_bench_elem_graph(apl_mesh* m, int material, unsigned what, const void* mu, const void* lam, const void* u,
+                         const void* p, double* energy, void* out_a, void* out_b, int iters, double* ms_pass,
+                         void* stream) {
+  if (m == nullptr || iters < 1 || ms_pass == nullptr) {
+    set_error("invalid argument");
+    return -1;
+  }
+  APL_CHECK_HIP(hipSetDevice(m->device));
+  (void)hipGetLastError();  // drop errors left by unrelated earlier HIP calls
+  const unsigned w = to_internal(what);
+  if (w != (W_E | W_G | W_H) && w != W_HQ && w != W_HP) {
+    set_error("bench pass must be ENERGY|GRAD|HESS_DIAG, HESS_QUAD or HESS_PROD");
+    return -1;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  hipEvent_t* ev = bench_events(m->device);
+  if (ev == nullptr) return -1;
+  const uintptr_t key[12] = {(uintptr_t)material, (uintptr_t)w, (uintptr_t)mu, (uintptr_t)lam, (uintptr_t)u,
+                             (uintptr_t)p, (uintptr_t)energy, (uintptr_t)out_a, (uintptr_t)out_b,
+                             (uintptr_t)iters, (uintptr_t)m->grid_pct, (uintptr_t)m->activation};
+  if (m->bench_exec == nullptr || std::memcmp(key, m->bench_key, sizeof key) != 0) {
+    // capture `iters` whole passes on a private stream (the caller's may be
+    // the null stream, which cannot be captured)
+    if (m->bench_exec) {
+      drop_hip_error(hipGraphExecDestroy(m->bench_exec));
+      m->bench_exec = nullptr;
+    }
+    hipStream_t cs = nullptr;
+    APL_CHECK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipGraph_t g = nullptr;
+    int rc = 0;
+    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) rc = -1;
+    for (int it = 0; it < iters && rc == 0; ++it) {
+      rc = launch_elem(m, material, w, mu, lam, u, p, nullptr, nullptr, out_a, out_b, 0, 0, cs);
+      if (rc == 0)
+        rc = launch_finish(m, w, out_a, out_b, (w & W_E) ? energy : nullptr, (w & W_HQ) ? energy : nullptr, 0,
+                           nullptr, cs);
+    }
+    const hipError_t ec = hipStreamEndCapture(cs, &g);
+    if (rc == 0 && ec == hipSuccess && g != nullptr &&
+        hipGraphInstantiate(&m->bench_exec, g, nullptr, nullptr, 0) != hipSuccess)
+      m->bench_exec = nullptr;
+    if (g) drop_hip_error(hipGraphDestroy(g));
+    drop_hip_error(hipStreamDestroy(cs));
+    if (rc != 0 || ec != hipSuccess || m->bench_exec == nullptr) {
+      (void)hipGetLastError();
+      if (rc == 0) set_error("capture of the benchmark graph failed");
+      return -1;
+    }
+    std::memcpy(m->bench_key, key, sizeof key);
+  }
+  APL_CHECK_HIP(hipEventRecord(ev[2], s));
+  APL_CHECK_HIP(hipGraphLaunch(m->bench_exec, s));
+  APL_CHECK_HIP(hipEventRecord(ev[3], s));
+  if (spin_wait(ev[3])) return -1;
+  float b = 0;
+  APL_CHECK_HIP(hipEventElapsedTime(&b, ev[2], ev[3]));
+  *ms_pass = b / iters;
+  return 0;
 }
 
 }  // extern "C"

@@ -74,6 +74,10 @@ struct apl_mesh {
   // vertex set, in segments of one material law each
   int model = 0;
   int64_t law_nblk[3] = {0, 0, 0};  // blocks per law (MAT_SNH, MAT_ARAP, MAT_MUSCLE)
+  // measurement support: the last graph of back-to-back passes that
+  // apl_bench_elem_graph captured on this mesh, and the arguments it holds
+  hipGraphExec_t bench_exec = nullptr;
+  uintptr_t bench_key[12] = {};
 };
 
 namespace apl {

@@ -234,10 +234,21 @@ int apl_ext_force_eval(apl_mesh* mesh, int64_t n, const int32_t* indices, const 
  * between two HIP events on `stream`; ms_pass (host, may be NULL): mean
  * time of `iters` whole passes (element kernel + vertex and energy sums)
  * measured the same way.  what: ENERGY|GRAD|HESS_DIAG (pass A), HESS_QUAD or
- * HESS_PROD. */
+ * HESS_PROD.  The timing events are created on a device's first call and kept
+ * for the process: concurrent calls on one device race. */
 int apl_bench_elem_pass(apl_mesh* mesh, int material, unsigned what, const void* mu, const void* lam, const void* u,
                         const void* p, double* energy, void* out_a, void* out_b, int iters, double* ms_kernel,
                         double* ms_pass, void* stream);
+
+/* Measurement support (no reference counterpart): the same `iters` whole
+ * passes as apl_bench_elem_pass, captured once into a HIP graph and replayed
+ * as one launch between two events on `stream` (ms_pass: the mean pass
+ * time).  The first call for a set of arguments captures (and runs) the
+ * graph; later calls with the same arguments only replay it.  The mesh keeps
+ * the last graph until it is destroyed. */
+int apl_bench_elem_graph(apl_mesh* mesh, int material, unsigned what, const void* mu, const void* lam, const void* u,
+                         const void* p, double* energy, void* out_a, void* out_b, int iters, double* ms_pass,
+                         void* stream);
 
 /* ---------------------------------------------------------------- solvers */
 

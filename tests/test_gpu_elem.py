@@ -145,3 +145,41 @@ def test_empty_mesh():
     dm.eval(0, mu, mu, u, energy=e, grad=g)
     torch.cuda.synchronize()
     assert e.item() == 0.0 and torch.count_nonzero(g).item() == 0
+
+
+def test_graph_replayed_passes_equal_eager_passes():
+    """The measurement entry point that replays K whole passes from a HIP
+    graph (apl_bench_elem_graph) writes what K eager passes write, bit for
+    bit, on a replay after its capture too."""
+    import ctypes
+
+    from apple_amd import _lib
+    from apple_amd.runtime import current_stream, ptr
+
+    m, region, u64, _ = snh_case(8, seed=3)
+    pot = fem.StableNeoHookean.from_region(region, dtype=torch.float32, device="cuda")
+    u = torch.as_tensor(u64, dtype=torch.float32, device="cuda")
+    dm = pot.mesh
+    what = _lib.APL_ENERGY | _lib.APL_GRAD | _lib.APL_HESS_DIAG
+    outs = []
+    for fn in ("apl_bench_elem_pass", "apl_bench_elem_graph", "apl_bench_elem_graph"):
+        a = torch.full((dm.n_verts, 3), 7.0, dtype=torch.float32, device="cuda")
+        b = torch.full_like(a, 7.0)
+        e = torch.full((1,), 7.0, dtype=torch.float64, device="cuda")
+        ms = ctypes.c_double()
+        args = (dm._handle, int(pot.material), what, ptr(pot.materials["mu"]), ptr(pot.materials["lambda"]), ptr(u),
+                None, ptr(e), ptr(a), ptr(b), 3)
+        if fn == "apl_bench_elem_pass":
+            _lib.check(getattr(_lib.lib(), fn)(*args, None, ctypes.byref(ms), current_stream(dm.device)))
+        else:
+            _lib.check(getattr(_lib.lib(), fn)(*args, ctypes.byref(ms), current_stream(dm.device)))
+        torch.cuda.synchronize()
+        assert ms.value > 0.0
+        outs.append((e.cpu().numpy(), a.cpu().numpy(), b.cpu().numpy()))
+    for got in outs[1:]:
+        for x, y in zip(outs[0], got):
+            assert np.array_equal(x, y)
+    ref = oracle_pot(region, "snh", m.cell_data["mu"], m.cell_data["lambda"], dtype_round=np.float32)
+    u_r = u64.astype(np.float32).astype(np.float64)
+    assert abs(outs[0][0][0] - ref.fun(u_r)) <= 1e-5 * abs(ref.fun(u_r))
+    assert rel(outs[0][1], ref.grad(u_r)) <= 1e-4
