@@ -273,7 +273,7 @@ class Batch:
         if redo and refine_fp64 != "checked":
             # a batch of just these instances, from their FP32 positions
             parts = self.split(x)
-            sub = [dataclasses.replace(self.instances[i], x0=parts[i].double().cpu().numpy()) for i in redo]
+            sub = [dataclasses.replace(self._current_instance(i), x0=parts[i].double().cpu().numpy()) for i in redo]
             bs = Batch(sub, self.material, dtype=torch.float64, device=self.device)
             atol = config.rtol * min(stats[i].grad_norm_first for i in redo)
             cfg64 = dataclasses.replace(config, atol=max(config.atol, atol), wolfe_eps=None)
@@ -298,6 +298,49 @@ class Batch:
             self.refine_stats = [st64[i] for i in redo]
         self.refined = redo
         return x, stats
+
+    def _current_instance(self, i: int) -> Instance:
+        """Instance i with the batch's current materials in FP64: its own
+        values where the batch still holds their rounding, else the current
+        values widened (adjoint.equilibrium and Inverse replace them)."""
+        inst = self.instances[i]
+        t0, t1 = int(self.tet_off[i]), int(self.tet_off[i + 1])
+        np_dt = np.float32 if self.dtype == torch.float32 else np.float64
+
+        def current(key, orig):
+            c64 = self.potential.materials[key][t0:t1].double().cpu().numpy().reshape(orig.shape)
+            return orig if np.array_equal(c64, orig.astype(np_dt).astype(np.float64)) else c64
+
+        if inst.parts is None:
+            upd = {"mu": current("mu", np.asarray(inst.mu, np.float64).reshape(-1))}
+            if "lambda" in self.potential.materials and inst.lam is not None:
+                upd["lam"] = current("lambda", np.asarray(inst.lam, np.float64).reshape(-1))
+            if "activation" in self.potential.materials and inst.activation is not None:
+                upd["activation"] = current("activation", np.asarray(inst.activation, np.float64).reshape(-1, 6))
+            return dataclasses.replace(inst, **upd)
+        # a model instance: the materials of its parts, in [part] tet order
+        sizes = [p.cells.shape[0] for p in inst.parts]
+        cuts = np.cumsum(sizes)[:-1]
+        mats = self.potential.materials
+        mu = current("mu", np.concatenate([np.asarray(p.mu, np.float64).reshape(-1) for p in inst.parts]))
+        lam = act = None
+        if "lambda" in mats:
+            lam = current("lambda", np.concatenate([
+                np.zeros(n) if p.lam is None or p.material == _lib.APL_ARAP else np.asarray(p.lam, np.float64).reshape(n)
+                for p, n in zip(inst.parts, sizes)]))
+        if "activation" in mats:
+            act = current("activation", np.concatenate([
+                np.zeros((n, 6)) if p.activation is None else np.asarray(p.activation, np.float64).reshape(n, 6)
+                for p, n in zip(inst.parts, sizes)]))
+        parts = []
+        for k, (p, m_k) in enumerate(zip(inst.parts, np.split(mu, cuts))):
+            upd = {"mu": m_k}
+            if lam is not None and p.material != _lib.APL_ARAP:
+                upd["lam"] = np.split(lam, cuts)[k]
+            if act is not None and p.material == _lib.APL_SNH_MUSCLE:
+                upd["activation"] = np.split(act, cuts)[k]
+            parts.append(dataclasses.replace(p, **upd))
+        return dataclasses.replace(inst, parts=parts)
 
     def _fp64_copy(self) -> "Batch":
         """FP64 copy of the batch (FP64 element data from the instances' regions),

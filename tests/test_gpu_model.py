@@ -340,3 +340,40 @@ def test_batch_of_models_equals_each_model_alone():
     rhs = (b.x0 - x).contiguous()
     p, cs = b.adjoint_solve(x, rhs, rtol=1e-5)
     assert all(s.result == Result.PRIMARY_SUCCESS for s in cs)
+
+
+def test_batch_of_models_refine_sees_replaced_materials():
+    """A batch of models whose materials were replaced after construction
+    (as Inverse does): the FP64 re-solve of refine_fp64="stagnated" builds its
+    instances from the replaced values, part by part in the model's [part]
+    tet order, and keeps the original FP64 values where the batch still
+    holds their rounding."""
+    print("[model] batch of models, refine with replaced materials", flush=True)
+    from apple_amd import _lib, batch
+
+    models = [_smas_builder(n, torch.float32, seed=40 + n)[0].finalize() for n in (3, 4)]
+    b = batch.Batch.from_models(models, dtype=torch.float32, device="cuda")
+    mats = b.potential.materials
+    mats["mu"][::3].mul_(1.25)
+    mats["activation"].add_(0.01)
+    for i, inst in enumerate(b.instances):
+        cur = b._current_instance(i)
+        t0 = int(b.tet_off[i])
+        off = t0
+        for p0, p1 in zip(inst.parts, cur.parts):
+            n = p0.cells.shape[0]
+            sl = slice(off, off + n)
+            mu_now = mats["mu"][sl].double().cpu().numpy()
+            keep = mu_now == np.asarray(p0.mu, np.float64).astype(np.float32).astype(np.float64)
+            assert not keep.all()
+            np.testing.assert_array_equal(p1.mu, mu_now if not keep.all() else p0.mu)
+            np.testing.assert_array_equal(p1.lam, p0.lam)  # unchanged: the original FP64 values
+            if p0.material == _lib.APL_SNH_MUSCLE:
+                np.testing.assert_array_equal(p1.activation, mats["activation"][sl].double().cpu().numpy())
+            else:
+                assert p1.activation is p0.activation
+            off += n
+        assert off == int(b.tet_off[i + 1])
+    cfg = PncgConfig(rtol=1e-7, max_steps=3)
+    x, st = b.solve(config=cfg, refine_fp64="stagnated")
+    assert b.refined == [0, 1] and torch.isfinite(x).all()

@@ -352,6 +352,38 @@ def test_batch_refine_fp64_follows_replaced_materials():
         assert float(torch.linalg.norm(g[sl])) <= 2 * cfg.rtol * st32[i].grad_norm_first, i
 
 
+def test_batch_stagnated_refine_uses_replaced_materials():
+    """refine_fp64="stagnated" re-solves the unfinished instances in FP64 with
+    the batch's current materials (here mu replaced by 1.5 mu after
+    construction), not the instances' original ones: its positions equal an
+    FP64 batch built from the replaced values, bit for bit."""
+    import dataclasses
+
+    print("[solver] batch stagnated refine after a material update", flush=True)
+    insts = []
+    for k, n in enumerate((3, 4)):
+        m, region, mu, lam, f, fm, x0 = _gravity_case(n, "snh", seed=70 + k)
+        insts.append(batch.Instance(region, mu, lam, f, fm, x0))
+    b32 = batch.Batch(insts, _lib_snh(), dtype=torch.float32, device="cuda")
+    b32.potential.materials["mu"].mul_(1.5)
+    cfg = PncgConfig(rtol=1e-7, max_steps=3)  # every instance ends MAX_STEPS_REACHED
+    x32, st32 = b32.solve(config=cfg)
+    assert all(st.result == Result.MAX_STEPS_REACHED for st in st32)
+    xr, _ = b32.solve(config=cfg, refine_fp64="stagnated")
+    assert b32.refined == [0, 1]
+    cur = b32.potential.materials["mu"].double().cpu().numpy()
+    ref_insts = []
+    for i, (inst, part) in enumerate(zip(insts, b32.split(x32))):
+        mu_i = cur[int(b32.tet_off[i]):int(b32.tet_off[i + 1])]
+        assert not np.array_equal(mu_i, inst.mu)
+        np.testing.assert_array_equal(b32._current_instance(i).mu, mu_i)
+        ref_insts.append(dataclasses.replace(inst, mu=mu_i, x0=part.double().cpu().numpy()))
+    b64 = batch.Batch(ref_insts, _lib_snh(), dtype=torch.float64, device="cuda")
+    atol = cfg.rtol * min(st.grad_norm_first for st in st32)
+    x64, _ = b64.solve(config=dataclasses.replace(cfg, atol=atol, wolfe_eps=None))
+    assert torch.equal(xr, x64.to(torch.float32))
+
+
 @pytest.mark.parametrize("material", ["snh", "arap", "muscle"])
 def test_trajectory_matches_oracle_pncg_step_by_step(material):
     """FP64: the device PNCG after k accepted steps equals the oracle's PNCG
