@@ -209,3 +209,32 @@ def test_lumped_volume_equals_add_at(dtype, chunks):
     f = grav.dense().double().cpu().numpy()
     np.testing.assert_allclose(f[:, 2], -9.8 * ref, rtol=1e-6 if dtype == torch.float32 else 1e-13)
     assert not f[:, :2].any()
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_instance_without_tets_equals_host_and_passes_run(dtype):
+    """A batch whose middle instance has vertices but no tets: both
+    partitions agree, that instance owns vertex blocks but no element block,
+    and an element pass gives it zero energy and a gradient of zero."""
+    print(f"[partition] instance without tets {dtype}", flush=True)
+    a, b = cube(2), cube(3)
+    n_empty = 5
+    cells = np.concatenate([a.cells, b.cells + a.n_points + n_empty]).astype(np.int32)
+    V = a.n_points + n_empty + b.n_points
+    toff = np.array([0, a.n_cells, a.n_cells, a.n_cells + b.n_cells])
+    voff = np.array([0, a.n_points, a.n_points + n_empty, V])
+    dev, host = _host_and_device(cells, V, dtype=dtype, inst=(toff, voff))
+    _assert_same(dev, host)
+    lay = dev.layout()
+    assert 1 not in lay["blk_inst"]
+    assert lay["inst_blk_off"][1] == lay["inst_blk_off"][2]
+    assert lay["inst_vblk_off"][2] > lay["inst_vblk_off"][1]
+    u = torch.as_tensor(0.01 * np.random.default_rng(1).standard_normal((V, 3)), dtype=dtype, device="cuda")
+    mu = torch.full((cells.shape[0],), 1e3, dtype=dtype, device="cuda")
+    lam = 10 * mu
+    e = torch.zeros(3, dtype=torch.float64, device="cuda")
+    g = torch.full((V, 3), 5.0, dtype=dtype, device="cuda")
+    dev.eval(_lib.APL_SNH, mu, lam, u, energy=e, grad=g)
+    torch.cuda.synchronize()
+    assert e[1].item() == 0.0 and e[0].item() > 0.0 and e[2].item() > 0.0
+    assert torch.count_nonzero(g[a.n_points:a.n_points + n_empty]).item() == 0
