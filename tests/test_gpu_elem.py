@@ -183,3 +183,34 @@ def test_graph_replayed_passes_equal_eager_passes():
     u_r = u64.astype(np.float32).astype(np.float64)
     assert abs(outs[0][0][0] - ref.fun(u_r)) <= 1e-5 * abs(ref.fun(u_r))
     assert rel(outs[0][1], ref.grad(u_r)) <= 1e-4
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("material", ["snh", "arap"])
+def test_inverted_and_flat_tets_match_oracle(dtype, material):
+    """A 6^3 cube displaced by noise of half a cell: 22 % of the tets are
+    inverted (det F down to -4.5) and 0.7 % nearly flat (|det F| < 1/64, the
+    ARAP kernels' FP64 finish); every output of the element passes still
+    matches the oracle, the ARAP ones through the rotation-variant SVD
+    convention (the sign goes to the smallest singular value)."""
+    m, region, u64, v64 = snh_case(6, seed=11, noise=0.08)
+    npdt = np.float32 if dtype == torch.float32 else np.float64
+    u_r = u64.astype(npdt).astype(np.float64)
+    v_r = v64.astype(npdt).astype(np.float64)
+    ref = oracle_pot(region, material, m.cell_data["mu"], m.cell_data["lambda"], dtype_round=npdt)
+    det = np.linalg.det(ref.F(u_r))
+    print(f"[elem] inverted {material} {dtype}: {np.mean(det < 0):.1%} inverted, "
+          f"{np.mean(np.abs(det) < 1 / 64):.1%} flat", flush=True)
+    assert np.mean(det < 0) > 0.2
+    cls = fem.StableNeoHookean if material == "snh" else fem.Arap
+    pot = cls.from_region(region, dtype=dtype, device="cuda")
+    got = _run(pot, torch.as_tensor(u_r, dtype=dtype, device="cuda"), torch.as_tensor(v_r, dtype=dtype, device="cuda"))
+    tol = TOL[dtype]
+    e_ref = ref.fun(u_r)
+    assert abs(got["e"] - e_ref) <= tol["e"] * abs(e_ref)
+    for k, r in (("g", ref.grad(u_r)), ("h", ref.hess_diag(u_r)), ("hp", ref.hess_prod(u_r, v_r))):
+        print(f"  {k}: rel {rel(got[k], r):.2e}, max-entry {rel_max(got[k], r):.2e}")
+        assert rel(got[k], r) <= tol["g"], k
+        assert rel_max(got[k], r) <= tol["g"], k
+    hq_ref = ref.hess_quad(u_r, v_r)
+    assert abs(got["hq"] - hq_ref) <= tol["g"] * abs(hq_ref)
