@@ -190,8 +190,8 @@ def test_graph_replayed_passes_equal_eager_passes():
 def test_inverted_and_flat_tets_match_oracle(dtype, material):
     """A 6^3 cube displaced by noise of half a cell: 22 % of the tets are
     inverted (det F down to -4.5) and 0.7 % nearly flat (|det F| < 1/64, the
-    ARAP kernels' FP64 finish); every output of the element passes still
-    matches the oracle, the ARAP ones through the rotation-variant SVD
+    ARAP kernels' FP64 finish); every output of the element passes (the
+    mixed material derivatives included) still matches the oracle, the ARAP ones through the rotation-variant SVD
     convention (the sign goes to the smallest singular value)."""
     m, region, u64, v64 = snh_case(6, seed=11, noise=0.08)
     npdt = np.float32 if dtype == torch.float32 else np.float64
@@ -214,3 +214,7 @@ def test_inverted_and_flat_tets_match_oracle(dtype, material):
         assert rel_max(got[k], r) <= tol["g"], k
     hq_ref = ref.hess_quad(u_r, v_r)
     assert abs(got["hq"] - hq_ref) <= tol["g"] * abs(hq_ref)
+    dmu_ref, dlam_ref = ref.mixed_cells(u_r, v_r)
+    assert rel(got["mu"], dmu_ref) <= tol["g"] and rel_max(got["mu"], dmu_ref) <= tol["g"]
+    if material == "snh":
+        assert rel(got["lambda"], dlam_ref) <= tol["g"] and rel_max(got["lambda"], dlam_ref) <= tol["g"]
