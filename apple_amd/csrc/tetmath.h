@@ -561,16 +561,6 @@ __host__ __device__ inline void jacobi_fast_sweeps(float (&a)[3][3], M3<float>& 
   }
 }
 
-__host__ __device__ inline Svd<float> svd_rv_e32(const M3<float>& G) {
-  float a[3][3];
-  ftf_minus_i_f64(G, a);
-  M3<float> V;
-  jacobi_fast_sweeps(a, V);
-  float e[3];
-  sort_eig(a, V, e);
-  return qr_sigma(G, V, e);
-}
-
 // The ARAP kernels' SVD.  Every element gets the FP32 basis (F^T F - I
 // formed in FP64, FP32 Jacobi sweeps); elements with det F > 1/64 (large
 // rotations and large strains included) finish in FP32 (sort, QR), the
