@@ -286,6 +286,8 @@ def main():
                     help="only 20 pass-C (hess_prod, SNH) launches and 3 calibration copies, for rocprofv3")
     ap.add_argument("--profile-batched", action="store_true",
                     help="only 20 pass-A launches over the 256-cube batch (n ~ U{6..32}) and 3 calibration copies")
+    ap.add_argument("--profile-material", choices=("snh", "arap", "muscle"), default="snh",
+                    help="material of the --profile-solve run")
     ap.add_argument("--profile-solve", type=int, default=0, metavar="STEPS",
                     help="only run a PNCG solve of STEPS iterations (for rocprofv3 kernel traces)")
     args = ap.parse_args()
@@ -385,6 +387,9 @@ def main():
     if args.profile_solve:
         from apple_amd.solver import DeviceSolver, PncgConfig, mask_bits
 
+        if args.profile_material != "snh":
+            prob = make_cube_problem(n, args.profile_material, dtype, dev, chunks=args.chunks)
+            pot = prob["pot"]
         solver = DeviceSolver(pot)
         mask = mask_bits(prob["fixed"], dev)
         x = prob["x0"].clone()
