@@ -143,6 +143,45 @@ def test_batched_equals_one_by_one():
 
 
 @pytest.mark.parametrize("material", ["snh", "arap"])
+def test_batched_instances_match_the_oracle(material):
+    """Three cubes of different sizes and materials in one FP64 batch, each
+    checked by the oracle on its own: SNH against the oracle's Newton
+    equilibrium from the same start (not from the device's answer),
+    positions within 1e-9 of the cube's diagonal; ARAP, whose exact Hessian
+    is indefinite far from equilibrium (Newton from the noisy start wanders
+    off), by the oracle's gradient at the device's answer, below 1e-9 of the
+    oracle's gradient at the start."""
+    from apple_amd import _lib
+
+    print(f"[solver] batch vs oracle {material}", flush=True)
+    insts, cases = [], []
+    for k, n in enumerate((3, 5, 4)):
+        m, region, mu, lam, f, fm, x0 = _gravity_case(n, material, seed=30 + k)
+        insts.append(batch.Instance(region, mu, lam if material == "snh" else None, f, fm, x0))
+        cases.append((region, mu, lam, f, fm, x0))
+    law = _lib.APL_SNH if material == "snh" else _lib.APL_ARAP
+    b = batch.Batch(insts, law, dtype=torch.float64, device="cuda")
+    xb, stb = b.solve(config=PncgConfig(rtol=1e-11, max_steps=50000))
+    parts = b.split(xb)
+    for i, (region, mu, lam, f, fm, x0) in enumerate(cases):
+        assert stb[i].result == Result.PRIMARY_SUCCESS, (i, stb[i])
+        x = parts[i].cpu().numpy()
+        if material == "snh":
+            xo, gn = _oracle_equilibrium(region, material, mu, lam, f, fm, x0)
+            err = np.abs(x - xo).max() / np.sqrt(3.0)
+            print(f"  instance {i}: |x - x_newton| / diag {err:.2e} (oracle |g| {gn:.1e})")
+            assert err <= 1e-9, (i, err)
+        else:
+            pot = of.FemPotential(of.ARAP, region.cells, region.dhdX, region.dV, mu, None)
+            dof = of.DofMap(fm, np.zeros_like(f))
+            prob = op.Problem(of.Model([pot, of.ExternalForce(np.arange(f.shape[0]), f)]), dof)
+            g0 = np.linalg.norm(prob.grad(dof.to_free(x0)))
+            g = np.linalg.norm(prob.grad(dof.to_free(x)))
+            print(f"  instance {i}: oracle |g| at the device's answer {g:.2e}, at the start {g0:.2e}")
+            assert g <= 1e-9 * g0, (i, g, g0)
+
+
+@pytest.mark.parametrize("material", ["snh", "arap"])
 def test_cg_matches_sparse_direct_solve(material):
     m, region, mu, lam, f, fm, x0 = _gravity_case(4, material, seed=5)
     m.cell_data["mu"] = mu
