@@ -101,7 +101,12 @@ def test_gravity_equilibrium_matches_oracle(material, dtype):
     assert st.result == Result.PRIMARY_SUCCESS, st
     assert st.relative_grad_norm <= 1e-6
     xg = x.double().cpu().numpy()
-    x_ref, gn = _oracle_equilibrium(region, material, mu, lam, f, fm, xg)
+    # SNH: the oracle's Newton from the same start, so a solve that lands on
+    # another equilibrium would fail; ARAP's exact Hessian is indefinite far
+    # from equilibrium and Newton from the noisy start does not converge, so
+    # it polishes the device's answer (the batched test checks ARAP answers
+    # by the oracle's gradient instead)
+    x_ref, gn = _oracle_equilibrium(region, material, mu, lam, f, fm, x0 if material == "snh" else xg)
     assert gn <= 1e-6 * max(st.grad_norm_first, 1.0)
     diag = np.sqrt(3.0)
     err = np.abs(xg - x_ref).max() / diag
