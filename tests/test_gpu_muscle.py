@@ -86,6 +86,36 @@ def test_muscle_element_pass_matches_oracle(dtype, n, chunks):
     assert rel_max(got["activation"], ref.mixed_activation_cells(u_r, v_r)) <= tol["g"]
 
 
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_muscle_inverted_tets_match_oracle(dtype):
+    """Displacement noise of half a cell on a 6^3 cube: over a fifth of the
+    activated deformation gradients G = F A are inverted; every element
+    output, the three mixed derivatives included, still matches the oracle."""
+    m, region, u64, v64, act = _muscle_case(6, seed=11, noise=0.08)
+    npdt = np.float32 if dtype == torch.float32 else np.float64
+    r = lambda a: np.asarray(a, np.float64).astype(npdt).astype(np.float64)
+    u_r, v_r = r(u64), r(v64)
+    ref = of.FemPotential(of.MUSCLE, region.cells, region.dhdX, region.dV, r(m.cell_data["mu"]),
+                          r(m.cell_data["lambda"]), r(act))
+    det = np.linalg.det(ref.F(u_r) @ of.activation_matrix(r(act)))
+    print(f"[muscle elem] inverted {dtype}: {np.mean(det < 0):.1%} of det G < 0", flush=True)
+    assert np.mean(det < 0) > 0.2
+    pot = fem.StableNeoHookeanMuscle.from_region(region, dtype=dtype, device="cuda")
+    got = _outputs(pot, torch.as_tensor(u_r, dtype=dtype, device="cuda"), torch.as_tensor(v_r, dtype=dtype, device="cuda"))
+    tol = TOL[dtype]
+    e_ref = ref.fun(u_r)
+    assert abs(got["e"] - e_ref) <= tol["e"] * abs(e_ref)
+    dmu_ref, dlam_ref = ref.mixed_cells(u_r, v_r)
+    refs = dict(g=ref.grad(u_r), h=ref.hess_diag(u_r), hp=ref.hess_prod(u_r, v_r), mu=dmu_ref,
+                **{"lambda": dlam_ref}, activation=ref.mixed_activation_cells(u_r, v_r))
+    for k, want in refs.items():
+        print(f"  {k}: rel {rel(got[k], want):.2e}, max-entry {rel_max(got[k], want):.2e}")
+        assert rel(got[k], want) <= tol["g"], k
+        assert rel_max(got[k], want) <= tol["g"], k
+    hq_ref = ref.hess_quad(u_r, v_r)
+    assert abs(got["hq"] - hq_ref) <= tol["g"] * abs(hq_ref)
+
+
 def test_zero_activation_is_bitwise_snh():
     m, region, u64, v64, _ = _muscle_case(6, seed=2)
     region.cell_data["activation"] = np.zeros((m.n_cells, 6))
