@@ -1,6 +1,6 @@
 """CPU oracle for the FEM elasticity hot path (test infrastructure only).
 
-This package is the correctness checker, never the product: only ``tests/``,
+This package is the reference for correctness, never the product: only ``tests/``,
 ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may
 import it, and only to check or to time the CPU restatement beside the GPU
 path.  Nothing in ``apple_amd`` imports it.
