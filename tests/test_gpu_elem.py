@@ -185,6 +185,64 @@ def test_graph_replayed_passes_equal_eager_passes():
     assert rel(outs[0][1], ref.grad(u_r)) <= 1e-4
 
 
+@pytest.mark.parametrize("material", ["snh", "arap"])
+def test_graph_replay_recaptures_when_the_pass_changes(material):
+    """One mesh, the graph entry point called for pass A, the quadratic form,
+    the Hessian-vector product and pass A again: each call captures the new
+    pass (its key changed) and writes what the eager passes write, bit for
+    bit, and what the oracle gives."""
+    import ctypes
+
+    from apple_amd import _lib
+    from apple_amd.runtime import current_stream, ptr
+
+    m, region, u64, v64 = snh_case(7, seed=5)
+    cls = fem.StableNeoHookean if material == "snh" else fem.Arap
+    pot = cls.from_region(region, dtype=torch.float32, device="cuda")
+    u = torch.as_tensor(u64, dtype=torch.float32, device="cuda")
+    v = torch.as_tensor(v64, dtype=torch.float32, device="cuda")
+    dm = pot.mesh
+    lam = ptr(pot.materials["lambda"]) if material == "snh" else None
+    A = _lib.APL_ENERGY | _lib.APL_GRAD | _lib.APL_HESS_DIAG
+    ref = oracle_pot(region, material, m.cell_data["mu"], m.cell_data["lambda"], dtype_round=np.float32)
+    u_r = u64.astype(np.float32).astype(np.float64)
+    v_r = v64.astype(np.float32).astype(np.float64)
+
+    def run(fn, what):
+        a = torch.full((dm.n_verts, 3), 7.0, dtype=torch.float32, device="cuda")
+        b = torch.full_like(a, 7.0)
+        e = torch.full((1,), 7.0, dtype=torch.float64, device="cuda")
+        ms = ctypes.c_double()
+        p = ptr(v) if what != A else None
+        args = (dm._handle, int(pot.material), what, ptr(pot.materials["mu"]), lam, ptr(u), p, ptr(e), ptr(a),
+                ptr(b), 2)
+        if fn == "apl_bench_elem_pass":
+            _lib.check(getattr(_lib.lib(), fn)(*args, None, ctypes.byref(ms), current_stream(dm.device)))
+        else:
+            _lib.check(getattr(_lib.lib(), fn)(*args, ctypes.byref(ms), current_stream(dm.device)))
+        torch.cuda.synchronize()
+        assert ms.value > 0.0
+        return e.cpu().numpy(), a.cpu().numpy(), b.cpu().numpy()
+
+    for what in (A, _lib.APL_HESS_QUAD, _lib.APL_HESS_PROD, A):
+        print(f"[elem] graph replay {material} what={what}", flush=True)
+        eager = run("apl_bench_elem_pass", what)
+        graph = run("apl_bench_elem_graph", what)
+        e, a, b = graph
+        if what == A:
+            for x, y in zip(eager, graph):
+                assert np.array_equal(x, y)
+            assert abs(e[0] - ref.fun(u_r)) <= 1e-5 * abs(ref.fun(u_r))
+            assert rel(a, ref.grad(u_r)) <= 1e-4 and rel(b, ref.hess_diag(u_r)) <= 1e-4
+        elif what == _lib.APL_HESS_QUAD:
+            assert np.array_equal(eager[0], e)
+            hq = ref.hess_quad(u_r, v_r)
+            assert abs(e[0] - hq) <= 1e-4 * abs(hq)
+        else:
+            assert np.array_equal(eager[1], a)
+            assert rel(a, ref.hess_prod(u_r, v_r)) <= 1e-4
+
+
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
 @pytest.mark.parametrize("material", ["snh", "arap"])
 def test_inverted_and_flat_tets_match_oracle(dtype, material):
