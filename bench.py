@@ -658,7 +658,8 @@ def bench_batched(dev, dtype, n_meshes=256, lo=6, hi=32, seed_sizes=2, seed_mat=
             res[key]["plain_adjoint_converged"] = int(ok_adj)
         log(f"[bench] batched {key}: {t_batch:.3f} s vs plain {t_plain:.3f} s ({t_plain / t_batch:.2f}x)")
     res["workload"] = (f"{n_meshes} cubes, n ~ U{{{lo}..{hi}}} (seed {seed_sizes}), {b.n_tets} tets total, SNH FP32, "
-                       f"mu ~ U[1e3,1e5], nu ~ U[0.3,0.45] (seed {seed_mat}), PNCG to 1e-6; adjoint: PCG to "
+                       f"mu ~ U[1e3,1e5], nu ~ U[0.3,0.45] (seed {seed_mat}), PNCG to 1e-6 (STAGNATION after "
+                       f"{cfg.stall_steps} steps without progress near it); adjoint: PCG to "
                        f"{adj_rtol:g} on H p = -(x - x0), then the mixed pass for dL/d(mu, lambda); both paths "
                        "replay captured graphs (PNCG and PCG)")
     its = np.array([s.n_steps for s in stb])
